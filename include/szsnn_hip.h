@@ -1,0 +1,165 @@
+/* szsnn_hip.h -- C ABI of the MI355X (gfx950) SZSNN simulation core.
+ *
+ * This library replaces the reference's OpenCL device runtime and kernels:
+ *   szsnn1.1/src/Context.hpp:77-303       (setup, createKernel, createBuffer,
+ *                                          writeBuffer/readBuffer, setKernelArg,
+ *                                          enqueueKernel, finish)
+ *   szsnn1.1/src/Simulation.ipp:42-138    (kernel construction, device data upload)
+ *   szsnn1.1/src/Simulation.ipp:359-530   (per-step gather -> update -> scatter)
+ *   szsnn1.1/src/cl/ kernels             (gather, scatter, X_update)
+ *
+ * Conventions
+ *   - Plain C types only.  Every function returns 0 (SZSNN_OK) or a negative
+ *     SZSNN_ERR_* code; szsnn_last_error() gives the message.  The C++ host
+ *     layer turns a negative code into szsnn::SzsnnException, the reference's
+ *     error contract (szsnn1.1/src/utils.h:144-155).
+ *   - The library owns device memory; callers own host buffers.
+ *   - One HIP stream per context.  Calls on one context are not reentrant;
+ *     each call returns after its work is complete unless documented otherwise.
+ *   - Ids: neurons 0..N-1, input sources N..N+M-1 (the reference's input
+ *     partition follows the neuron partitions, NetworkGraph.hpp:153-188).
+ *   - Delays: an API delay k (integer steps, k < max_delay <= 64) delivers a
+ *     spike emitted in step c into the update of step c+k+1
+ *     (NetworkGraph.ipp:416 stores k+1; scatter_kernel.cl:154-169,204-213;
+ *     gather_kernel.cl:55-58,104).
+ *   - Model parameters are in per-step units (times in steps, capacitance
+ *     divided by dt): the reference kernels carry no dt.
+ */
+#ifndef SZSNN_HIP_H_
+#define SZSNN_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SZSNN_OK 0
+#define SZSNN_ERR_INVALID (-1)  /* bad argument or description */
+#define SZSNN_ERR_HIP (-2)      /* HIP runtime failure */
+#define SZSNN_ERR_OVERFLOW (-3) /* a device queue overflowed (spikes would be lost) */
+#define SZSNN_ERR_NOMEM (-4)    /* allocation failed */
+#define SZSNN_ERR_STATE (-5)    /* call not valid in the current state */
+
+#define SZSNN_MODEL_IF_CURR_EXP 0 /* params: v_rest cm tau_m tau_refrac tau_syn_E tau_syn_I i_offset v_reset v_thresh; vars: v ie ii fired */
+#define SZSNN_MODEL_IZHIKEVICH 1  /* params: a b c d; vars: v u */
+#define SZSNN_MODEL_IF_COND_EXP 2 /* params: the IF_curr_exp nine, e_rev_E e_rev_I; vars: v ge gi fired */
+
+#define SZSNN_SOURCE_NONE 0
+#define SZSNN_SOURCE_CONST_RATE 1 /* szsnn1.1/src/models/Const_rate_InputModel.hpp:11-32 */
+#define SZSNN_SOURCE_POISSON 2    /* spike iff draw(key, t, 3) < src_thresh */
+
+typedef struct szsnn_ctx szsnn_ctx;
+typedef struct szsnn_net szsnn_net;
+
+/* Network description handed to szsnn_net_upload.  Replaces the reference's
+ * NeuronData / SynapseData / SynapseIndex / GroupData initialisation
+ * (szsnn1.1/src/Simulation.ipp:108-118): the library builds a delay-bucketed
+ * CSR (rows keyed by (source id, delay)) from the synapse list. */
+typedef struct szsnn_net_desc {
+  int32_t model;              /* SZSNN_MODEL_* */
+  uint32_t n_neurons;         /* N > 0 */
+  uint32_t n_sources;         /* M >= 0 */
+  int32_t source_model;       /* SZSNN_SOURCE_* */
+  const float* params;        /* [n_params][N], SoA */
+  const float* vars;          /* [n_vars][N], SoA, initial state */
+  const float* src_rate;      /* [M] Const_rate rate (steps) or NULL */
+  const float* src_last;      /* [M] Const_rate initial `last` or NULL (0) */
+  const uint64_t* src_thresh; /* [M] Poisson thresholds or NULL */
+  uint64_t n_syn;
+  const uint32_t* syn_src;    /* [n_syn] source id (neuron or source) */
+  const uint32_t* syn_tgt;    /* [n_syn] target neuron id */
+  const float* syn_w;         /* [n_syn] weight; sign picks the E/I channel, 0 is inert */
+  const uint16_t* syn_delay;  /* [n_syn] API delay k < max_delay */
+  uint32_t max_delay;         /* D, 1..64 */
+  int32_t frac_bits;          /* F: weights are used as multiples of 2^-F (rounded to nearest) */
+  uint64_t seed;              /* random drive seed; key(id) = splitmix64 output #id of seed */
+  const float* noise_amp;     /* [N] Gaussian current amplitude or NULL */
+  uint32_t drive_k;           /* binomial drive: number of CDF thresholds (0 = none, <= 32) */
+  const uint64_t* drive_cdf;  /* [drive_k] events = #{j : draw(key, t, 2) >= cdf[j]} */
+  float drive_w;              /* weight of one drive event, E channel */
+  uint32_t queue_capacity;    /* due-queue entries per slot; 0 = worst case (never overflows) */
+} szsnn_net_desc;
+
+typedef struct szsnn_net_info {
+  uint32_t n_neurons, n_sources, max_delay, n_params, n_vars;
+  int32_t frac_bits;
+  uint64_t n_syn_stored;      /* nonzero-weight synapses in the CSR */
+  uint64_t n_rows;            /* (source, delay) rows = (N+M) * D */
+  uint64_t queue_capacity;    /* entries per due-queue slot */
+  uint64_t device_bytes;      /* device memory held by the network */
+  uint64_t raster_words;      /* u64 words per raster row = ceil((N+M)/64) */
+} szsnn_net_info;
+
+/* Counters since upload (or the last szsnn_reset_counters). */
+typedef struct szsnn_counters {
+  uint64_t cycle;         /* completed steps (szsnn1.1/src/SzsnnTimer.cpp:66-71) */
+  uint64_t events;        /* synaptic events delivered by gather */
+  uint64_t spikes;        /* spikes of neurons and sources */
+  uint64_t drive_events;  /* binomial drive events (not synaptic) */
+  uint64_t overflow;      /* nonzero if a due queue overflowed */
+} szsnn_counters;
+
+/* ---- context / device runtime (replaces Context.hpp:77-303) ---- */
+int szsnn_create(int device, szsnn_ctx** out);          /* Context::setup, Context.cpp:136-224 */
+int szsnn_destroy(szsnn_ctx* ctx);
+const char* szsnn_last_error(const szsnn_ctx* ctx);     /* ctx may be NULL: last global error */
+int szsnn_device_name(szsnn_ctx* ctx, char* buf, size_t len);
+int szsnn_alloc(szsnn_ctx* ctx, size_t bytes, void** dptr);   /* Context::createBuffer */
+int szsnn_free(szsnn_ctx* ctx, void* dptr);
+int szsnn_memcpy_h2d(szsnn_ctx* ctx, void* dst, const void* src, size_t bytes); /* writeBuffer(CL_TRUE) */
+int szsnn_memcpy_d2h(szsnn_ctx* ctx, void* dst, const void* src, size_t bytes); /* readBuffer(CL_TRUE) */
+int szsnn_synchronize(szsnn_ctx* ctx);                  /* getCommandQueue().finish() */
+void* szsnn_stream(szsnn_ctx* ctx);                     /* the context's hipStream_t */
+
+/* ---- network (replaces Simulation ctor + prepareGPUMemory, Simulation.ipp:42-138) ---- */
+int szsnn_net_upload(szsnn_ctx* ctx, const szsnn_net_desc* desc, szsnn_net** out);
+int szsnn_net_destroy(szsnn_net* net);
+int szsnn_net_get_info(szsnn_net* net, szsnn_net_info* out);
+
+/* One step: gather -> update(+input model) -> scatter (Simulation.ipp:471-530).
+ * forced_ids: ids to fire this step regardless of state (SpikeBuffer::setInput,
+ * SpikeBuffer.cpp:97-129; OR-ed into the spike condition, also during the
+ * refractory period).  out_bitmap: NULL or raster_words u64 receiving this
+ * step's dense spike bitmap (bit i = id i fired; SpikeBuffer::getOutput,
+ * SpikeBuffer.cpp:136-165). */
+int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, uint64_t* out_bitmap);
+
+/* Run n steps with no host input: HIP-graph-captured step loop.  When a raster
+ * buffer is set (szsnn_set_recording), steps with cycle in
+ * [rec_first, rec_first + rec_steps) write their bitmap to it.  Returns after
+ * completion; the due-queue overflow flag is checked at the end. */
+int szsnn_run(szsnn_net* net, uint64_t n_steps);
+/* Same, but launched asynchronously: returns once queued on the stream. */
+int szsnn_run_async(szsnn_net* net, uint64_t n_steps);
+
+/* Volley input: every step of the following szsnn_run / szsnn_step fires, in
+ * addition to the dynamics, the ids in [0, n_forced_prefix).  0 disables. */
+int szsnn_set_volley(szsnn_net* net, uint32_t n_forced_prefix);
+
+int szsnn_set_recording(szsnn_net* net, uint64_t rec_first, uint64_t rec_steps);
+int szsnn_read_raster(szsnn_net* net, uint64_t* host_out); /* rec_steps * raster_words u64 */
+
+int szsnn_get_vars(szsnn_net* net, float* host_out);       /* [n_vars][N] */
+int szsnn_set_vars(szsnn_net* net, const float* host_in);  /* [n_vars][N] */
+int szsnn_get_counters(szsnn_net* net, szsnn_counters* out);
+int szsnn_reset_counters(szsnn_net* net);
+
+/* Per-kernel timing with HIP events on the context stream: runs n steps
+ * launched one kernel at a time (no graph) and returns the summed durations of
+ * the gather and update kernels in milliseconds. */
+int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, double* update_ms);
+
+/* Launch configuration (block sizes and rows per gather block); 0 keeps the
+ * current value.  szsnn_autotune times candidates on the live network for
+ * `steps` steps each, keeps the fastest, and restores the state it found. */
+int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_block, uint32_t gather_grid);
+int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_block, uint32_t* gather_grid);
+int szsnn_autotune(szsnn_net* net, uint32_t steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SZSNN_HIP_H_ */

@@ -1,0 +1,79 @@
+// szsnn_internal.h -- device-side view of an uploaded network and the launch
+// interface shared by the runtime (szsnn_runtime.hip) and the kernels
+// (szsnn_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace szsnn {
+
+constexpr uint64_t kGolden = 0x9e3779b97f4a7c15ULL;
+
+// SplitMix64 output mixer (Steele, Lea, Flood 2014); counter-based streams.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+// Output number `idx` (0-based) of a SplitMix64 generator seeded with `seed`.
+__host__ __device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t idx) {
+  return mix64(seed + kGolden * (idx + 1));
+}
+// Per-step draw of stream s (0,1 Gaussian; 2 binomial drive; 3 Poisson source).
+__host__ __device__ inline uint64_t rng_draw(uint64_t key, uint64_t t, unsigned s) {
+  return splitmix_at(key, 4 * t + s);
+}
+
+enum : int { kIfCurrExp = 0, kIzhikevich = 1, kIfCondExp = 2 };
+enum : int { kSrcNone = 0, kSrcConstRate = 1, kSrcPoisson = 2 };
+
+// Everything a step kernel reads, passed by value as one kernel argument.
+struct DevNet {
+  int32_t model;
+  int32_t source_model;
+  uint32_t N;        // neurons
+  uint32_t M;        // input sources
+  uint32_t T;        // N + M ids
+  uint32_t D;        // delay slots (API delay k < D)
+  uint32_t R;        // due-queue ring length = D + 1
+  uint32_t W;        // raster words per step = ceil(T / 64)
+  uint32_t npad;     // stride of the SoA arrays (N rounded up to 64)
+  uint32_t drive_k;
+  int32_t frac_bits;
+  float acc_scale;   // 2^-frac_bits
+  long long drive_w; // fixed-point weight of one drive event
+  const float* params;      // [np][npad]
+  float* vars;              // [nv][npad]
+  long long* acc;           // [2][npad] fixed-point E/I accumulators
+  const uint64_t* keys;     // [T] random-stream keys
+  const float* noise_amp;   // [N] or null
+  const float* src_rate;    // [M] or null
+  float* src_last;          // [M] or null
+  const uint64_t* src_thresh;  // [M] or null
+  const uint64_t* drive_cdf;   // [drive_k] or null
+  const uint32_t* row_off;  // [T*D + 1] CSR row offsets, row = id*D + k
+  const uint2* syn;         // [S] (target, fixed-point weight as int32 bits)
+  const uint64_t* delay_bits;  // [T] bit k set iff row (id, k) is nonempty
+  uint32_t* q_count;        // [R] entries in each due-queue slot
+  uint32_t* q_rows;         // [R][q_cap] row ids due at the slot's step
+  uint64_t q_cap;
+  uint64_t* raster;         // [rec_steps][W] or null
+  uint64_t rec_first;
+  uint64_t rec_steps;
+  unsigned long long* counters;  // [0]=events [1]=spikes [2]=drive events [3]=overflow
+  const uint64_t* t_base;   // device cycle counter
+  uint32_t volley;          // ids [0, volley) are forced every step
+  const uint64_t* forced_bits;   // [W] forced ids this step, or null
+};
+
+// Counter slots.
+enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntSlots = 4 };
+
+// Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
+hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);
+hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s);
+hipError_t launch_advance(uint64_t* t_base, uint64_t k, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
+
+}  // namespace szsnn

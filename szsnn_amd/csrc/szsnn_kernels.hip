@@ -1,0 +1,300 @@
+// szsnn_kernels.hip -- per-step CDNA4 kernels of the SZSNN core (gfx950, wave64).
+//
+// Per step t (reference order, szsnn1.1/src/Simulation.ipp:471-483):
+//   gather(t)  : drain due-queue slot t % R; expand every due (source, delay)
+//                CSR row; add each synapse's fixed-point weight into the
+//                target's E or I accumulator.  Replaces gather_kernel.cl:67-200
+//                and the global half of scatter_kernel.cl:194-355.
+//   update(t)  : one id per lane, SoA state; neuron model or input-source model;
+//                spikes compacted per wavefront with ballot; each spiking id
+//                appends its nonempty delay rows to the due-queue slot of the
+//                step that will gather them (wave-aggregated atomics).  Replaces
+//                *_update_kernel.cl and the local half of scatter_kernel.cl:97-190.
+//
+// Arithmetic is FP32 with one written evaluation order; the library is built
+// with -ffp-contract=off and IEEE division, so results are bit-identical to
+// the FP32 CPU restatement of the same equations.
+#include "szsnn_internal.h"
+
+namespace szsnn {
+
+namespace {
+
+__device__ inline int32_t gauss_int(uint64_t key, uint64_t t) {
+  const uint64_t a = rng_draw(key, t, 0);
+  const uint64_t b = rng_draw(key, t, 1);
+  uint32_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    sum += (uint32_t)((a >> (16 * j)) & 0xffffu);
+    sum += (uint32_t)((b >> (16 * j)) & 0xffffu);
+  }
+  return (int32_t)(2u * sum) - 524280;
+}
+
+// Unit-variance scale of the centred Irwin-Hall sum: 0.5/sqrt(8*(2^32-1)/12).
+constexpr float kGaussScale = 0x1.3988e2p-17f;
+
+__device__ inline uint64_t lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// IF_curr_exp (IF_curr_exp_update_kernel.cl:230-297) and IF_cond_exp (same
+// scheme, conductances in the current slots).  Returns spike.
+template <int MODEL>
+__device__ inline bool if_update(const float* __restrict__ p, float* __restrict__ x, uint32_t i,
+                                 uint32_t stride, float accE, float accI, float iinj, bool forced) {
+  const float v_rest = p[0 * stride + i], cm = p[1 * stride + i], tau_m = p[2 * stride + i];
+  const float tau_refrac = p[3 * stride + i], tau_e = p[4 * stride + i], tau_i = p[5 * stride + i];
+  const float i_offset = p[6 * stride + i], v_reset = p[7 * stride + i], v_thresh = p[8 * stride + i];
+  float v = x[0 * stride + i];
+  const float se = ((1.0f - 1.0f / tau_e) * x[1 * stride + i]) + accE;
+  const float si = ((1.0f - 1.0f / tau_i) * x[2 * stride + i]) + accI;
+  float I;
+  if (MODEL == kIfCurrExp) {
+    I = ((se + si) + iinj) + i_offset;
+  } else {
+    const float e_rev_e = p[9 * stride + i], e_rev_i = p[10 * stride + i];
+    I = (((se * (e_rev_e - v)) - (si * (e_rev_i - v))) + iinj) + i_offset;
+  }
+  float fired = x[3 * stride + i];
+  const bool refr = fired <= tau_refrac;
+  fired = fired + 1.0f;
+  if (!refr) {
+    const float delta = (I / cm) + ((v_rest - v) / tau_m);
+    v = v + delta;
+  }
+  const bool spike = (!refr && v >= v_thresh) || forced;
+  if (spike) {
+    v = v_reset;
+    fired = 1.0f;
+  }
+  x[0 * stride + i] = v;
+  x[1 * stride + i] = se;
+  x[2 * stride + i] = si;
+  x[3 * stride + i] = fired;
+  return spike;
+}
+
+// Izhikevich (Izhikevich_update_kernel.cl:219,247-251,264-268): one Euler step
+// of both equations from the old (v, u).
+__device__ inline bool izh_update(const float* __restrict__ p, float* __restrict__ x, uint32_t i,
+                                  uint32_t stride, float accE, float accI, float iinj, bool forced) {
+  const float a = p[0 * stride + i], b = p[1 * stride + i];
+  float v = x[0 * stride + i], u = x[1 * stride + i];
+  const float I = (accE + accI) + iinj;
+  const float dv = ((((0.04f * v) + 5.0f) * v + 140.0f) - u) + I;
+  const float du = a * ((b * v) - u);
+  v = v + dv;
+  u = u + du;
+  const bool spike = (v >= 30.0f) || forced;
+  if (spike) {
+    v = p[2 * stride + i];
+    u = u + p[3 * stride + i];
+  }
+  x[0 * stride + i] = v;
+  x[1 * stride + i] = u;
+  return spike;
+}
+
+template <int MODEL>
+__global__ void __launch_bounds__(1024) update_kernel(DevNet n, uint32_t j) {
+  const uint64_t t = *n.t_base + j;
+  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) n.q_count[t % n.R] = 0;  // slot gathered this step
+  bool spike = false;
+  uint64_t bits = 0;
+  if (id < n.T) {
+    bool forced = id < n.volley;
+    if (n.forced_bits) forced = forced || ((n.forced_bits[id >> 6] >> (id & 63)) & 1ull);
+    if (id < n.N) {
+      long long ae = n.acc[id];
+      const long long ai = n.acc[n.npad + id];
+      n.acc[id] = 0;
+      n.acc[n.npad + id] = 0;
+      if (n.drive_k) {
+        const uint64_t u = rng_draw(n.keys[id], t, 2);
+        uint32_t k = 0;
+        while (k < n.drive_k && u >= n.drive_cdf[k]) ++k;
+        if (k) {
+          ae += (long long)k * n.drive_w;
+          atomicAdd(&n.counters[kCntDrive], (unsigned long long)k);
+        }
+      }
+      const float accE = (float)ae * n.acc_scale;
+      const float accI = (float)ai * n.acc_scale;
+      const float iinj = n.noise_amp ? n.noise_amp[id] * ((float)gauss_int(n.keys[id], t) * kGaussScale) : 0.0f;
+      if (MODEL == kIzhikevich)
+        spike = izh_update(n.params, n.vars, id, n.npad, accE, accI, iinj, forced);
+      else
+        spike = if_update<MODEL>(n.params, n.vars, id, n.npad, accE, accI, iinj, forced);
+    } else {
+      const uint32_t s = id - n.N;
+      spike = forced;
+      if (n.source_model == kSrcConstRate) {
+        // Const_rate_update_kernel.cl:213-227
+        float last = n.src_last[s] + 1.0f;
+        spike = spike || (last == n.src_rate[s]);
+        if (spike) last = 0.0f;
+        n.src_last[s] = last;
+      } else if (n.source_model == kSrcPoisson) {
+        spike = spike || (rng_draw(n.keys[id], t, 3) < n.src_thresh[s]);
+      }
+    }
+    if (spike) bits = n.delay_bits[id];
+  }
+  // dense spike output, one u64 word per wavefront (storeDenseSpikes,
+  // IF_curr_exp_update_kernel.cl:58-94)
+  const uint64_t mask = __ballot(spike);
+  const uint32_t lane = __lane_id();
+  const uint32_t word = id >> 6;
+  if (lane == 0 && word < n.W) {
+    if (n.raster && t >= n.rec_first && t - n.rec_first < n.rec_steps)
+      n.raster[(t - n.rec_first) * n.W + word] = mask;
+    if (mask) atomicAdd(&n.counters[kCntSpikes], (unsigned long long)__popcll(mask));
+  }
+  if (mask == 0) return;
+  // scatter: for each delay some spiking lane of this wave has, one atomic
+  // reserves room for all of the wave's rows of that delay.
+  uint64_t uni = bits;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
+  const uint64_t lt = lanemask_lt();
+  while (uni) {
+    const uint32_t k = (uint32_t)__ffsll((unsigned long long)uni) - 1;
+    uni &= uni - 1;
+    const bool mine = (bits >> k) & 1ull;
+    const uint64_t m = __ballot(mine);
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    const uint32_t slot = (uint32_t)((t + k + 1) % n.R);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&n.q_count[slot], (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (mine) {
+      const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(m & lt);
+      if (pos < n.q_cap)
+        n.q_rows[(uint64_t)slot * n.q_cap + pos] = id * n.D + k;
+      else
+        atomicOr(&n.counters[kCntOverflow], 1ull);
+    }
+  }
+}
+
+// Block-wide exclusive scan of one u32 per thread; returns the total.
+template <int BLK>
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, uint32_t* s_wave) {
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t inc = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off);
+    if (lane >= (uint32_t)off) inc += y;
+  }
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int w = 0; w < BLK / 64; ++w) {
+      const uint32_t v = s_wave[w];
+      s_wave[w] = acc;
+      acc += v;
+    }
+    s_wave[BLK / 64] = acc;
+  }
+  __syncthreads();
+  s_pref[threadIdx.x] = s_wave[wave] + inc - x;
+  if (threadIdx.x == 0) s_pref[BLK] = s_wave[BLK / 64];
+  __syncthreads();
+  return s_pref[BLK];
+}
+
+// gather(t): BLK due rows per chunk; their (start, length) are staged in LDS
+// and scanned, then the block walks the chunk's synapses contiguously so each
+// row segment is read with coalesced 8-byte loads.
+template <int BLK>
+__global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
+  __shared__ uint32_t s_start[BLK];
+  __shared__ uint32_t s_pref[BLK + 1];
+  __shared__ uint32_t s_wave[BLK / 64 + 1];
+  const uint64_t t = *n.t_base + j;
+  const uint32_t slot = (uint32_t)(t % n.R);
+  const uint32_t nrows = n.q_count[slot];
+  const uint32_t* rows = n.q_rows + (uint64_t)slot * n.q_cap;
+  unsigned long long delivered = 0;
+  for (uint32_t chunk = blockIdx.x * BLK; chunk < nrows; chunk += gridDim.x * BLK) {
+    const uint32_t r = chunk + threadIdx.x;
+    uint32_t start = 0, len = 0;
+    if (r < nrows) {
+      const uint32_t row = rows[r];
+      start = n.row_off[row];
+      len = n.row_off[row + 1] - start;
+    }
+    s_start[threadIdx.x] = start;
+    const uint32_t total = block_exclusive_scan<BLK>(len, s_pref, s_wave);
+    for (uint32_t e = threadIdx.x; e < total; e += BLK) {
+      uint32_t lo = 0, hi = BLK;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= e) lo = mid; else hi = mid;
+      }
+      const uint2 sw = n.syn[s_start[lo] + (e - s_pref[lo])];
+      const int32_t w = (int32_t)sw.y;
+      long long* dst = n.acc + (w < 0 ? n.npad : 0u) + sw.x;
+      atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)w);
+    }
+    delivered += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && delivered) atomicAdd(&n.counters[kCntEvents], delivered);
+}
+
+__global__ void advance_kernel(uint64_t* t_base, uint64_t k) { *t_base += k; }
+
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t value, uint64_t count) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+}  // namespace
+
+hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s) {
+  if (block == 0 || block % 64 || block > 1024) return hipErrorInvalidValue;
+  const uint32_t grid = (n.T + block - 1) / block;
+  switch (n.model) {
+    case kIfCurrExp: update_kernel<kIfCurrExp><<<grid, block, 0, s>>>(n, j); break;
+    case kIzhikevich: update_kernel<kIzhikevich><<<grid, block, 0, s>>>(n, j); break;
+    case kIfCondExp: update_kernel<kIfCondExp><<<grid, block, 0, s>>>(n, j); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {
+  if (grid == 0) return hipErrorInvalidValue;
+  switch (block) {
+    case 128: gather_kernel<128><<<grid, 128, 0, s>>>(n, j); break;
+    case 256: gather_kernel<256><<<grid, 256, 0, s>>>(n, j); break;
+    case 512: gather_kernel<512><<<grid, 512, 0, s>>>(n, j); break;
+    case 1024: gather_kernel<1024><<<grid, 1024, 0, s>>>(n, j); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(uint64_t* t_base, uint64_t k, hipStream_t s) {
+  advance_kernel<<<1, 1, 0, s>>>(t_base, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t value, uint64_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  fill_u64_kernel<<<(uint32_t)blocks, 256, 0, s>>>(p, value, count);
+  return hipGetLastError();
+}
+
+}  // namespace szsnn

@@ -1,0 +1,797 @@
+// szsnn_runtime.hip -- host side of the C ABI (include/szsnn_hip.h).
+//
+// Replaces the reference's OpenCL runtime (szsnn1.1/src/Context.cpp,
+// DataBuffer.hpp, KernelWrapper.cpp) and the device-format builders
+// (NeuronData.ipp, SynapseData.ipp, SynapseIndex.cpp, GroupData.cpp,
+// GlobalQueue.cpp, LocalQueue.cpp): one HIP stream per context, device
+// memory owned by the library, a delay-bucketed CSR in place of the bundle
+// matrix, and HIP-graph-captured step loops in place of blocking per-kernel
+// enqueue/wait/finish (Simulation.ipp:402-460).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/szsnn_hip.h"
+#include "szsnn_internal.h"
+
+using namespace szsnn;
+
+namespace {
+std::mutex g_err_mu;
+std::string g_err;  // last error of any context
+
+void set_global_error(const std::string& m) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = m;
+}
+}  // namespace
+
+struct szsnn_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  std::string err;
+  std::string name;
+};
+
+struct Tiling {
+  uint32_t update_block = 256;
+  uint32_t gather_block = 256;
+  uint32_t gather_grid = 0;
+};
+
+struct szsnn_net {
+  szsnn_ctx* ctx = nullptr;
+  DevNet dn{};
+  szsnn_net_info info{};
+  uint64_t t_host = 0;
+  Tiling tiling;
+  std::vector<void*> allocs;
+  uint64_t* forced_bits = nullptr;
+  uint64_t* step_bitmap = nullptr;
+  uint64_t* t_base = nullptr;
+  // graph of kGraphSteps steps, valid for `graph_dn`/`graph_tiling`
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool graph_dirty = true;
+  // autotune snapshot buffers
+  std::vector<std::pair<void*, size_t>> snap;
+};
+
+namespace {
+
+constexpr uint32_t kGraphSteps = 50;
+
+int fail(szsnn_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  set_global_error(msg);
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                               \
+  do {                                                                                   \
+    hipError_t e__ = (expr);                                                             \
+    if (e__ != hipSuccess) {                                                             \
+      std::ostringstream os__;                                                           \
+      os__ << #expr << " failed: " << hipGetErrorString(e__) << " (" << (int)e__ << ")"; \
+      return fail((ctx), SZSNN_ERR_HIP, os__.str());                                     \
+    }                                                                                    \
+  } while (0)
+
+int n_params_of(int model) {
+  return model == kIfCurrExp ? 9 : model == kIzhikevich ? 4 : model == kIfCondExp ? 11 : -1;
+}
+int n_vars_of(int model) {
+  return model == kIfCurrExp ? 4 : model == kIzhikevich ? 2 : model == kIfCondExp ? 4 : -1;
+}
+
+template <typename T>
+int dev_alloc(szsnn_net* net, T** out, size_t count) {
+  void* p = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    std::ostringstream os;
+    os << "hipMalloc(" << bytes << ") failed: " << hipGetErrorString(e);
+    return fail(net->ctx, SZSNN_ERR_NOMEM, os.str());
+  }
+  net->allocs.push_back(p);
+  net->info.device_bytes += bytes;
+  *out = static_cast<T*>(p);
+  return SZSNN_OK;
+}
+
+template <typename T>
+int upload(szsnn_net* net, T* dst, const T* src, size_t count) {
+  if (count == 0) return SZSNN_OK;
+  HIP_TRY(net->ctx, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, net->ctx->stream));
+  return SZSNN_OK;
+}
+
+void free_graph(szsnn_net* net) {
+  if (net->graph_exec) (void)hipGraphExecDestroy(net->graph_exec);
+  if (net->graph) (void)hipGraphDestroy(net->graph);
+  net->graph_exec = nullptr;
+  net->graph = nullptr;
+  net->graph_dirty = true;
+}
+
+uint32_t default_gather_grid(const szsnn_net* net, uint32_t block) {
+  const uint64_t chunks = (net->dn.q_cap + block - 1) / block;
+  const uint64_t cap = (uint64_t)std::max(net->ctx->cu_count, 1) * 8;
+  return (uint32_t)std::max<uint64_t>(1, std::min(chunks, cap));
+}
+
+int launch_step(szsnn_net* net, const DevNet& dn, uint32_t j) {
+  szsnn_ctx* ctx = net->ctx;
+  HIP_TRY(ctx, launch_gather(dn, j, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream));
+  HIP_TRY(ctx, launch_update(dn, j, net->tiling.update_block, ctx->stream));
+  return SZSNN_OK;
+}
+
+int build_graph(szsnn_net* net) {
+  free_graph(net);
+  szsnn_ctx* ctx = net->ctx;
+  HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = SZSNN_OK;
+  for (uint32_t j = 0; j < kGraphSteps && rc == SZSNN_OK; ++j) rc = launch_step(net, net->dn, j);
+  if (rc == SZSNN_OK) {
+    hipError_t e = launch_advance(net->t_base, kGraphSteps, ctx->stream);
+    if (e != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("advance launch: ") + hipGetErrorString(e));
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != SZSNN_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(ctx, SZSNN_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  net->graph = g;
+  HIP_TRY(ctx, hipGraphInstantiate(&net->graph_exec, g, nullptr, nullptr, 0));
+  net->graph_dirty = false;
+  return SZSNN_OK;
+}
+
+int run_steps(szsnn_net* net, uint64_t n) {
+  szsnn_ctx* ctx = net->ctx;
+  if (n >= kGraphSteps && net->graph_dirty) {
+    int rc = build_graph(net);
+    if (rc) return rc;
+  }
+  uint64_t done = 0;
+  while (n - done >= kGraphSteps) {
+    HIP_TRY(ctx, hipGraphLaunch(net->graph_exec, ctx->stream));
+    done += kGraphSteps;
+  }
+  const uint64_t rem = n - done;
+  for (uint32_t j = 0; j < rem; ++j) {
+    int rc = launch_step(net, net->dn, j);
+    if (rc) return rc;
+  }
+  if (rem) HIP_TRY(ctx, launch_advance(net->t_base, rem, ctx->stream));
+  net->t_host += n;
+  return SZSNN_OK;
+}
+
+int check_overflow(szsnn_net* net) {
+  unsigned long long c[kCntSlots];
+  HIP_TRY(net->ctx, hipMemcpyAsync(c, net->dn.counters, sizeof(c), hipMemcpyDeviceToHost, net->ctx->stream));
+  HIP_TRY(net->ctx, hipStreamSynchronize(net->ctx->stream));
+  if (c[kCntOverflow])
+    return fail(net->ctx, SZSNN_ERR_OVERFLOW,
+                "due-queue overflow: spikes were dropped (raise queue_capacity or use 0 for worst case)");
+  return SZSNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* szsnn_last_error(const szsnn_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  return g_err.c_str();
+}
+
+int szsnn_create(int device, szsnn_ctx** out) {
+  if (!out) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_create: out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(nullptr, SZSNN_ERR_HIP, std::string("no HIP device: ") + hipGetErrorString(e));
+  if (device < 0 || device >= count) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_create: device index out of range");
+  auto* ctx = new (std::nothrow) szsnn_ctx();
+  if (!ctx) return fail(nullptr, SZSNN_ERR_NOMEM, "szsnn_create: out of host memory");
+  ctx->device = device;
+  e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  hipDeviceProp_t prop;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(nullptr, SZSNN_ERR_HIP, std::string("szsnn_create: ") + hipGetErrorString(e));
+  }
+  ctx->cu_count = prop.multiProcessorCount;
+  ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+  *out = ctx;
+  return SZSNN_OK;
+}
+
+int szsnn_destroy(szsnn_ctx* ctx) {
+  if (!ctx) return SZSNN_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SZSNN_OK;
+}
+
+int szsnn_device_name(szsnn_ctx* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || len == 0) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_device_name: bad argument");
+  std::snprintf(buf, len, "%s cus=%d", ctx->name.c_str(), ctx->cu_count);
+  return SZSNN_OK;
+}
+
+void* szsnn_stream(szsnn_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int szsnn_alloc(szsnn_ctx* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_alloc: bad argument");
+  hipError_t e = hipMalloc(dptr, std::max<size_t>(bytes, 1));
+  if (e != hipSuccess) return fail(ctx, SZSNN_ERR_NOMEM, std::string("szsnn_alloc: ") + hipGetErrorString(e));
+  return SZSNN_OK;
+}
+int szsnn_free(szsnn_ctx* ctx, void* dptr) {
+  if (!ctx) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_free: ctx is NULL");
+  HIP_TRY(ctx, hipFree(dptr));
+  return SZSNN_OK;
+}
+int szsnn_memcpy_h2d(szsnn_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_memcpy_h2d: ctx is NULL");
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SZSNN_OK;
+}
+int szsnn_memcpy_d2h(szsnn_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_memcpy_d2h: ctx is NULL");
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SZSNN_OK;
+}
+int szsnn_synchronize(szsnn_ctx* ctx) {
+  if (!ctx) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_synchronize: ctx is NULL");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SZSNN_OK;
+}
+
+int szsnn_net_destroy(szsnn_net* net) {
+  if (!net) return SZSNN_OK;
+  (void)hipSetDevice(net->ctx->device);
+  (void)hipStreamSynchronize(net->ctx->stream);
+  free_graph(net);
+  for (void* p : net->allocs) (void)hipFree(p);
+  for (auto& s : net->snap) (void)hipFree(s.first);
+  delete net;
+  return SZSNN_OK;
+}
+
+int szsnn_net_upload(szsnn_ctx* ctx, const szsnn_net_desc* d, szsnn_net** out) {
+  if (!ctx || !d || !out) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: bad argument");
+  *out = nullptr;
+  // Simulation.ipp:97-101
+  if (d->n_neurons == 0) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: network has 0 neurons");
+  const int np = n_params_of(d->model), nv = n_vars_of(d->model);
+  if (np < 0) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: unknown model");
+  if (d->max_delay == 0 || d->max_delay > 64)
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: max_delay must be in 1..64");
+  if (d->frac_bits < 0 || d->frac_bits > 62)
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: frac_bits must be in 0..62");
+  if (!d->params || !d->vars) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: params/vars are NULL");
+  if (d->n_sources) {
+    if (d->source_model == kSrcConstRate && !d->src_rate)
+      return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: Const_rate sources need src_rate");
+    if (d->source_model == kSrcPoisson && !d->src_thresh)
+      return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: Poisson sources need src_thresh");
+    if (d->source_model < 0 || d->source_model > 2)
+      return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: unknown source model");
+  }
+  if (d->drive_k > 32 || (d->drive_k && !d->drive_cdf))
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: drive_k must be <= 32 with drive_cdf");
+  if (d->n_syn && (!d->syn_src || !d->syn_tgt || !d->syn_w || !d->syn_delay))
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: synapse arrays are NULL");
+  const uint64_t N = d->n_neurons, M = d->n_sources, T = N + M, D = d->max_delay;
+  if (T * D >= 0xffffffffull) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: (N+M)*max_delay too large");
+  const double scale = std::ldexp(1.0, d->frac_bits);
+  const uint64_t nrows = T * D;
+
+  // --- delay-bucketed CSR: row = id*D + k, stable in input order
+  std::vector<uint32_t> row_off(nrows + 1, 0);
+  std::vector<int32_t> wfix(d->n_syn);
+  for (uint64_t s = 0; s < d->n_syn; ++s) {
+    const uint32_t src = d->syn_src[s], tgt = d->syn_tgt[s], k = d->syn_delay[s];
+    if (src >= T || tgt >= N) {
+      std::ostringstream os;
+      os << "szsnn_net_upload: synapse " << s << " connects nonexisting ids source=" << src << " target=" << tgt;
+      return fail(ctx, SZSNN_ERR_INVALID, os.str());
+    }
+    if (k >= D) {
+      std::ostringstream os;
+      os << "szsnn_net_upload: synapse " << s << " delay " << k << " >= max_delay " << D;
+      return fail(ctx, SZSNN_ERR_INVALID, os.str());
+    }
+    const double q = std::nearbyint((double)d->syn_w[s] * scale);
+    if (!(std::fabs(q) < 2147483648.0)) {
+      std::ostringstream os;
+      os << "szsnn_net_upload: weight " << d->syn_w[s] << " does not fit int32 at frac_bits " << d->frac_bits;
+      return fail(ctx, SZSNN_ERR_INVALID, os.str());
+    }
+    wfix[s] = (int32_t)q;
+    if (wfix[s] != 0) row_off[(uint64_t)src * D + k + 1]++;
+  }
+  for (uint64_t r = 0; r < nrows; ++r) {
+    const uint64_t next = (uint64_t)row_off[r + 1] + row_off[r];
+    if (next > 0xffffffffull) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: more than 2^32-1 synapses");
+    row_off[r + 1] = (uint32_t)next;
+  }
+  const uint64_t S = row_off[nrows];
+  std::vector<uint2> syn(S);
+  {
+    std::vector<uint32_t> pos(row_off.begin(), row_off.end() - 1);
+    for (uint64_t s = 0; s < d->n_syn; ++s) {
+      if (wfix[s] == 0) continue;
+      const uint64_t r = (uint64_t)d->syn_src[s] * D + d->syn_delay[s];
+      syn[pos[r]++] = make_uint2(d->syn_tgt[s], (uint32_t)wfix[s]);
+    }
+  }
+  std::vector<int32_t>().swap(wfix);
+  std::vector<uint64_t> delay_bits(T, 0);
+  uint64_t nonempty = 0;
+  for (uint64_t id = 0; id < T; ++id)
+    for (uint64_t k = 0; k < D; ++k)
+      if (row_off[id * D + k + 1] != row_off[id * D + k]) {
+        delay_bits[id] |= 1ull << k;
+        ++nonempty;
+      }
+  uint64_t cap = std::max<uint64_t>(nonempty, 1);
+  if (d->queue_capacity) cap = std::min<uint64_t>(cap, d->queue_capacity);
+
+  const double drive_q = std::nearbyint((double)d->drive_w * scale);
+  if (!(std::fabs(drive_q) < 9.2e18)) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: drive_w too large");
+
+  auto* net = new (std::nothrow) szsnn_net();
+  if (!net) return fail(ctx, SZSNN_ERR_NOMEM, "szsnn_net_upload: out of host memory");
+  net->ctx = ctx;
+  hipError_t he = hipSetDevice(ctx->device);
+  if (he != hipSuccess) {
+    delete net;
+    return fail(ctx, SZSNN_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(he));
+  }
+  DevNet& n = net->dn;
+  n.model = d->model;
+  n.source_model = M ? d->source_model : kSrcNone;
+  n.N = (uint32_t)N;
+  n.M = (uint32_t)M;
+  n.T = (uint32_t)T;
+  n.D = (uint32_t)D;
+  n.R = (uint32_t)D + 1;
+  n.W = (uint32_t)((T + 63) / 64);
+  n.npad = (uint32_t)((N + 63) / 64 * 64);
+  n.drive_k = d->drive_k;
+  n.frac_bits = d->frac_bits;
+  n.acc_scale = (float)std::ldexp(1.0, -d->frac_bits);
+  n.drive_w = (long long)drive_q;
+  n.q_cap = cap;
+
+  int rc = SZSNN_OK;
+  auto guard = [&](int r) {
+    if (r != SZSNN_OK && rc == SZSNN_OK) rc = r;
+    return rc == SZSNN_OK;
+  };
+  float *params = nullptr, *vars = nullptr, *noise = nullptr, *src_rate = nullptr, *src_last = nullptr;
+  long long* acc = nullptr;
+  uint64_t *keys = nullptr, *src_thresh = nullptr, *drive_cdf = nullptr, *dbits = nullptr, *raster = nullptr;
+  uint32_t *roff = nullptr, *qc = nullptr, *qr = nullptr;
+  uint2* dsyn = nullptr;
+  unsigned long long* counters = nullptr;
+  guard(dev_alloc(net, &params, (size_t)np * n.npad)) && guard(dev_alloc(net, &vars, (size_t)nv * n.npad)) &&
+      guard(dev_alloc(net, &acc, (size_t)2 * n.npad)) && guard(dev_alloc(net, &keys, T)) &&
+      guard(dev_alloc(net, &roff, nrows + 1)) && guard(dev_alloc(net, &dsyn, S)) &&
+      guard(dev_alloc(net, &dbits, T)) && guard(dev_alloc(net, &qc, n.R)) &&
+      guard(dev_alloc(net, &qr, (size_t)n.R * cap)) && guard(dev_alloc(net, &counters, kCntSlots)) &&
+      guard(dev_alloc(net, &net->t_base, 1)) && guard(dev_alloc(net, &net->forced_bits, n.W)) &&
+      guard(dev_alloc(net, &net->step_bitmap, n.W));
+  if (rc == SZSNN_OK && d->noise_amp) guard(dev_alloc(net, &noise, N));
+  if (rc == SZSNN_OK && M && n.source_model == kSrcConstRate)
+    guard(dev_alloc(net, &src_rate, M)) && guard(dev_alloc(net, &src_last, M));
+  if (rc == SZSNN_OK && M && n.source_model == kSrcPoisson) guard(dev_alloc(net, &src_thresh, M));
+  if (rc == SZSNN_OK && d->drive_k) guard(dev_alloc(net, &drive_cdf, d->drive_k));
+  if (rc != SZSNN_OK) {
+    szsnn_net_destroy(net);
+    return rc;
+  }
+  (void)raster;
+  // host staging with the padded SoA stride
+  std::vector<float> hp((size_t)np * n.npad, 0.0f), hv((size_t)nv * n.npad, 0.0f);
+  for (int k = 0; k < np; ++k) std::memcpy(&hp[(size_t)k * n.npad], d->params + (size_t)k * N, N * 4);
+  for (int k = 0; k < nv; ++k) std::memcpy(&hv[(size_t)k * n.npad], d->vars + (size_t)k * N, N * 4);
+  std::vector<uint64_t> hk(T);
+  for (uint64_t i = 0; i < T; ++i) hk[i] = splitmix_at(d->seed, i);
+  std::vector<float> hsl;
+  if (src_last) {
+    hsl.assign(M, 0.0f);
+    if (d->src_last) std::memcpy(hsl.data(), d->src_last, M * 4);
+  }
+  guard(upload(net, params, hp.data(), hp.size())) && guard(upload(net, vars, hv.data(), hv.size())) &&
+      guard(upload(net, keys, hk.data(), T)) && guard(upload(net, roff, row_off.data(), nrows + 1)) &&
+      guard(upload(net, dsyn, syn.data(), S)) && guard(upload(net, dbits, delay_bits.data(), T));
+  if (rc == SZSNN_OK && noise) guard(upload(net, noise, d->noise_amp, N));
+  if (rc == SZSNN_OK && src_rate) guard(upload(net, src_rate, d->src_rate, M)) && guard(upload(net, src_last, hsl.data(), M));
+  if (rc == SZSNN_OK && src_thresh) guard(upload(net, src_thresh, d->src_thresh, M));
+  if (rc == SZSNN_OK && drive_cdf) guard(upload(net, drive_cdf, d->drive_cdf, d->drive_k));
+  if (rc == SZSNN_OK) {
+    hipStream_t s = ctx->stream;
+    he = hipMemsetAsync(acc, 0, (size_t)2 * n.npad * 8, s);
+    if (he == hipSuccess) he = hipMemsetAsync(qc, 0, n.R * 4, s);
+    if (he == hipSuccess) he = hipMemsetAsync(counters, 0, kCntSlots * 8, s);
+    if (he == hipSuccess) he = hipMemsetAsync(net->t_base, 0, 8, s);
+    if (he == hipSuccess) he = hipMemsetAsync(net->forced_bits, 0, (size_t)n.W * 8, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_net_upload: ") + hipGetErrorString(he));
+  }
+  if (rc != SZSNN_OK) {
+    szsnn_net_destroy(net);
+    return rc;
+  }
+  n.params = params;
+  n.vars = vars;
+  n.acc = acc;
+  n.keys = keys;
+  n.noise_amp = noise;
+  n.src_rate = src_rate;
+  n.src_last = src_last;
+  n.src_thresh = src_thresh;
+  n.drive_cdf = drive_cdf;
+  n.row_off = roff;
+  n.syn = dsyn;
+  n.delay_bits = dbits;
+  n.q_count = qc;
+  n.q_rows = qr;
+  n.raster = nullptr;
+  n.rec_first = 0;
+  n.rec_steps = 0;
+  n.counters = counters;
+  n.t_base = net->t_base;
+  n.volley = 0;
+  n.forced_bits = nullptr;
+
+  net->info.n_neurons = (uint32_t)N;
+  net->info.n_sources = (uint32_t)M;
+  net->info.max_delay = (uint32_t)D;
+  net->info.n_params = (uint32_t)np;
+  net->info.n_vars = (uint32_t)nv;
+  net->info.frac_bits = d->frac_bits;
+  net->info.n_syn_stored = S;
+  net->info.n_rows = nrows;
+  net->info.queue_capacity = cap;
+  net->info.raster_words = n.W;
+  net->tiling.gather_grid = default_gather_grid(net, net->tiling.gather_block);
+  *out = net;
+  return SZSNN_OK;
+}
+
+int szsnn_net_get_info(szsnn_net* net, szsnn_net_info* out) {
+  if (!net || !out) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_net_get_info: bad argument");
+  *out = net->info;
+  return SZSNN_OK;
+}
+
+int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, uint64_t* out_bitmap) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_step: net is NULL");
+  szsnn_ctx* ctx = net->ctx;
+  DevNet dn = net->dn;
+  if (n_forced) {
+    if (!forced_ids) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_step: forced_ids is NULL");
+    std::vector<uint64_t> bits(dn.W, 0);
+    for (uint32_t f = 0; f < n_forced; ++f) {
+      const uint32_t id = forced_ids[f];
+      if (id >= dn.T) {  // SpikeBuffer.cpp:110-113
+        std::ostringstream os;
+        os << "szsnn_step: neuron with idx " << id << " not in the network";
+        return fail(ctx, SZSNN_ERR_INVALID, os.str());
+      }
+      bits[id >> 6] |= 1ull << (id & 63);
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(net->forced_bits, bits.data(), dn.W * 8, hipMemcpyHostToDevice, ctx->stream));
+    dn.forced_bits = net->forced_bits;
+  }
+  const uint64_t t = net->t_host;
+  uint64_t* row = nullptr;
+  if (dn.raster && t >= dn.rec_first && t - dn.rec_first < dn.rec_steps) {
+    row = dn.raster + (t - dn.rec_first) * dn.W;
+  } else if (out_bitmap) {
+    dn.raster = net->step_bitmap;
+    dn.rec_first = t;
+    dn.rec_steps = 1;
+    row = net->step_bitmap;
+  }
+  int rc = launch_step(net, dn, 0);
+  if (rc) return rc;
+  HIP_TRY(ctx, launch_advance(net->t_base, 1, ctx->stream));
+  net->t_host += 1;
+  if (out_bitmap)
+    HIP_TRY(ctx, hipMemcpyAsync(out_bitmap, row, dn.W * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return check_overflow(net);
+}
+
+int szsnn_run_async(szsnn_net* net, uint64_t n_steps) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_run_async: net is NULL");
+  return run_steps(net, n_steps);
+}
+
+int szsnn_run(szsnn_net* net, uint64_t n_steps) {
+  int rc = szsnn_run_async(net, n_steps);
+  if (rc) return rc;
+  return check_overflow(net);
+}
+
+int szsnn_set_volley(szsnn_net* net, uint32_t n_forced_prefix) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_volley: net is NULL");
+  if (n_forced_prefix > net->dn.T) return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_volley: prefix beyond the network");
+  net->dn.volley = n_forced_prefix;
+  net->graph_dirty = true;
+  return SZSNN_OK;
+}
+
+int szsnn_set_recording(szsnn_net* net, uint64_t rec_first, uint64_t rec_steps) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_recording: net is NULL");
+  szsnn_ctx* ctx = net->ctx;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (net->dn.raster) {
+    auto it = std::find(net->allocs.begin(), net->allocs.end(), (void*)net->dn.raster);
+    if (it != net->allocs.end()) net->allocs.erase(it);
+    hipFree(net->dn.raster);
+    net->info.device_bytes -= std::max<uint64_t>(net->dn.rec_steps * net->dn.W, 1) * 8;
+    net->dn.raster = nullptr;
+  }
+  net->dn.rec_first = rec_first;
+  net->dn.rec_steps = rec_steps;
+  if (rec_steps) {
+    uint64_t* r = nullptr;
+    int rc = dev_alloc(net, &r, rec_steps * net->dn.W);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemsetAsync(r, 0, rec_steps * net->dn.W * 8, ctx->stream));
+    net->dn.raster = r;
+  }
+  net->graph_dirty = true;
+  return SZSNN_OK;
+}
+
+int szsnn_read_raster(szsnn_net* net, uint64_t* host_out) {
+  if (!net || !host_out) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_read_raster: bad argument");
+  if (!net->dn.raster) return fail(net->ctx, SZSNN_ERR_STATE, "szsnn_read_raster: recording is off");
+  szsnn_ctx* ctx = net->ctx;
+  HIP_TRY(ctx, hipMemcpyAsync(host_out, net->dn.raster, net->dn.rec_steps * net->dn.W * 8, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SZSNN_OK;
+}
+
+int szsnn_get_vars(szsnn_net* net, float* host_out) {
+  if (!net || !host_out) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_vars: bad argument");
+  szsnn_ctx* ctx = net->ctx;
+  const size_t N = net->dn.N;
+  for (uint32_t k = 0; k < net->info.n_vars; ++k)
+    HIP_TRY(ctx, hipMemcpyAsync(host_out + k * N, net->dn.vars + (size_t)k * net->dn.npad, N * 4,
+                                hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SZSNN_OK;
+}
+
+int szsnn_set_vars(szsnn_net* net, const float* host_in) {
+  if (!net || !host_in) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_set_vars: bad argument");
+  szsnn_ctx* ctx = net->ctx;
+  const size_t N = net->dn.N;
+  for (uint32_t k = 0; k < net->info.n_vars; ++k)
+    HIP_TRY(ctx, hipMemcpyAsync(net->dn.vars + (size_t)k * net->dn.npad, host_in + k * N, N * 4,
+                                hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SZSNN_OK;
+}
+
+int szsnn_get_counters(szsnn_net* net, szsnn_counters* out) {
+  if (!net || !out) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_counters: bad argument");
+  unsigned long long c[kCntSlots];
+  szsnn_ctx* ctx = net->ctx;
+  HIP_TRY(ctx, hipMemcpyAsync(c, net->dn.counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  out->cycle = net->t_host;
+  out->events = c[kCntEvents];
+  out->spikes = c[kCntSpikes];
+  out->drive_events = c[kCntDrive];
+  out->overflow = c[kCntOverflow];
+  return SZSNN_OK;
+}
+
+int szsnn_reset_counters(szsnn_net* net) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_reset_counters: net is NULL");
+  HIP_TRY(net->ctx, hipMemsetAsync(net->dn.counters, 0, kCntSlots * 8, net->ctx->stream));
+  HIP_TRY(net->ctx, hipStreamSynchronize(net->ctx->stream));
+  return SZSNN_OK;
+}
+
+int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, double* update_ms) {
+  if (!net || !gather_ms || !update_ms) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_time_kernels: bad argument");
+  szsnn_ctx* ctx = net->ctx;
+  // three events per step: before gather, between gather and update, after update
+  std::vector<hipEvent_t> ev(3 * n_steps);
+  for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
+  int rc = SZSNN_OK;
+  for (uint64_t s = 0; s < n_steps && rc == SZSNN_OK; ++s) {
+    hipError_t e = hipEventRecord(ev[3 * s], ctx->stream);
+    if (e == hipSuccess) e = launch_gather(net->dn, 0, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[3 * s + 1], ctx->stream);
+    if (e == hipSuccess) e = launch_update(net->dn, 0, net->tiling.update_block, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[3 * s + 2], ctx->stream);
+    if (e == hipSuccess) e = launch_advance(net->t_base, 1, ctx->stream);
+    if (e != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_time_kernels: ") + hipGetErrorString(e));
+    net->t_host += 1;
+  }
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  double g = 0, u = 0;
+  if (rc == SZSNN_OK && e == hipSuccess) {
+    for (uint64_t s = 0; s < n_steps; ++s) {
+      float a = 0, b = 0;
+      hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]);
+      hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]);
+      g += a;
+      u += b;
+    }
+  }
+  for (auto& x : ev) hipEventDestroy(x);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_time_kernels: ") + hipGetErrorString(e));
+  *gather_ms = g;
+  *update_ms = u;
+  return check_overflow(net);
+}
+
+int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_block, uint32_t gather_grid) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_tiling: net is NULL");
+  if (update_block && (update_block % 64 || update_block > 1024))
+    return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling: update_block must be a multiple of 64, <= 1024");
+  if (gather_block && gather_block != 128 && gather_block != 256 && gather_block != 512 && gather_block != 1024)
+    return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling: gather_block must be 128, 256, 512 or 1024");
+  if (update_block) net->tiling.update_block = update_block;
+  if (gather_block) net->tiling.gather_block = gather_block;
+  net->tiling.gather_grid = gather_grid ? gather_grid : default_gather_grid(net, net->tiling.gather_block);
+  net->graph_dirty = true;
+  return SZSNN_OK;
+}
+
+int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_block, uint32_t* gather_grid) {
+  if (!net || !update_block || !gather_block || !gather_grid)
+    return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_tiling: bad argument");
+  *update_block = net->tiling.update_block;
+  *gather_block = net->tiling.gather_block;
+  *gather_grid = net->tiling.gather_grid;
+  return SZSNN_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- autotuner
+namespace {
+
+struct Region {
+  void* p;
+  size_t bytes;
+};
+
+std::vector<Region> state_regions(szsnn_net* net) {
+  const DevNet& n = net->dn;
+  std::vector<Region> r = {
+      {n.vars, (size_t)net->info.n_vars * n.npad * 4},
+      {n.acc, (size_t)2 * n.npad * 8},
+      {n.q_count, (size_t)n.R * 4},
+      {n.q_rows, (size_t)n.R * n.q_cap * 4},
+      {n.counters, (size_t)kCntSlots * 8},
+      {net->t_base, 8},
+  };
+  if (n.src_last) r.push_back({n.src_last, (size_t)n.M * 4});
+  return r;
+}
+
+int snapshot(szsnn_net* net, bool restore) {
+  szsnn_ctx* ctx = net->ctx;
+  auto regs = state_regions(net);
+  if (net->snap.empty()) {
+    for (auto& r : regs) {
+      void* p = nullptr;
+      hipError_t e = hipMalloc(&p, r.bytes);
+      if (e != hipSuccess) return fail(ctx, SZSNN_ERR_NOMEM, "autotune: snapshot allocation failed");
+      net->snap.push_back({p, r.bytes});
+    }
+  }
+  for (size_t i = 0; i < regs.size(); ++i) {
+    if (restore)
+      HIP_TRY(ctx, hipMemcpyAsync(regs[i].p, net->snap[i].first, regs[i].bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    else
+      HIP_TRY(ctx, hipMemcpyAsync(net->snap[i].first, regs[i].p, regs[i].bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return SZSNN_OK;
+}
+
+}  // namespace
+
+extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
+  if (!net || steps == 0) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_autotune: bad argument");
+  szsnn_ctx* ctx = net->ctx;
+  const uint64_t t0 = net->t_host;
+  int rc = snapshot(net, false);
+  if (rc) return rc;
+  const int cus = std::max(ctx->cu_count, 1);
+  const uint32_t ublocks[] = {64, 128, 256, 512, 1024};
+  const uint32_t gblocks[] = {128, 256, 512, 1024};
+  const uint32_t gmult[] = {2, 4, 8, 16};
+  Tiling best = net->tiling;
+  float best_ms = 1e30f;
+  hipEvent_t e0, e1;
+  HIP_TRY(ctx, hipEventCreate(&e0));
+  HIP_TRY(ctx, hipEventCreate(&e1));
+  auto trial = [&](const Tiling& tl, float* ms) -> int {
+    net->tiling = tl;
+    free_graph(net);
+    int r = snapshot(net, true);
+    net->t_host = t0;
+    if (r) return r;
+    r = run_steps(net, kGraphSteps);  // warm-up + graph build
+    if (r) return r;
+    HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+    r = run_steps(net, steps);
+    if (r) return r;
+    HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(e1));
+    HIP_TRY(ctx, hipEventElapsedTime(ms, e0, e1));
+    return SZSNN_OK;
+  };
+  // stage 1: update block at the current gather tiling; stage 2: gather tiling
+  for (uint32_t ub : ublocks) {
+    Tiling tl = best;
+    tl.update_block = ub;
+    float ms = 0;
+    if ((rc = trial(tl, &ms))) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = tl;
+    }
+  }
+  for (uint32_t gb : gblocks) {
+    if (rc) break;
+    for (uint32_t gm : gmult) {
+      Tiling tl = best;
+      tl.gather_block = gb;
+      tl.gather_grid = (uint32_t)cus * gm;
+      float ms = 0;
+      if ((rc = trial(tl, &ms))) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = tl;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  net->tiling = best;
+  free_graph(net);
+  int r2 = snapshot(net, true);
+  net->t_host = t0;
+  if (rc) return rc;
+  return r2;
+}
