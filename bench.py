@@ -1,0 +1,160 @@
+"""Benchmark: synaptic events/s and realtime factor of the plain IF_cond_exp
+100k-neuron network (1e8 synapses, dt = 0.1 ms) on one MI355X.
+
+A "step" is one simulation step (gather -> update -> scatter) of the whole
+network.  Inputs (network, state) are resident in HBM before the timed region.
+Prints one JSON line (the driver's contract) with:
+  value          synaptic events/s over the timed steps (natural dynamics)
+  roofline       gather kernel at the volley load (all neurons forced every
+                 step: every synapse row due each step), HIP-event timed
+  cpu_baseline   the FP32 CPU restatement (one thread) on a bounded sample
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--volley-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=3000, help="CPU baseline sample (steps of the same net)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="scaled-down network (smoke use only)")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from szsnn_amd.configs import if_cond_exp_100k
+    from szsnn_amd.simulation import Context, Simulation
+
+    t0 = time.time()
+    if args.small:
+        net = if_cond_exp_100k(n_exc=8000, n_inh=2000, k_exc=400, k_inh=100)
+    else:
+        net = if_cond_exp_100k()
+    log(f"[bench] network built in {time.time() - t0:.1f}s: {net.n_neurons} neurons, {net.n_syn} synapses")
+    dt_ms = net.meta["dt_ms"]
+    ctx = Context(0)
+    t0 = time.time()
+    sim = Simulation(ctx, net)
+    log(f"[bench] uploaded in {time.time() - t0:.1f}s to {ctx.name}; device bytes {sim.info.device_bytes}")
+
+    # warmup (also builds the step graph)
+    sim.run(args.warmup)
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    c0 = sim.counters()
+    t1 = time.perf_counter()
+    sim.run(args.steps)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    c1 = sim.counters()
+    events = c1["events"] - c0["events"]
+    spikes = c1["spikes"] - c0["spikes"]
+    rows = c1["rows"] - c0["rows"]
+    ms_per_step = wall * 1e3 / args.steps
+    ev_per_s = events / wall
+    rtf = args.steps * dt_ms / (wall * 1e3)
+    rate_hz = spikes / net.n_neurons / (args.steps * dt_ms * 1e-3)
+    log(f"[bench] natural: {ms_per_step * 1e3:.2f} us/step, {ev_per_s:.3e} events/s, RTF {rtf:.2f}, "
+        f"{rate_hz:.2f} Hz, {events / args.steps:.0f} events/step")
+
+    # per-kernel time on the natural dynamics (HIP events, same stream)
+    kn = 500
+    cg0 = sim.counters()
+    g_ms, u_ms = sim.time_kernels(kn)
+    cg1 = sim.counters()
+    nat_g_bytes = 8 * (cg1["events"] - cg0["events"]) + 8 * (cg1["rows"] - cg0["rows"])
+    nat_u_bytes = 48 * net.n_neurons * kn
+    natural = {
+        "gather_us": g_ms * 1e3 / kn, "update_us": u_ms * 1e3 / kn,
+        "gather_gbs": nat_g_bytes / (g_ms * 1e-3) / 1e9 if g_ms > 0 else None,
+        "update_gbs": nat_u_bytes / (u_ms * 1e-3) / 1e9 if u_ms > 0 else None,
+    }
+    log(f"[bench] natural kernels: {natural}")
+
+    # volley: every neuron forced every step -> every (source, delay) row is due
+    sim.set_volley(net.n_neurons)
+    sim.run(net.max_delay + 2)  # fill all delay slots
+    cv0 = sim.counters()
+    gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
+    cv1 = sim.counters()
+    sim.set_volley(0)
+    v_events = cv1["events"] - cv0["events"]
+    v_rows = cv1["rows"] - cv0["rows"]
+    alg_bytes = 8 * v_events + 8 * v_rows
+    per_launch = alg_bytes / args.volley_steps
+    g_avg_s = gv_ms * 1e-3 / args.volley_steps
+    achieved = per_launch / g_avg_s / 1e9
+    log(f"[bench] volley gather: {g_avg_s * 1e6:.1f} us/launch, {v_events / args.volley_steps:.3e} events/launch,"
+        f" {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak); update {uv_ms * 1e3 / args.volley_steps:.1f} us")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.pyoracle import Oracle
+        t0 = time.time()
+        o = Oracle(net, 32)
+        log(f"[bench] CPU oracle built in {time.time() - t0:.1f}s")
+        secs = o.time_steps(args.cpu_steps)
+        cpu = {"value": o.events / secs, "unit": "events/s", "cores": 1, "kind": "port",
+               "sample": f"same network, FP32 CPU restatement, first {args.cpu_steps} steps "
+                         f"({args.cpu_steps * dt_ms:.0f} ms simulated) from the initial state; "
+                         f"{secs:.1f} s wall, RTF {args.cpu_steps * dt_ms / (secs * 1e3):.3f}",
+               "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count()}
+        log(f"[bench] CPU: {cpu}")
+
+    out = {
+        "metric": "synaptic events/s (plain IF_cond_exp 100k net); realtime factor; gather % HBM peak",
+        "value": ev_per_s,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (int64 fixed-point accumulation)",
+        "data": "synthetic (seeded random network and drive)",
+        "config": {"workload": "if_cond_exp_100k" if not args.small else "if_cond_exp_10k_small",
+                   "neurons": net.n_neurons, "synapses": net.n_syn, "dt_ms": dt_ms,
+                   "max_delay_steps": net.max_delay, "drive": "Binomial(100, 5 Hz * dt) per neuron-step"},
+        "realtime_factor": rtf,
+        "firing_rate_hz": rate_hz,
+        "events_per_step": events / args.steps,
+        "natural_kernels": natural,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "gather_kernel (volley: all 100k neurons forced each step)",
+                     "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

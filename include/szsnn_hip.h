@@ -99,6 +99,7 @@ typedef struct szsnn_counters {
   uint64_t spikes;        /* spikes of neurons and sources */
   uint64_t drive_events;  /* binomial drive events (not synaptic) */
   uint64_t overflow;      /* nonzero if a due queue overflowed */
+  uint64_t rows;          /* (source, delay) rows drained by gather */
 } szsnn_counters;
 
 /* ---- context / device runtime (replaces Context.hpp:77-303) ---- */

@@ -132,6 +132,10 @@ inline bool update_neuron(int model, const R* p, R* x, R accE, R accI, R iinj, b
   const R v_rest = p[0], cm = p[1], tau_m = p[2], tau_refrac = p[3];
   const R tau_e = p[4], tau_i = p[5], i_offset = p[6], v_reset = p[7], v_thresh = p[8];
   R v = x[0];
+  // Division is evaluated as a product with the reciprocal, x * (1/y): the
+  // reference kernels were built without correctly rounded division
+  // (Simulation.ipp:13-21), and its FP32 trace (szsnn1.1/bin/myfile.txt~)
+  // agrees with this form, not with the IEEE quotient.
   const R se = ((one - one / tau_e) * x[1]) + accE;
   const R si = ((one - one / tau_i) * x[2]) + accI;
   R I;
@@ -145,7 +149,7 @@ inline bool update_neuron(int model, const R* p, R* x, R accE, R accI, R iinj, b
   const bool refr = fired <= tau_refrac;
   fired = fired + one;
   if (!refr) {
-    const R delta = (I / cm) + ((v_rest - v) / tau_m);
+    const R delta = (I * (one / cm)) + ((v_rest - v) * (one / tau_m));
     v = v + delta;
   }
   const bool spike = (!refr && v >= v_thresh) || forced;

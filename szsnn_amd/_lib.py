@@ -80,6 +80,7 @@ class Counters(ctypes.Structure):
         ("spikes", ctypes.c_uint64),
         ("drive_events", ctypes.c_uint64),
         ("overflow", ctypes.c_uint64),
+        ("rows", ctypes.c_uint64),
     ]
 
 

@@ -50,16 +50,22 @@ def izhikevich2003(seed=DEFAULT_SEED, n_exc=800, n_inh=200, fanout=100, max_dela
 
 def _fixed_indegree(rng, n_post, pre_lo, pre_hi, k, exclude_self):
     """k distinct presynaptic ids in [pre_lo, pre_hi) for each target; a
-    target inside the range never draws itself."""
+    target inside the range never draws itself.  Draws k+1 distinct ids, then
+    drops the target itself if present, else one uniformly chosen id: both
+    leave a uniform k-subset of the allowed ids."""
     pool = pre_hi - pre_lo
     tgt = np.arange(n_post)
-    if exclude_self:
-        inside = (tgt >= pre_lo) & (tgt < pre_hi)
-        ex = np.where(inside, tgt - pre_lo, pool)  # pool = "nothing to exclude" sentinel
-        x = distinct_choice(rng, n_post, k, pool + 1, exclude=ex)  # draw from pool+1 minus one
-        # rows outside the range drew from [0, pool] \ {pool}: same as [0, pool)
-    else:
+    if not exclude_self:
         x = distinct_choice(rng, n_post, k, pool)
+        return (x + pre_lo).ravel(), np.repeat(tgt, k)
+    x = distinct_choice(rng, n_post, k + 1, pool)
+    own = np.where((tgt >= pre_lo) & (tgt < pre_hi), tgt - pre_lo, -1)
+    hit = x == own[:, None]
+    has = hit.any(axis=1)
+    drop = np.where(has, hit.argmax(axis=1), rng.integers(0, k + 1, n_post))
+    keep = np.ones(x.shape, dtype=bool)
+    keep[np.arange(n_post), drop] = False
+    x = x[keep].reshape(n_post, k)
     return (x + pre_lo).ravel(), np.repeat(tgt, k)
 
 

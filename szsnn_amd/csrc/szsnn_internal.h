@@ -61,14 +61,14 @@ struct DevNet {
   uint64_t* raster;         // [rec_steps][W] or null
   uint64_t rec_first;
   uint64_t rec_steps;
-  unsigned long long* counters;  // [0]=events [1]=spikes [2]=drive events [3]=overflow
+  unsigned long long* counters;  // [0]=events [1]=spikes [2]=drive events [3]=overflow [4]=rows
   const uint64_t* t_base;   // device cycle counter
   uint32_t volley;          // ids [0, volley) are forced every step
   const uint64_t* forced_bits;   // [W] forced ids this step, or null
 };
 
 // Counter slots.
-enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntSlots = 4 };
+enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntRows = 4, kCntSlots = 5 };
 
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);

@@ -62,7 +62,9 @@ __device__ inline bool if_update(const float* __restrict__ p, float* __restrict_
   const bool refr = fired <= tau_refrac;
   fired = fired + 1.0f;
   if (!refr) {
-    const float delta = (I / cm) + ((v_rest - v) / tau_m);
+    // x / y as x * (1/y), the reference's FP32 division (no correctly
+    // rounded divide in its build, Simulation.ipp:13-21)
+    const float delta = (I * (1.0f / cm)) + ((v_rest - v) * (1.0f / tau_m));
     v = v + delta;
   }
   const bool spike = (!refr && v >= v_thresh) || forced;
@@ -222,7 +224,7 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   const uint32_t slot = (uint32_t)(t % n.R);
   const uint32_t nrows = n.q_count[slot];
   const uint32_t* rows = n.q_rows + (uint64_t)slot * n.q_cap;
-  unsigned long long delivered = 0;
+  unsigned long long delivered = 0, rows_done = 0;
   for (uint32_t chunk = blockIdx.x * BLK; chunk < nrows; chunk += gridDim.x * BLK) {
     const uint32_t r = chunk + threadIdx.x;
     uint32_t start = 0, len = 0;
@@ -245,9 +247,13 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
       atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)w);
     }
     delivered += total;
+    rows_done += min((uint32_t)BLK, nrows - chunk);
     __syncthreads();
   }
-  if (threadIdx.x == 0 && delivered) atomicAdd(&n.counters[kCntEvents], delivered);
+  if (threadIdx.x == 0 && rows_done) {
+    atomicAdd(&n.counters[kCntEvents], delivered);
+    atomicAdd(&n.counters[kCntRows], rows_done);
+  }
 }
 
 __global__ void advance_kernel(uint64_t* t_base, uint64_t k) { *t_base += k; }

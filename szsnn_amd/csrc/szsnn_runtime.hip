@@ -615,6 +615,7 @@ int szsnn_get_counters(szsnn_net* net, szsnn_counters* out) {
   out->spikes = c[kCntSpikes];
   out->drive_events = c[kCntDrive];
   out->overflow = c[kCntOverflow];
+  out->rows = c[kCntRows];
   return SZSNN_OK;
 }
 

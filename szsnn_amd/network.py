@@ -70,10 +70,10 @@ def choose_frac_bits(max_abs_w):
 
 
 def quantize_weights(w, frac_bits):
-    """Round weights to multiples of 2^-F; the result is exactly representable
+    """Round weights (as float32) to multiples of 2^-F; the result is exactly representable
     in float32 and as an int32 multiple of 2^-F, so the fixed-point sum on the
     device and the float weights seen by any checker are the same numbers."""
-    w = np.asarray(w, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float32).astype(np.float64)
     q = np.rint(w * 2.0 ** frac_bits) * 2.0 ** -frac_bits
     out = q.astype(np.float32)
     assert np.all(out.astype(np.float64) == q), "quantised weight not representable in float32"

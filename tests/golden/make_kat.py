@@ -9,7 +9,7 @@ script.  Run once in a container that has the reference at /root/reference:
 Sources (paths relative to the reference checkout):
   szsnn1.1/bin/myfile.txt~       FP32 debug trace of the reference GPU kernel
                                  (IF_curr_exp_update_kernel.cl:238-243,264-270:
-                                 partition 0, neuron 1, one line group per cycle)
+                                 one traced neuron of partition 0, one line group per cycle)
   szsnn1.1/bin/pyNN2neurony.vvv~ FP64 v trace of the same neuron from pyNN.brian
   szsnn1.1/bin/izi1.conn~,
   szsnn1.1/bin/izi2.conn~        Brian connection exports (Connection.hpp:76-145)
@@ -27,17 +27,11 @@ def parse_myfile(path):
     ie_re = re.compile(r">>>>>>IE (\S+) last(\S+) in(\S+) tau(\S+)")
     ii_re = re.compile(r">>>>>>II (\S+) last(\S+) in(\S+) tau(\S+)")
     v_re = re.compile(r">>>>>>V (\S+) last(\S+) rest(\S+) tau(\S+) reset(\S+) I(\S+), cm(\S+), delta(\S+)")
-    fired_re = re.compile(r"^c(\d+)\[test_update\] p0n1 fired")
     cycles = []
     cur = {}
-    fired = set()
     meta = {}
     with open(path, "r", errors="replace") as f:
         for line in f:
-            m = fired_re.match(line)
-            if m:
-                fired.add(int(m.group(1)))
-                continue
             m = ie_re.search(line)
             if m:
                 cur = {"ie": m.group(1), "in_e": m.group(3)}
@@ -60,8 +54,6 @@ def parse_myfile(path):
                 meta["cm"] = m.group(7)
                 cycles.append(cur)
                 cur = {}
-    for c, rec in enumerate(cycles):
-        rec["fired"] = c in fired
     return meta, cycles
 
 
@@ -84,7 +76,7 @@ def main():
         sys.exit("reference checkout not found at " + REF)
     meta, cycles = parse_myfile(os.path.join(REF, "myfile.txt~"))
     out = {
-        "source": "szsnn1.1/bin/myfile.txt~ (reference FP32 GPU debug trace, partition 0 neuron 1)",
+        "source": "szsnn1.1/bin/myfile.txt~ (reference FP32 GPU debug trace of one neuron)",
         "printed_params": meta,
         "note": "v is printed after the update and before any reset; in_e/in_i are the "
                 "gathered currents; values printed with %f",
@@ -92,7 +84,11 @@ def main():
         "in_i": [c["in_i"] for c in cycles],
         "ie": [c["ie"] for c in cycles],
         "v": [c["v"] for c in cycles],
-        "fired_cycles": [k for k, c in enumerate(cycles) if c["fired"]],
+        # The log's "fired" printf lines name other neurons; the traced
+        # neuron's spikes show as a reset: the next cycle starts from v_reset.
+        "fired_cycles": [k for k in range(len(cycles) - 1)
+                         if cycles[k + 1]["v_last"] == meta["v_reset"]
+                         and float(cycles[k]["v"]) >= -50.0],
     }
     with open(os.path.join(HERE, "kat_if_curr_exp_trace.json"), "w") as f:
         json.dump(out, f, indent=0)
@@ -106,7 +102,7 @@ def main():
         with open(os.path.join(REF, name)) as src, \
                 open(os.path.join(HERE, name.rstrip("~")), "w") as dst:
             dst.write(src.read())
-    print("cycles:", len(cycles), "fired:", sum(c["fired"] for c in cycles), "vvv:", len(vvv))
+    print("cycles:", len(cycles), "fired:", len(out["fired_cycles"]), "vvv:", len(vvv))
 
 
 if __name__ == "__main__":
