@@ -119,6 +119,19 @@ int szsnn_net_upload(szsnn_ctx* ctx, const szsnn_net_desc* desc, szsnn_net** out
 int szsnn_net_destroy(szsnn_net* net);
 int szsnn_net_get_info(szsnn_net* net, szsnn_net_info* out);
 
+/* Batched mode: n_instances independent networks of one model (uneven sizes,
+ * own seeds, own parameters) become one device network that every step
+ * kernel covers in a single launch.  Neurons of all instances come first, in
+ * instance order, then their input sources.  Each instance keeps the random
+ * streams of its own seed and local ids, so it evolves exactly as its plain
+ * upload would.  Instances must share model, frac_bits, drive table and
+ * source model.  No reference counterpart: the reference runs one network
+ * per Simulation (Simulation.hpp:109-112). */
+int szsnn_batch_upload(szsnn_ctx* ctx, const szsnn_net_desc* instances, uint32_t n_instances,
+                       szsnn_net** out);
+int szsnn_batch_instance(szsnn_net* net, uint32_t instance, uint32_t* neuron_offset, uint32_t* n_neurons,
+                         uint32_t* source_offset, uint32_t* n_sources);
+
 /* One step: gather -> update(+input model) -> scatter (Simulation.ipp:471-530).
  * forced_ids: ids to fire this step regardless of state (SpikeBuffer::setInput,
  * SpikeBuffer.cpp:97-129; OR-ed into the spike condition, also during the
@@ -152,12 +165,46 @@ int szsnn_reset_counters(szsnn_net* net);
  * the gather and update kernels in milliseconds. */
 int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, double* update_ms);
 
-/* Launch configuration (block sizes and rows per gather block); 0 keeps the
- * current value.  szsnn_autotune times candidates on the live network for
- * `steps` steps each, keeps the fastest, and restores the state it found. */
+/* Launch configuration of the graph mode (update block, gather block and
+ * grid); 0 keeps the current value.  szsnn_autotune times candidates of both
+ * modes on the live network for `steps` steps each, keeps the fastest, and
+ * restores the state it found. */
 int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_block, uint32_t gather_grid);
 int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_block, uint32_t* gather_grid);
+
+/* Execution mode of szsnn_run.  GRAPH: each step is a gather launch and an
+ * update launch, replayed from a captured HIP graph.  PERSISTENT: one
+ * cooperative launch runs all steps, grid barriers between the phases; block
+ * 256/512/1024, grid 0 = the co-resident maximum (clamped to it). */
+#define SZSNN_MODE_GRAPH 0
+#define SZSNN_MODE_PERSISTENT 1
+int szsnn_set_mode(szsnn_net* net, int32_t mode, uint32_t block, uint32_t grid);
+int szsnn_get_mode(szsnn_net* net, int32_t* mode, uint32_t* block, uint32_t* grid);
 int szsnn_autotune(szsnn_net* net, uint32_t steps);
+
+/* ---- discrete adjoint of the IF_curr_exp step (spike pattern held fixed) ----
+ * No reference counterpart (the reference has no gradients); these are the
+ * kernels behind the PyTorch autograd Function szsnn_amd.adjoint.  All
+ * pointers are device pointers, laid out [.][B*H] (trial-major, neuron
+ * fastest); `stream` is a hipStream_t (NULL: default stream); calls are
+ * asynchronous on it.  K <= 64 steps per segment.
+ *   forward_segment : K steps from `state` ([4][B*H]: v, se, si, fired; updated
+ *                     in place) given synaptic inputs AE/AI [K][B*H]; writes
+ *                     h = v - v_rest [K][B*H] and spike bits [K][ceil(B*H/64)].
+ *   backward_segment: adjoint over the segment; ckpt = state at its start;
+ *                     g = dL/dh [K][B*H]; carry [3][B*H] in/out; writes
+ *                     LE/LI = dL/dse, dL/dsi [K][B*H].
+ *   readout_forward : y_t = alpha y_{t-1} + drive_t over [T][BR]; y0 may be NULL.
+ *   readout_backward: lam_t = scale (y_t - target_t) + alpha lam_{t+1}. */
+int szsnn_adj_forward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params, float* state,
+                              const float* AE, const float* AI, float* h, uint64_t* spike_bits);
+int szsnn_adj_backward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params,
+                               const float* ckpt, const uint64_t* spike_bits, const float* g, float* carry,
+                               float* LE, float* LI);
+int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,
+                          float* y);
+int szsnn_readout_backward(void* stream, uint32_t BR, uint32_t T, float alpha, float scale, const float* y,
+                           const float* target, float* lam);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,9 @@ SOURCE_NONE = 0
 SOURCE_CONST_RATE = 1
 SOURCE_POISSON = 2
 
+MODE_GRAPH = 0
+MODE_PERSISTENT = 1
+
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 c_u64p = ctypes.POINTER(ctypes.c_uint64)
 c_u16p = ctypes.POINTER(ctypes.c_uint16)
@@ -99,6 +102,8 @@ PROTOTYPES = [
     ("szsnn_stream", _vp, [_vp]),
     ("szsnn_net_upload", ctypes.c_int, [_vp, ctypes.POINTER(NetDesc), ctypes.POINTER(_vp)]),
     ("szsnn_net_destroy", ctypes.c_int, [_vp]),
+    ("szsnn_batch_upload", ctypes.c_int, [_vp, ctypes.POINTER(NetDesc), ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    ("szsnn_batch_instance", ctypes.c_int, [_vp, ctypes.c_uint32, c_u32p, c_u32p, c_u32p, c_u32p]),
     ("szsnn_net_get_info", ctypes.c_int, [_vp, ctypes.POINTER(NetInfo)]),
     ("szsnn_step", ctypes.c_int, [_vp, c_u32p, ctypes.c_uint32, c_u64p]),
     ("szsnn_run", ctypes.c_int, [_vp, ctypes.c_uint64]),
@@ -115,6 +120,16 @@ PROTOTYPES = [
     ("szsnn_set_tiling", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("szsnn_get_tiling", ctypes.c_int, [_vp, c_u32p, c_u32p, c_u32p]),
     ("szsnn_autotune", ctypes.c_int, [_vp, ctypes.c_uint32]),
+    ("szsnn_set_mode", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("szsnn_adj_forward_segment", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                 _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("szsnn_adj_backward_segment", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("szsnn_readout_forward", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, _vp, _vp,
+                                             _vp]),
+    ("szsnn_readout_backward", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                              ctypes.c_float, _vp, _vp, _vp]),
+    ("szsnn_get_mode", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), c_u32p, c_u32p]),
 ]
 
 _lib = None

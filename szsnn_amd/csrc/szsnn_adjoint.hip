@@ -1,0 +1,206 @@
+// szsnn_adjoint.hip -- discrete adjoint of the IF_curr_exp step with the spike
+// pattern held fixed (no reference counterpart: the reference has no
+// gradients).  Used by the PyTorch autograd Function in szsnn_amd/adjoint.py.
+//
+// Hidden layer, one thread per (trial b, neuron j), state SoA [var][b][j]:
+//   se_t = (1 - 1/tau_e) se_{t-1} + AE_t        (AE_t = W1+^T x_{t-1}, delay 1)
+//   si_t = (1 - 1/tau_i) si_{t-1} + AI_t        (AI_t = W1-^T x_{t-1})
+//   refr_t = fired_{t-1} <= tau_refrac; fired += 1
+//   v'_t = refr_t ? v_{t-1} : v_{t-1} + I*(1/cm) + (v_rest - v_{t-1})*(1/tau_m)
+//   spike_t: v_t = v_reset, fired = 1; else v_t = v'_t
+//   h_t = v_t - v_rest drives the readout
+// (the IF_curr_exp update of IF_curr_exp_update_kernel.cl:230-297 in this
+// library's FP32 evaluation order).
+//
+// Adjoint, backwards over t with masks s_t (spike) and r_t (refractory):
+//   lv_t   = g_t + Cv            (g_t = dL/dh_t from the readout)
+//   lv'_t  = s_t ? 0 : lv_t
+//   lse_t  = lv'_t * (r_t ? 0 : 1/cm) + Ce,  lsi_t likewise with Ci
+//   Cv = lv'_t * (r_t ? 1 : 1 - 1/tau_m);  Ce = (1-1/tau_e) lse_t;  Ci = (1-1/tau_i) lsi_t
+// The carries (Cv, Ce, Ci) cross segment boundaries through `carry`.
+// dL/dW1 = sum_t x_{t-1} (lse_t or lsi_t by the weight's sign) is formed by
+// the caller with library GEMMs.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/szsnn_hip.h"
+
+namespace {
+
+constexpr int kMaxSeg = 64;
+
+// params: [9][H] IF_curr_exp parameters per hidden neuron (per-step units)
+struct HidParams {
+  float v_rest, rc, rt, tau_refrac, ae, ai, i_offset, v_reset, v_thresh;
+};
+
+__device__ inline HidParams load_params(const float* p, uint32_t H, uint32_t j) {
+  HidParams q;
+  q.v_rest = p[0 * H + j];
+  q.rc = 1.0f / p[1 * H + j];
+  q.rt = 1.0f / p[2 * H + j];
+  q.tau_refrac = p[3 * H + j];
+  q.ae = 1.0f - 1.0f / p[4 * H + j];
+  q.ai = 1.0f - 1.0f / p[5 * H + j];
+  q.i_offset = p[6 * H + j];
+  q.v_reset = p[7 * H + j];
+  q.v_thresh = p[8 * H + j];
+  return q;
+}
+
+// Forward over K steps from `state` ([4][B*H]: v, se, si, fired), writing
+// h [K][B*H] and spike bits [K][ceil(B*H/64)] (u64 words).
+__global__ void adj_forward_kernel(uint32_t BH, uint32_t H, uint32_t K, const float* __restrict__ params,
+                                   float* __restrict__ state, const float* __restrict__ AE,
+                                   const float* __restrict__ AI, float* __restrict__ h, uint64_t* __restrict__ bits) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t words = (BH + 63) / 64;
+  const bool live = idx < BH;
+  const uint32_t j = live ? idx % H : 0;
+  HidParams q = load_params(params, H, j);
+  float v = 0, se = 0, si = 0, fired = 0;
+  if (live) {
+    v = state[idx];
+    se = state[BH + idx];
+    si = state[2 * BH + idx];
+    fired = state[3 * BH + idx];
+  }
+  for (uint32_t t = 0; t < K; ++t) {
+    bool spike = false;
+    if (live) {
+      se = (q.ae * se) + AE[(size_t)t * BH + idx];
+      si = (q.ai * si) + AI[(size_t)t * BH + idx];
+      const float I = (se + si) + q.i_offset;
+      const bool refr = fired <= q.tau_refrac;
+      fired = fired + 1.0f;
+      if (!refr) v = v + ((I * q.rc) + ((q.v_rest - v) * q.rt));
+      spike = !refr && v >= q.v_thresh;
+      if (spike) {
+        v = q.v_reset;
+        fired = 1.0f;
+      }
+      h[(size_t)t * BH + idx] = v - q.v_rest;
+    }
+    const uint64_t m = __ballot(spike);
+    if (__lane_id() == 0 && (idx >> 6) < words) bits[(size_t)t * words + (idx >> 6)] = m;
+  }
+  if (live) {
+    state[idx] = v;
+    state[BH + idx] = se;
+    state[2 * BH + idx] = si;
+    state[3 * BH + idx] = fired;
+  }
+}
+
+// Backward over one segment of K steps.  ckpt: state at the segment start
+// ([4][B*H], only `fired` is needed: the masks follow from it and the fixed
+// spike bits).  g: dL/dh [K][B*H].  carry [3][B*H] in/out (Cv, Ce, Ci).
+// LE/LI out: dL/dse_t, dL/dsi_t [K][B*H].
+__global__ void adj_backward_kernel(uint32_t BH, uint32_t H, uint32_t K, const float* __restrict__ params,
+                                    const float* __restrict__ ckpt, const uint64_t* __restrict__ bits,
+                                    const float* __restrict__ g, float* __restrict__ carry,
+                                    float* __restrict__ LE, float* __restrict__ LI) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= BH) return;
+  const uint32_t words = (BH + 63) / 64;
+  const uint32_t j = idx % H;
+  HidParams q = load_params(params, H, j);
+  // recompute the refractory mask of each step from the checkpoint
+  uint64_t rmask = 0, smask = 0;
+  float fired = ckpt[3 * BH + idx];
+  for (uint32_t t = 0; t < K; ++t) {
+    const bool refr = fired <= q.tau_refrac;
+    fired = fired + 1.0f;
+    const bool spike = (bits[(size_t)t * words + (idx >> 6)] >> (idx & 63)) & 1ull;
+    if (spike) fired = 1.0f;
+    rmask |= (uint64_t)refr << t;
+    smask |= (uint64_t)spike << t;
+  }
+  float cv = carry[idx], ce = carry[BH + idx], ci = carry[2 * BH + idx];
+  for (int t = (int)K - 1; t >= 0; --t) {
+    const bool refr = (rmask >> t) & 1ull;
+    const bool spike = (smask >> t) & 1ull;
+    const float lv = g[(size_t)t * BH + idx] + cv;
+    const float lvp = spike ? 0.0f : lv;
+    const float dse = refr ? 0.0f : q.rc;
+    const float lse = lvp * dse + ce;
+    const float lsi = lvp * dse + ci;
+    LE[(size_t)t * BH + idx] = lse;
+    LI[(size_t)t * BH + idx] = lsi;
+    cv = lvp * (refr ? 1.0f : 1.0f - q.rt);
+    ce = q.ae * lse;
+    ci = q.ai * lsi;
+  }
+  carry[idx] = cv;
+  carry[BH + idx] = ce;
+  carry[2 * BH + idx] = ci;
+}
+
+// Leaky readout y_t = alpha y_{t-1} + drive_t, [T][B*R]; y0 the state before step 0.
+__global__ void readout_forward_kernel(uint32_t BR, uint32_t T, float alpha, const float* __restrict__ y0,
+                                       const float* __restrict__ drive, float* __restrict__ y) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= BR) return;
+  float acc = y0 ? y0[idx] : 0.0f;
+  for (uint32_t t = 0; t < T; ++t) {
+    acc = (alpha * acc) + drive[(size_t)t * BR + idx];
+    y[(size_t)t * BR + idx] = acc;
+  }
+}
+
+// lam_t = scale (y_t - target_t) + alpha lam_{t+1}; scale = 2 / (T*B*R) for the mean
+// squared error times the incoming gradient of the loss.
+__global__ void readout_backward_kernel(uint32_t BR, uint32_t T, float alpha, float scale,
+                                        const float* __restrict__ y, const float* __restrict__ target,
+                                        float* __restrict__ lam) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= BR) return;
+  float acc = 0.0f;
+  for (int t = (int)T - 1; t >= 0; --t) {
+    const size_t o = (size_t)t * BR + idx;
+    acc = (scale * (y[o] - target[o])) + (alpha * acc);
+    lam[o] = acc;
+  }
+}
+
+int launch_check(hipError_t e) { return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP; }
+
+}  // namespace
+
+extern "C" {
+
+int szsnn_adj_forward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params, float* state,
+                              const float* AE, const float* AI, float* h, uint64_t* spike_bits) {
+  if (!B || !H || !K || !params || !state || !AE || !AI || !h || !spike_bits) return SZSNN_ERR_INVALID;
+  const uint32_t BH = B * H;
+  const uint32_t blocks = (BH + 255) / 256;
+  adj_forward_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(BH, H, K, params, state, AE, AI, h, spike_bits);
+  return launch_check(hipGetLastError());
+}
+
+int szsnn_adj_backward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params,
+                               const float* ckpt, const uint64_t* spike_bits, const float* g, float* carry,
+                               float* LE, float* LI) {
+  if (!B || !H || !K || K > kMaxSeg || !params || !ckpt || !spike_bits || !g || !carry || !LE || !LI)
+    return SZSNN_ERR_INVALID;
+  const uint32_t BH = B * H;
+  const uint32_t blocks = (BH + 255) / 256;
+  adj_backward_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(BH, H, K, params, ckpt, spike_bits, g, carry, LE, LI);
+  return launch_check(hipGetLastError());
+}
+
+int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,
+                          float* y) {
+  if (!BR || !T || !drive || !y) return SZSNN_ERR_INVALID;
+  readout_forward_kernel<<<(BR + 255) / 256, 256, 0, (hipStream_t)stream>>>(BR, T, alpha, y0, drive, y);
+  return launch_check(hipGetLastError());
+}
+
+int szsnn_readout_backward(void* stream, uint32_t BR, uint32_t T, float alpha, float scale, const float* y,
+                           const float* target, float* lam) {
+  if (!BR || !T || !y || !target || !lam) return SZSNN_ERR_INVALID;
+  readout_backward_kernel<<<(BR + 255) / 256, 256, 0, (hipStream_t)stream>>>(BR, T, alpha, scale, y, target, lam);
+  return launch_check(hipGetLastError());
+}
+
+}  // extern "C"

@@ -73,6 +73,11 @@ enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kC
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);
 hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s);
+// One cooperative launch running nsteps steps (block 256/512/1024; grid must
+// not exceed persistent_max_blocks_per_cu(model, block) * CUs).
+hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint32_t block, uint32_t grid,
+                             hipStream_t s);
+int persistent_max_blocks_per_cu(int model, uint32_t block);
 hipError_t launch_advance(uint64_t* t_base, uint64_t k, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
 

@@ -14,6 +14,8 @@
 // Arithmetic is FP32 with one written evaluation order; the library is built
 // with -ffp-contract=off and IEEE division, so results are bit-identical to
 // the FP32 CPU restatement of the same equations.
+#include <hip/hip_cooperative_groups.h>
+
 #include "szsnn_internal.h"
 
 namespace szsnn {
@@ -100,11 +102,11 @@ __device__ inline bool izh_update(const float* __restrict__ p, float* __restrict
   return spike;
 }
 
+// update(t) for one id (one lane); every lane of the wave must call it
+// (ballots).  Resets the due-queue slot gathered at t when id == 0.
 template <int MODEL>
-__global__ void __launch_bounds__(1024) update_kernel(DevNet n, uint32_t j) {
-  const uint64_t t = *n.t_base + j;
-  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) n.q_count[t % n.R] = 0;  // slot gathered this step
+__device__ inline void update_one(const DevNet& n, uint64_t t, uint32_t id) {
+  if (id == 0) n.q_count[t % n.R] = 0;  // slot gathered this step
   bool spike = false;
   uint64_t bits = 0;
   if (id < n.T) {
@@ -183,6 +185,12 @@ __global__ void __launch_bounds__(1024) update_kernel(DevNet n, uint32_t j) {
   }
 }
 
+template <int MODEL>
+__global__ void __launch_bounds__(1024) update_kernel(DevNet n, uint32_t j) {
+  const uint64_t t = *n.t_base + j;
+  update_one<MODEL>(n, t, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Block-wide exclusive scan of one u32 per thread; returns the total.
 template <int BLK>
 __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, uint32_t* s_wave) {
@@ -212,20 +220,25 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, ui
   return s_pref[BLK];
 }
 
-// gather(t): BLK due rows per chunk; their (start, length) are staged in LDS
-// and scanned, then the block walks the chunk's synapses contiguously so each
-// row segment is read with coalesced 8-byte loads.
 template <int BLK>
-__global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
-  __shared__ uint32_t s_start[BLK];
-  __shared__ uint32_t s_pref[BLK + 1];
-  __shared__ uint32_t s_wave[BLK / 64 + 1];
-  const uint64_t t = *n.t_base + j;
+struct GatherSmem {
+  uint32_t start[BLK];
+  uint32_t pref[BLK + 1];
+  uint32_t wave[BLK / 64 + 1];
+};
+
+// gather(t) for blocks `block` of `nblocks`: BLK due rows per chunk; their
+// (start, length) are staged in LDS and scanned, then the block walks the
+// chunk's synapses contiguously so each row segment is read with coalesced
+// 8-byte loads.
+template <int BLK>
+__device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, uint32_t nblocks,
+                                   GatherSmem<BLK>& sm) {
   const uint32_t slot = (uint32_t)(t % n.R);
   const uint32_t nrows = n.q_count[slot];
   const uint32_t* rows = n.q_rows + (uint64_t)slot * n.q_cap;
   unsigned long long delivered = 0, rows_done = 0;
-  for (uint32_t chunk = blockIdx.x * BLK; chunk < nrows; chunk += gridDim.x * BLK) {
+  for (uint32_t chunk = block * BLK; chunk < nrows; chunk += nblocks * BLK) {
     const uint32_t r = chunk + threadIdx.x;
     uint32_t start = 0, len = 0;
     if (r < nrows) {
@@ -233,15 +246,15 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
       start = n.row_off[row];
       len = n.row_off[row + 1] - start;
     }
-    s_start[threadIdx.x] = start;
-    const uint32_t total = block_exclusive_scan<BLK>(len, s_pref, s_wave);
+    sm.start[threadIdx.x] = start;
+    const uint32_t total = block_exclusive_scan<BLK>(len, sm.pref, sm.wave);
     for (uint32_t e = threadIdx.x; e < total; e += BLK) {
       uint32_t lo = 0, hi = BLK;
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= e) lo = mid; else hi = mid;
+        if (sm.pref[mid] <= e) lo = mid; else hi = mid;
       }
-      const uint2 sw = n.syn[s_start[lo] + (e - s_pref[lo])];
+      const uint2 sw = n.syn[sm.start[lo] + (e - sm.pref[lo])];
       const int32_t w = (int32_t)sw.y;
       long long* dst = n.acc + (w < 0 ? n.npad : 0u) + sw.x;
       atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)w);
@@ -254,6 +267,30 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
     atomicAdd(&n.counters[kCntEvents], delivered);
     atomicAdd(&n.counters[kCntRows], rows_done);
   }
+}
+
+template <int BLK>
+__global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
+  __shared__ GatherSmem<BLK> sm;
+  gather_body<BLK>(n, *n.t_base + j, blockIdx.x, gridDim.x, sm);
+}
+
+// Persistent step loop: one cooperative launch runs `nsteps` steps; grid
+// barriers separate gather(t) from update(t) and update(t) from gather(t+1).
+template <int MODEL, int BLK>
+__global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nsteps, uint64_t* t_base) {
+  __shared__ GatherSmem<BLK> sm;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const uint64_t t0 = *t_base;
+  for (uint64_t s = 0; s < nsteps; ++s) {
+    const uint64_t t = t0 + s;
+    gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
+    grid.sync();
+    for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
+      update_one<MODEL>(n, t, base + threadIdx.x);
+    grid.sync();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
 }
 
 __global__ void advance_kernel(uint64_t* t_base, uint64_t k) { *t_base += k; }
@@ -288,6 +325,61 @@ hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t g
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+namespace {
+template <int MODEL, int BLK>
+hipError_t coop_launch(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint32_t grid, hipStream_t s) {
+  DevNet nn = n;
+  uint64_t ns = nsteps;
+  uint64_t* tb = t_base;
+  void* args[] = {&nn, &ns, &tb};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&persistent_kernel<MODEL, BLK>), dim3(grid),
+                                    dim3(BLK), args, 0, s);
+}
+template <int MODEL, int BLK>
+int coop_max_blocks() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&persistent_kernel<MODEL, BLK>),
+                                                   BLK, 0) != hipSuccess)
+    return 0;
+  return nb;
+}
+}  // namespace
+
+hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint32_t block, uint32_t grid,
+                             hipStream_t s) {
+#define SZSNN_COOP(M)                                                          \
+  switch (block) {                                                             \
+    case 256: return coop_launch<M, 256>(n, nsteps, t_base, grid, s);          \
+    case 512: return coop_launch<M, 512>(n, nsteps, t_base, grid, s);          \
+    case 1024: return coop_launch<M, 1024>(n, nsteps, t_base, grid, s);        \
+    default: return hipErrorInvalidValue;                                      \
+  }
+  switch (n.model) {
+    case kIfCurrExp: SZSNN_COOP(kIfCurrExp)
+    case kIzhikevich: SZSNN_COOP(kIzhikevich)
+    case kIfCondExp: SZSNN_COOP(kIfCondExp)
+    default: return hipErrorInvalidValue;
+  }
+#undef SZSNN_COOP
+}
+
+int persistent_max_blocks_per_cu(int model, uint32_t block) {
+#define SZSNN_OCC(M)                                       \
+  switch (block) {                                         \
+    case 256: return coop_max_blocks<M, 256>();            \
+    case 512: return coop_max_blocks<M, 512>();            \
+    case 1024: return coop_max_blocks<M, 1024>();          \
+    default: return 0;                                     \
+  }
+  switch (model) {
+    case kIfCurrExp: SZSNN_OCC(kIfCurrExp)
+    case kIzhikevich: SZSNN_OCC(kIzhikevich)
+    case kIfCondExp: SZSNN_OCC(kIfCondExp)
+    default: return 0;
+  }
+#undef SZSNN_OCC
 }
 
 hipError_t launch_advance(uint64_t* t_base, uint64_t k, hipStream_t s) {

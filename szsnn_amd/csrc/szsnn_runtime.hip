@@ -44,6 +44,9 @@ struct Tiling {
   uint32_t update_block = 256;
   uint32_t gather_block = 256;
   uint32_t gather_grid = 0;
+  int32_t mode = 0;              // SZSNN_MODE_GRAPH or SZSNN_MODE_PERSISTENT
+  uint32_t persist_block = 256;
+  uint32_t persist_grid = 0;     // 0: the co-resident maximum
 };
 
 struct szsnn_net {
@@ -62,6 +65,8 @@ struct szsnn_net {
   bool graph_dirty = true;
   // autotune snapshot buffers
   std::vector<std::pair<void*, size_t>> snap;
+  // batched networks: per-instance id ranges (empty for a plain network)
+  std::vector<uint32_t> inst_noff, inst_soff, inst_n, inst_m;
 };
 
 namespace {
@@ -158,8 +163,26 @@ int build_graph(szsnn_net* net) {
   return SZSNN_OK;
 }
 
+uint32_t persistent_grid(szsnn_net* net, uint32_t block, uint32_t want) {
+  const int per_cu = persistent_max_blocks_per_cu(net->dn.model, block);
+  const uint32_t maxg = (uint32_t)std::max(per_cu, 0) * (uint32_t)std::max(net->ctx->cu_count, 1);
+  const uint32_t needed = (uint32_t)std::max<uint64_t>(1, ((uint64_t)net->dn.T + block - 1) / block);
+  uint32_t g = want ? want : maxg;
+  g = std::min(g, maxg);
+  g = std::min(g, std::max(needed, (uint32_t)std::max(net->ctx->cu_count, 1)));
+  return g;
+}
+
 int run_steps(szsnn_net* net, uint64_t n) {
   szsnn_ctx* ctx = net->ctx;
+  if (net->tiling.mode == SZSNN_MODE_PERSISTENT) {
+    if (n == 0) return SZSNN_OK;
+    const uint32_t g = persistent_grid(net, net->tiling.persist_block, net->tiling.persist_grid);
+    if (g == 0) return fail(ctx, SZSNN_ERR_INVALID, "persistent mode: block size has no co-resident occupancy");
+    HIP_TRY(ctx, launch_persistent(net->dn, n, net->t_base, net->tiling.persist_block, g, ctx->stream));
+    net->t_host += n;
+    return SZSNN_OK;
+  }
   if (n >= kGraphSteps && net->graph_dirty) {
     int rc = build_graph(net);
     if (rc) return rc;
@@ -280,7 +303,13 @@ int szsnn_net_destroy(szsnn_net* net) {
   return SZSNN_OK;
 }
 
-int szsnn_net_upload(szsnn_ctx* ctx, const szsnn_net_desc* d, szsnn_net** out) {
+}  // extern "C"
+
+namespace {
+
+// Shared body of szsnn_net_upload and szsnn_batch_upload.  keys: per-id
+// random-stream keys, or null to derive them from d->seed.
+int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys, szsnn_net** out) {
   if (!ctx || !d || !out) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: bad argument");
   *out = nullptr;
   // Simulation.ipp:97-101
@@ -420,7 +449,7 @@ int szsnn_net_upload(szsnn_ctx* ctx, const szsnn_net_desc* d, szsnn_net** out) {
   for (int k = 0; k < np; ++k) std::memcpy(&hp[(size_t)k * n.npad], d->params + (size_t)k * N, N * 4);
   for (int k = 0; k < nv; ++k) std::memcpy(&hv[(size_t)k * n.npad], d->vars + (size_t)k * N, N * 4);
   std::vector<uint64_t> hk(T);
-  for (uint64_t i = 0; i < T; ++i) hk[i] = splitmix_at(d->seed, i);
+  for (uint64_t i = 0; i < T; ++i) hk[i] = id_keys ? id_keys[i] : splitmix_at(d->seed, i);
   std::vector<float> hsl;
   if (src_last) {
     hsl.assign(M, 0.0f);
@@ -481,6 +510,128 @@ int szsnn_net_upload(szsnn_ctx* ctx, const szsnn_net_desc* d, szsnn_net** out) {
   net->info.raster_words = n.W;
   net->tiling.gather_grid = default_gather_grid(net, net->tiling.gather_block);
   *out = net;
+  return SZSNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int szsnn_net_upload(szsnn_ctx* ctx, const szsnn_net_desc* d, szsnn_net** out) {
+  return upload_impl(ctx, d, nullptr, out);
+}
+
+int szsnn_batch_upload(szsnn_ctx* ctx, const szsnn_net_desc* inst, uint32_t n_inst, szsnn_net** out) {
+  if (!ctx || !inst || !out || n_inst == 0) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_batch_upload: bad argument");
+  *out = nullptr;
+  const szsnn_net_desc& d0 = inst[0];
+  const int np = n_params_of(d0.model), nv = n_vars_of(d0.model);
+  if (np < 0) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_batch_upload: unknown model");
+  uint64_t N = 0, M = 0, S = 0;
+  uint32_t D = 0;
+  bool noise = false;
+  for (uint32_t b = 0; b < n_inst; ++b) {
+    const szsnn_net_desc& d = inst[b];
+    if (d.model != d0.model || d.frac_bits != d0.frac_bits || d.drive_k != d0.drive_k ||
+        d.drive_w != d0.drive_w || (d.n_sources && d0.n_sources && d.source_model != d0.source_model))
+      return fail(ctx, SZSNN_ERR_INVALID,
+                  "szsnn_batch_upload: instances must share model, frac_bits, drive and source model");
+    if (d.n_neurons == 0) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_batch_upload: instance with 0 neurons");
+    if (d.drive_k && std::memcmp(d.drive_cdf, d0.drive_cdf, d.drive_k * 8) != 0)
+      return fail(ctx, SZSNN_ERR_INVALID, "szsnn_batch_upload: instances must share the drive table");
+    N += d.n_neurons;
+    M += d.n_sources;
+    S += d.n_syn;
+    D = std::max(D, d.max_delay);
+    noise = noise || d.noise_amp;
+  }
+  if (N + M >= 0xffffffffull) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_batch_upload: too many ids");
+  int src_model = kSrcNone;
+  for (uint32_t b = 0; b < n_inst; ++b)
+    if (inst[b].n_sources) src_model = inst[b].source_model;
+  // concatenation: all instances' neurons, then all instances' sources
+  std::vector<float> params((size_t)np * N), vars((size_t)nv * N), amp(noise ? N : 0, 0.0f);
+  std::vector<float> rate(M, 0.0f), last(M, 0.0f);
+  std::vector<uint64_t> thresh(M, 0), keys(N + M);
+  std::vector<uint32_t> ssrc(S), stgt(S);
+  std::vector<float> sw(S);
+  std::vector<uint16_t> sd(S);
+  std::vector<uint32_t> noff(n_inst), soff(n_inst);
+  uint64_t no = 0, so = 0, sy = 0;
+  for (uint32_t b = 0; b < n_inst; ++b) {
+    const szsnn_net_desc& d = inst[b];
+    noff[b] = (uint32_t)no;
+    soff[b] = (uint32_t)(N + so);
+    const uint64_t n = d.n_neurons, m = d.n_sources;
+    for (int k = 0; k < np; ++k) std::memcpy(&params[(size_t)k * N + no], d.params + (size_t)k * n, n * 4);
+    for (int k = 0; k < nv; ++k) std::memcpy(&vars[(size_t)k * N + no], d.vars + (size_t)k * n, n * 4);
+    if (d.noise_amp) std::memcpy(&amp[no], d.noise_amp, n * 4);
+    if (m && d.src_rate) std::memcpy(&rate[so], d.src_rate, m * 4);
+    if (m && d.src_last) std::memcpy(&last[so], d.src_last, m * 4);
+    if (m && d.src_thresh) std::memcpy(&thresh[so], d.src_thresh, m * 8);
+    // keys follow each instance's own seed and local ids, as in its plain run
+    for (uint64_t i = 0; i < n; ++i) keys[no + i] = splitmix_at(d.seed, i);
+    for (uint64_t j = 0; j < m; ++j) keys[N + so + j] = splitmix_at(d.seed, n + j);
+    for (uint64_t q = 0; q < d.n_syn; ++q) {
+      const uint32_t s = d.syn_src[q], t = d.syn_tgt[q];
+      if (s >= n + m || t >= n) {
+        std::ostringstream os;
+        os << "szsnn_batch_upload: instance " << b << " synapse " << q << " connects nonexisting ids";
+        return fail(ctx, SZSNN_ERR_INVALID, os.str());
+      }
+      ssrc[sy] = s < n ? (uint32_t)(no + s) : (uint32_t)(N + so + (s - n));
+      stgt[sy] = (uint32_t)(no + t);
+      sw[sy] = d.syn_w[q];
+      sd[sy] = d.syn_delay[q];
+      ++sy;
+    }
+    no += n;
+    so += m;
+  }
+  szsnn_net_desc c = d0;
+  c.n_neurons = (uint32_t)N;
+  c.n_sources = (uint32_t)M;
+  c.source_model = src_model;
+  c.params = params.data();
+  c.vars = vars.data();
+  c.src_rate = src_model == kSrcConstRate ? rate.data() : nullptr;
+  c.src_last = src_model == kSrcConstRate ? last.data() : nullptr;
+  c.src_thresh = src_model == kSrcPoisson ? thresh.data() : nullptr;
+  c.n_syn = S;
+  c.syn_src = ssrc.data();
+  c.syn_tgt = stgt.data();
+  c.syn_w = sw.data();
+  c.syn_delay = sd.data();
+  c.max_delay = D;
+  c.noise_amp = noise ? amp.data() : nullptr;
+  c.queue_capacity = 0;
+  for (uint32_t b = 0; b < n_inst; ++b) c.queue_capacity += inst[b].queue_capacity;
+  if (c.queue_capacity) {
+    for (uint32_t b = 0; b < n_inst; ++b)
+      if (!inst[b].queue_capacity) c.queue_capacity = 0;  // any worst-case instance: worst case overall
+  }
+  int rc = upload_impl(ctx, &c, keys.data(), out);
+  if (rc) return rc;
+  (*out)->inst_noff = std::move(noff);
+  (*out)->inst_soff = std::move(soff);
+  (*out)->inst_n.resize(n_inst);
+  (*out)->inst_m.resize(n_inst);
+  for (uint32_t b = 0; b < n_inst; ++b) {
+    (*out)->inst_n[b] = inst[b].n_neurons;
+    (*out)->inst_m[b] = inst[b].n_sources;
+  }
+  return SZSNN_OK;
+}
+
+int szsnn_batch_instance(szsnn_net* net, uint32_t b, uint32_t* neuron_offset, uint32_t* n_neurons,
+                         uint32_t* source_offset, uint32_t* n_sources) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_batch_instance: net is NULL");
+  if (b >= net->inst_n.size())
+    return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_batch_instance: instance index out of range");
+  if (neuron_offset) *neuron_offset = net->inst_noff[b];
+  if (n_neurons) *n_neurons = net->inst_n[b];
+  if (source_offset) *source_offset = net->inst_soff[b];
+  if (n_sources) *n_sources = net->inst_m[b];
   return SZSNN_OK;
 }
 
@@ -555,7 +706,7 @@ int szsnn_set_recording(szsnn_net* net, uint64_t rec_first, uint64_t rec_steps) 
   if (net->dn.raster) {
     auto it = std::find(net->allocs.begin(), net->allocs.end(), (void*)net->dn.raster);
     if (it != net->allocs.end()) net->allocs.erase(it);
-    hipFree(net->dn.raster);
+    (void)hipFree(net->dn.raster);
     net->info.device_bytes -= std::max<uint64_t>(net->dn.rec_steps * net->dn.W, 1) * 8;
     net->dn.raster = nullptr;
   }
@@ -648,13 +799,13 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
   if (rc == SZSNN_OK && e == hipSuccess) {
     for (uint64_t s = 0; s < n_steps; ++s) {
       float a = 0, b = 0;
-      hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]);
-      hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]);
+      (void)hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]);
+      (void)hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]);
       g += a;
       u += b;
     }
   }
-  for (auto& x : ev) hipEventDestroy(x);
+  for (auto& x : ev) (void)hipEventDestroy(x);
   if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_time_kernels: ") + hipGetErrorString(e));
   *gather_ms = g;
@@ -672,6 +823,31 @@ int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_bloc
   if (gather_block) net->tiling.gather_block = gather_block;
   net->tiling.gather_grid = gather_grid ? gather_grid : default_gather_grid(net, net->tiling.gather_block);
   net->graph_dirty = true;
+  return SZSNN_OK;
+}
+
+int szsnn_set_mode(szsnn_net* net, int32_t mode, uint32_t block, uint32_t grid) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_mode: net is NULL");
+  if (mode != SZSNN_MODE_GRAPH && mode != SZSNN_MODE_PERSISTENT)
+    return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_mode: unknown mode");
+  if (mode == SZSNN_MODE_PERSISTENT) {
+    if (block == 0) block = net->tiling.persist_block;
+    if (block != 256 && block != 512 && block != 1024)
+      return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_mode: persistent block must be 256, 512 or 1024");
+    if (persistent_grid(net, block, grid) == 0)
+      return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_mode: no co-resident occupancy for that block");
+    net->tiling.persist_block = block;
+    net->tiling.persist_grid = grid;
+  }
+  net->tiling.mode = mode;
+  return SZSNN_OK;
+}
+
+int szsnn_get_mode(szsnn_net* net, int32_t* mode, uint32_t* block, uint32_t* grid) {
+  if (!net || !mode || !block || !grid) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_mode: bad argument");
+  *mode = net->tiling.mode;
+  *block = net->tiling.persist_block;
+  *grid = persistent_grid(net, net->tiling.persist_block, net->tiling.persist_grid);
   return SZSNN_OK;
 }
 
@@ -762,33 +938,49 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
     HIP_TRY(ctx, hipEventElapsedTime(ms, e0, e1));
     return SZSNN_OK;
   };
-  // stage 1: update block at the current gather tiling; stage 2: gather tiling
-  for (uint32_t ub : ublocks) {
-    Tiling tl = best;
-    tl.update_block = ub;
+  auto consider = [&](const Tiling& tl) -> int {
     float ms = 0;
-    if ((rc = trial(tl, &ms))) break;
-    if (ms < best_ms) {
+    int r = trial(tl, &ms);
+    if (r == SZSNN_OK && ms < best_ms) {
       best_ms = ms;
       best = tl;
     }
+    return r;
+  };
+  // stage 1: graph mode, update block at the current gather tiling
+  for (uint32_t ub : ublocks) {
+    Tiling tl = best;
+    tl.mode = SZSNN_MODE_GRAPH;
+    tl.update_block = ub;
+    if ((rc = consider(tl))) break;
   }
+  // stage 2: graph mode, gather block x grid (grid in CU multiples)
   for (uint32_t gb : gblocks) {
-    if (rc) break;
     for (uint32_t gm : gmult) {
+      if (rc) break;
       Tiling tl = best;
+      tl.mode = SZSNN_MODE_GRAPH;
       tl.gather_block = gb;
       tl.gather_grid = (uint32_t)cus * gm;
-      float ms = 0;
-      if ((rc = trial(tl, &ms))) break;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = tl;
-      }
+      rc = consider(tl);
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  // stage 3: persistent mode, block x grid (CUs/4, CUs/2, CUs, co-resident max)
+  const uint32_t pblocks[] = {256, 512, 1024};
+  const uint32_t pdiv[] = {4, 2, 1};
+  for (uint32_t pb : pblocks) {
+    if (persistent_max_blocks_per_cu(net->dn.model, pb) <= 0) continue;
+    for (int gi = 0; gi < 4; ++gi) {
+      if (rc) break;
+      Tiling tl = best;
+      tl.mode = SZSNN_MODE_PERSISTENT;
+      tl.persist_block = pb;
+      tl.persist_grid = gi < 3 ? std::max<uint32_t>(1, (uint32_t)cus / pdiv[gi]) : 0;
+      rc = consider(tl);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   net->tiling = best;
   free_graph(net);
   int r2 = snapshot(net, true);

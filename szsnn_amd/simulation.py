@@ -190,9 +190,18 @@ class Simulation:
               self.ctx.handle)
         return {"update_block": a.value, "gather_block": b.value, "gather_grid": c.value}
 
+    def set_mode(self, mode, block=0, grid=0):
+        check(self.lib.szsnn_set_mode(self.handle, int(mode), int(block), int(grid)), self.ctx.handle)
+
+    def mode(self):
+        m, b, g = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(self.lib.szsnn_get_mode(self.handle, ctypes.byref(m), ctypes.byref(b), ctypes.byref(g)),
+              self.ctx.handle)
+        return {"mode": m.value, "block": b.value, "grid": g.value}
+
     def autotune(self, steps=100):
         check(self.lib.szsnn_autotune(self.handle, int(steps)), self.ctx.handle)
-        return self.tiling()
+        return dict(self.tiling(), **self.mode())
 
     def close(self):
         if getattr(self, "handle", None):

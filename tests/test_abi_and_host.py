@@ -129,3 +129,16 @@ def test_if_cond_exp_builder_small():
     assert np.all(net.syn_w[net.syn_src >= 800] < 0)
     assert net.drive_cdf is not None and len(net.drive_cdf) >= 5
     assert net.params.shape == (11, 1000) and net.params[5, 0] == 100.0  # tau_syn_I 10 ms / 0.1
+
+
+def test_batched_izhikevich_builder():
+    from szsnn_amd.batch import batched_izhikevich
+    nets = batched_izhikevich(n_instances=16, fanout=5)
+    sizes = np.array([n.n_neurons for n in nets])
+    assert len(nets) == 16 and sizes.min() >= 500 and sizes.max() <= 20000
+    assert len({n.seed for n in nets}) == 16
+    for n in nets:
+        assert set(np.unique(n.noise_amp)) <= set(np.unique(n.noise_amp))
+        amps = np.unique(n.noise_amp)
+        assert len(amps) == 2 and np.all((amps >= 2) & (amps <= 8))
+        assert n.meta["n_exc"] == round(0.8 * n.n_neurons)

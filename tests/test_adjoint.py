@@ -1,0 +1,179 @@
+"""Discrete adjoint of the feedforward IF_curr_exp network.
+
+CPU: the FP64 torch restatement (oracle/adjoint_ref.py) is pinned to the C++
+oracle's IF_curr_exp trace, and its fixed-spike autograd gradient to central
+finite differences of the free-running forward (perturbations checked not to
+flip any spike).
+GPU: the HIP adjoint's dL/dW1, dL/dW2 against the FP64 autograd reference
+with the GPU's spike pattern held fixed (relative L2 error <= 1e-4), and
+against central finite differences.
+"""
+import numpy as np
+import pytest
+import torch
+
+REL_TOL = 1e-4
+
+
+def _to64(P, keys=("W1", "W2", "X", "target", "params")):
+    return {k: torch.tensor(P[k], dtype=torch.float64) for k in keys}
+
+
+def test_fp64_reference_matches_oracle_neuron_trace(oracle_mod):
+    from oracle import adjoint_ref
+    from szsnn_amd.adjoint import make_problem
+    P = make_problem(batch=3, n_in=50, n_hidden=20, n_out=4, steps=80, w1_mean=0.3, w1_std=0.5)
+    d = _to64(P)
+    _, sp, v = adjoint_ref.forward(d["W1"], d["W2"], d["X"], d["target"], d["params"], P["alpha"],
+                                   return_v=True)
+    assert sp.any()
+    W1p = d["W1"].clamp(min=0)
+    W1n = d["W1"].clamp(max=0)
+    for b, j in ((0, 0), (1, 7), (2, 19)):
+        xprev = torch.cat([torch.zeros(1, 50, dtype=torch.float64), d["X"][:-1, b]], 0)
+        ae = (xprev @ W1p[:, j]).numpy()
+        ai = (xprev @ W1n[:, j]).numpy()
+        tr, spk = oracle_mod.neuron_trace(0, 64, P["params"][:, j].astype(np.float64),
+                                          [P["params"][0, j], 0, 0, 10000.0], ae, ai)
+        np.testing.assert_allclose(v[:, b, j].numpy(), tr[:, 0], rtol=0, atol=1e-10)
+        np.testing.assert_array_equal(sp[:, b, j].numpy(), spk)
+
+
+def test_fp64_autograd_matches_finite_differences():
+    from oracle import adjoint_ref
+    from szsnn_amd.adjoint import make_problem
+    P = make_problem(batch=4, n_in=60, n_hidden=30, n_out=5, steps=60, w1_mean=0.3, w1_std=0.5)
+    d = _to64(P)
+    W1 = d["W1"].clone().requires_grad_(True)
+    W2 = d["W2"].clone().requires_grad_(True)
+    loss, sp = adjoint_ref.forward(W1, W2, d["X"], d["target"], d["params"], P["alpha"])
+    assert sp.any() and not sp.all()
+    loss_fixed, _ = adjoint_ref.forward(W1, W2, d["X"], d["target"], d["params"], P["alpha"], spikes=sp)
+    loss_fixed.backward()
+    rng = np.random.default_rng(0)
+    checked = 0
+    for _ in range(40):
+        which = rng.integers(0, 2)
+        W = d["W1"] if which == 0 else d["W2"]
+        i, j = (int(rng.integers(0, W.shape[0])), int(rng.integers(0, W.shape[1])))
+        eps = 1e-6
+        vals = []
+        for sgn in (1, -1):
+            Wp = W.clone()
+            Wp[i, j] += sgn * eps
+            args = (Wp, d["W2"]) if which == 0 else (d["W1"], Wp)
+            l, s2 = adjoint_ref.forward(*args, d["X"], d["target"], d["params"], P["alpha"])
+            if not torch.equal(s2, sp):
+                break
+            vals.append(float(l))
+        if len(vals) < 2:
+            continue  # a spike flipped: not a valid finite-difference point
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        g = float((W1.grad if which == 0 else W2.grad)[i, j])
+        assert abs(fd - g) <= 1e-6 * max(1.0, abs(g)) + 1e-7, (which, i, j, fd, g)
+        checked += 1
+    assert checked >= 20
+
+
+@pytest.fixture(scope="module")
+def cuda():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _gpu_grads(P, device, K=20):
+    from szsnn_amd.adjoint import snn_loss
+    t = {k: torch.tensor(P[k], dtype=torch.float32, device=device).contiguous()
+         for k in ("W1", "W2", "X", "target", "params")}
+    W1 = t["W1"].clone().requires_grad_(True)
+    W2 = t["W2"].clone().requires_grad_(True)
+    loss, y, bits = snn_loss(W1, W2, t["X"], t["target"], t["params"], P["alpha"], K=K)
+    loss.backward()
+    torch.cuda.synchronize()
+    return float(loss), W1.grad.cpu().double(), W2.grad.cpu().double(), bits.cpu()
+
+
+def _bits_to_spikes(bits, T, B, H):
+    b = bits.numpy().view(np.uint64)
+    u = np.unpackbits(b.view(np.uint8).reshape(T, -1), axis=1, bitorder="little")[:, :B * H]
+    return torch.tensor(u.reshape(T, B, H).astype(bool))
+
+
+def _ref_grads(P, spikes):
+    from oracle import adjoint_ref
+    d = _to64(P)
+    W1 = d["W1"].clone().requires_grad_(True)
+    W2 = d["W2"].clone().requires_grad_(True)
+    loss, _ = adjoint_ref.forward(W1, W2, d["X"], d["target"], d["params"], P["alpha"], spikes=spikes)
+    loss.backward()
+    return float(loss), W1.grad, W2.grad
+
+
+def _rel(a, b):
+    return float((a - b).norm() / max(float(b.norm()), 1e-30))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["spec", "spiking"])
+def test_hip_adjoint_matches_fp64_autograd(cuda, variant):
+    from szsnn_amd.adjoint import make_problem
+    if variant == "spec":
+        P = make_problem()  # B=64, 1000 -> 2000 -> 100, T=200, W1 ~ N(0, 0.02)
+    else:
+        P = make_problem(batch=16, n_hidden=1000, w1_mean=0.02, w1_std=0.1)
+    T, B, _ = P["X"].shape
+    H = P["W1"].shape[1]
+    print(f"case: adjoint {variant} T={T} B={B} H={H}", flush=True)
+    loss, g1, g2, bits = _gpu_grads(P, cuda)
+    spikes = _bits_to_spikes(bits, T, B, H)
+    if variant == "spiking":
+        assert spikes.float().mean() > 0.02
+    rloss, r1, r2 = _ref_grads(P, spikes)
+    assert abs(loss - rloss) <= 1e-5 * abs(rloss)
+    assert _rel(g1, r1) <= REL_TOL, _rel(g1, r1)
+    assert _rel(g2, r2) <= REL_TOL, _rel(g2, r2)
+
+
+@pytest.mark.gpu
+def test_hip_adjoint_checkpoint_interval_invariant(cuda):
+    from szsnn_amd.adjoint import make_problem
+    P = make_problem(batch=8, n_hidden=500, steps=90, w1_mean=0.02, w1_std=0.1)
+    _, a1, a2, _ = _gpu_grads(P, cuda, K=20)
+    _, b1, b2, _ = _gpu_grads(P, cuda, K=7)
+    _, c1, c2, _ = _gpu_grads(P, cuda, K=64)
+    assert torch.equal(a1, b1) and torch.equal(a1, c1)
+    assert torch.equal(a2, b2) and torch.equal(a2, c2)
+
+
+@pytest.mark.gpu
+def test_hip_adjoint_matches_finite_differences(cuda):
+    from oracle import adjoint_ref
+    from szsnn_amd.adjoint import make_problem
+    P = make_problem(batch=8, n_hidden=300, n_in=200, n_out=20, steps=100, w1_mean=0.05, w1_std=0.2)
+    _, g1, g2, bits = _gpu_grads(P, cuda)
+    d = _to64(P)
+    _, sp = adjoint_ref.forward(d["W1"], d["W2"], d["X"], d["target"], d["params"], P["alpha"])
+    rng = np.random.default_rng(3)
+    checked = 0
+    for _ in range(30):
+        which = int(rng.integers(0, 2))
+        W = d["W1"] if which == 0 else d["W2"]
+        i, j = int(rng.integers(0, W.shape[0])), int(rng.integers(0, W.shape[1]))
+        eps = 1e-5
+        vals = []
+        for sgn in (1, -1):
+            Wp = W.clone()
+            Wp[i, j] += sgn * eps
+            args = (Wp, d["W2"]) if which == 0 else (d["W1"], Wp)
+            l, s2 = adjoint_ref.forward(*args, d["X"], d["target"], d["params"], P["alpha"])
+            if not torch.equal(s2, sp):
+                break
+            vals.append(float(l))
+        if len(vals) < 2:
+            continue
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        g = float((g1 if which == 0 else g2)[i, j])
+        assert abs(fd - g) <= 1e-3 * max(abs(fd), 1e-3), (which, i, j, fd, g)
+        checked += 1
+    assert checked >= 15

@@ -256,3 +256,27 @@ def test_full_size_if_cond_exp_event_checksum(ctx):
     assert cnt["events"] == exp
     assert cnt["spikes"] == int(spikes.sum())
     assert cnt["overflow"] == 0
+
+
+@pytest.mark.parametrize("block,grid", [(256, 0), (512, 16), (1024, 4)])
+def test_persistent_mode_equals_graph_mode(ctx, block, grid):
+    """The cooperative persistent step loop gives the same rasters and state
+    as the graph-captured per-kernel loop."""
+    from szsnn_amd._lib import MODE_PERSISTENT
+    from szsnn_amd.configs import if_curr_exp_10k, izhikevich2003
+    from szsnn_amd.simulation import Simulation
+    for net, steps in ((izhikevich2003(n_exc=1600, n_inh=400), 300),
+                       (if_curr_exp_10k(n_exc=2000, n_inh=500, n_poisson=300, poisson_hz=100.0,
+                                        w_poisson=1.5), 500)):
+        a = Simulation(ctx, net)
+        a.set_recording(0, steps)
+        a.run(steps)
+        b = Simulation(ctx, net)
+        b.set_mode(MODE_PERSISTENT, block, grid)
+        b.set_recording(0, steps)
+        b.run(steps // 3)
+        b.run(steps - steps // 3)
+        np.testing.assert_array_equal(a.raster(), b.raster())
+        np.testing.assert_array_equal(a.get_vars(), b.get_vars())
+        ca, cb = a.counters(), b.counters()
+        assert ca["events"] == cb["events"] and cb["cycle"] == steps

@@ -1,0 +1,102 @@
+"""The C++ host layer (szsnn_amd/host/szsnn.hpp) driven like the reference's
+main.cpp: a .conn file (tests/golden/izi1.conn, a Brian export shipped with
+the reference) plus Const_rate inputs, stepAndFired with forced input, and a
+SpikeRecorder file."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "host_api_demo.bin")
+CONN = os.path.join(ROOT, "tests", "golden", "izi1.conn")
+
+
+def _binary():
+    if not os.path.exists(BIN):
+        pytest.skip("host_api_demo.bin not built (run __graft_entry__.build())")
+    return BIN
+
+
+def test_cpp_driver_reports_missing_device_as_exception():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    r = subprocess.run([_binary(), CONN, "10", "/tmp/szsnn_spikes.txt"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True)
+    assert r.returncode == 3
+    assert "SZSNN::Context::setup() error" in r.stderr
+
+
+def load_conn(path):
+    """Connection::loadFromFile (Connection.hpp:76-145): skip two header lines,
+    then 4 whitespace-separated values per line; ids and delay parsed as float."""
+    rows = []
+    with open(path) as f:
+        f.readline()
+        f.readline()
+        for line in f:
+            v = line.split()
+            if not v:
+                continue
+            assert len(v) == 4
+            rows.append((int(float(v[0])), int(float(v[1])), float(v[2]), int(float(v[3]))))
+    return rows
+
+
+def demo_network():
+    """The network host_api_demo.cpp builds, as a Network for the oracle."""
+    from szsnn_amd.network import Network, quantize_weights
+    import math
+    n = 100
+    a = np.full(n, 0.02, np.float32)
+    b = np.full(n, 0.2, np.float32)
+    c = np.full(n, -65.0, np.float32)
+    d = np.where(np.arange(n) < 80, 8.0, 2.0).astype(np.float32)
+    conn = {}
+    for s, t, w, k in load_conn(CONN):  # std::map keyed by (src, tgt): last wins, sorted
+        conn[(s, t)] = (np.float32(np.float32(w) * np.float32(5e10)), k)
+    src, tgt, w, dl = [], [], [], []
+    for (s, t), (ww, k) in sorted(conn.items()):
+        src.append(s), tgt.append(t), w.append(ww), dl.append(k)
+    for j in range(3):
+        for t in range(20):
+            src.append(n + j), tgt.append((j * 31 + t * 7) % 100), w.append(np.float32(12.0)), dl.append(j)
+    w = np.array(w, np.float32)
+    maxw = float(np.abs(w).max())
+    F = min(40, 30 - math.ceil(math.log2(maxw)))
+    return Network(model=1, n_neurons=n, params=np.stack([a, b, c, d]),
+                   vars=np.stack([np.full(n, -65.0, np.float32), np.full(n, -13.0, np.float32)]),
+                   syn_src=np.array(src, np.uint32), syn_tgt=np.array(tgt, np.uint32),
+                   syn_w=quantize_weights(w, F), syn_delay=np.array(dl, np.uint16), max_delay=max(dl) + 1,
+                   frac_bits=F, seed=7, n_sources=3, source_model=1,
+                   src_rate=np.array([5, 8, 11], np.float32), src_last=np.zeros(3, np.float32)).validate()
+
+
+def test_conn_fixture_parses():
+    rows = load_conn(CONN)
+    assert len(rows) == 77 and all(k == 0 for *_, k in rows)
+    assert rows[0] == (0, 13, 2e-10, 0)
+
+
+@pytest.mark.gpu
+def test_cpp_driver_matches_oracle(tmp_path):
+    from oracle.pyoracle import Oracle
+    steps = 300
+    out = tmp_path / "spikes.txt"
+    r = subprocess.run([_binary(), CONN, str(steps), str(out)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = out.read_text().splitlines()
+    assert lines[0] == "#spikes by cycles" and lines[1].startswith("#record start at: 0")
+    got = np.zeros((steps, 100), bool)
+    for ln in lines[2:]:
+        cyc, nid = ln.split("\t")
+        got[int(cyc), int(nid)] = True
+    net = demo_network()
+    forced = [(t, i) for t in range(steps) if t % 25 == 3 for i in (1, 50, 99)]
+    raster, _ = Oracle(net, 32).run(steps, forced=forced)
+    bits = np.unpackbits(raster.view(np.uint8).reshape(steps, -1), axis=1, bitorder="little")[:, :100]
+    np.testing.assert_array_equal(got, bits.astype(bool))
+    assert got.sum() > 50
