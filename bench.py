@@ -31,11 +31,68 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3000, help="CPU baseline sample (steps of the same net)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="scaled-down network (smoke use only)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--autotune", action="store_true", help="autotune the plain network before timing")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="counter pass: a 1 GiB device copy (calibration) then volley steps only")
     return ap.parse_args()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def run_batched(args, ctx, log):
+    """Batched Izhikevich instances (log-uniform sizes 500..20000, own seeds and
+    noise amplitudes) in one launch vs the same instances run one after
+    another as plain networks on the same GPU.  Rasters must match per instance."""
+    import numpy as np
+    from szsnn_amd.batch import BatchSimulation, batched_izhikevich
+    from szsnn_amd.simulation import Simulation, raster_to_bool
+
+    t0 = time.time()
+    nets = batched_izhikevich(n_instances=args.batch_instances)
+    n_tot = sum(n.n_neurons for n in nets)
+    s_tot = sum(n.n_syn for n in nets)
+    log(f"[bench] batched: {len(nets)} instances, {n_tot} neurons, {s_tot} synapses built in {time.time() - t0:.1f}s")
+    T = args.batch_steps
+    bs = BatchSimulation(ctx, nets)
+    tiling = bs.autotune(100)
+    log(f"[bench] batched tiling: {tiling}")
+    bs.set_recording(0, T)
+    ctx.synchronize()
+    c0 = bs.counters()
+    t1 = time.perf_counter()
+    bs.run(T)
+    ctx.synchronize()
+    wall_b = time.perf_counter() - t1
+    ev_b = bs.counters()["events"] - c0["events"]
+    rb = bs.raster()
+    per_inst = [bs.instance_raster(b, rb) for b in range(len(nets))]
+    del bs, rb
+    wall_p = 0.0
+    ev_p = 0
+    same = True
+    for b, net in enumerate(nets):
+        s = Simulation(ctx, net)
+        s.set_recording(0, T)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        s.run(T)
+        ctx.synchronize()
+        wall_p += time.perf_counter() - t1
+        ev_p += s.counters()["events"]
+        same = same and np.array_equal(raster_to_bool(s.raster(), net.n_neurons), per_inst[b])
+        del s
+    out = {"instances": len(nets), "neurons": n_tot, "synapses": s_tot, "steps": T, "dt_ms": 1.0,
+           "batched_events_per_s": ev_b / wall_b, "plain_sequential_events_per_s": ev_p / wall_p,
+           "speedup": (ev_b / wall_b) / (ev_p / wall_p), "batched_wall_s": wall_b,
+           "plain_sequential_wall_s": wall_p, "events": ev_b, "rasters_identical": bool(same and ev_b == ev_p),
+           "tiling": tiling}
+    log(f"[bench] batched: {out}")
+    return out
 
 
 def main():
@@ -62,6 +119,22 @@ def main():
     sim = Simulation(ctx, net)
     log(f"[bench] uploaded in {time.time() - t0:.1f}s to {ctx.name}; device bytes {sim.info.device_bytes}")
 
+    if args.profile_pass:
+        # calibration: a plain copy of known size (FETCH_SIZE / WRITE_SIZE of 1 GiB each)
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        sim.set_volley(net.n_neurons)
+        sim.run(net.max_delay + 2)
+        g_ms, u_ms = sim.time_kernels(args.volley_steps)
+        log(f"[bench] profile pass: volley gather {g_ms * 1e3 / args.volley_steps:.1f} us/launch")
+        return
+    if args.autotune:
+        log(f"[bench] autotuned: {sim.autotune(100)}")
     # warmup (also builds the step graph)
     sim.run(args.warmup)
     torch.cuda.synchronize()
@@ -113,6 +186,9 @@ def main():
     log(f"[bench] volley gather: {g_avg_s * 1e6:.1f} us/launch, {v_events / args.volley_steps:.3e} events/launch,"
         f" {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak); update {uv_ms * 1e3 / args.volley_steps:.1f} us")
 
+    del sim
+    batched = None if args.no_batched else run_batched(args, ctx, log)
+
     cpu = None
     if not args.no_cpu:
         from oracle.pyoracle import Oracle
@@ -152,6 +228,7 @@ def main():
                      "kernel": "gather_kernel (volley: all 100k neurons forced each step)",
                      "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
 
