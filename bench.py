@@ -171,20 +171,29 @@ def main():
     log(f"[bench] natural kernels: {natural}")
 
     # volley: every neuron forced every step -> every (source, delay) row is due
-    sim.set_volley(net.n_neurons)
-    sim.run(net.max_delay + 2)  # fill all delay slots
-    cv0 = sim.counters()
-    gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
-    cv1 = sim.counters()
-    sim.set_volley(0)
-    v_events = cv1["events"] - cv0["events"]
-    v_rows = cv1["rows"] - cv0["rows"]
-    alg_bytes = 8 * v_events + 8 * v_rows
-    per_launch = alg_bytes / args.volley_steps
-    g_avg_s = gv_ms * 1e-3 / args.volley_steps
-    achieved = per_launch / g_avg_s / 1e9
-    log(f"[bench] volley gather: {g_avg_s * 1e6:.1f} us/launch, {v_events / args.volley_steps:.3e} events/launch,"
-        f" {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak); update {uv_ms * 1e3 / args.volley_steps:.1f} us")
+    def volley(fraction):
+        sim.set_gather_policy(fraction)
+        sim.set_volley(net.n_neurons)
+        sim.run(net.max_delay + 2)  # fill all delay slots
+        cv0 = sim.counters()
+        gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
+        cv1 = sim.counters()
+        sim.set_volley(0)
+        ev = cv1["events"] - cv0["events"]
+        rw = cv1["rows"] - cv0["rows"]
+        per = (8 * ev + 8 * rw) / args.volley_steps
+        g_s = gv_ms * 1e-3 / args.volley_steps
+        return {"gather_us": g_s * 1e6, "update_us": uv_ms * 1e3 / args.volley_steps,
+                "events_per_launch": ev / args.volley_steps, "rows_per_launch": rw / args.volley_steps,
+                "alg_bytes_per_launch": per, "gbs": per / g_s / 1e9}
+    v_push = volley(2.0)
+    log(f"[bench] volley push-only: {v_push}")
+    v_auto = volley(0.3)
+    log(f"[bench] volley auto (pull): {v_auto}")
+    sim.set_gather_policy(0.3)
+    per_launch = v_auto["alg_bytes_per_launch"]
+    g_avg_s = v_auto["gather_us"] * 1e-6
+    achieved = v_auto["gbs"]
 
     del sim
     batched = None if args.no_batched else run_batched(args, ctx, log)
@@ -223,6 +232,7 @@ def main():
         "firing_rate_hz": rate_hz,
         "events_per_step": events / args.steps,
         "natural_kernels": natural,
+        "volley_push_only": v_push,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "gather_kernel (volley: all 100k neurons forced each step)",

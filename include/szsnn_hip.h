@@ -80,6 +80,8 @@ typedef struct szsnn_net_desc {
   const uint64_t* drive_cdf;  /* [drive_k] events = #{j : draw(key, t, 2) >= cdf[j]} */
   float drive_w;              /* weight of one drive event, E channel */
   uint32_t queue_capacity;    /* due-queue entries per slot; 0 = worst case (never overflows) */
+  int32_t pull_layout;        /* 0: also build the target-blocked layout for dense gathers when the
+                                 network allows it (ids < 2^24, bounded source spans); -1: never */
 } szsnn_net_desc;
 
 typedef struct szsnn_net_info {
@@ -171,6 +173,15 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
  * restores the state it found. */
 int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_block, uint32_t gather_grid);
 int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_block, uint32_t* gather_grid);
+
+/* Gather policy: each step gathers by pushing the due rows (cost ~ events)
+ * or, with the pull layout, by pulling every synapse of each target block
+ * against the staged spike bitmaps (cost ~ synapses, no global atomics).  It
+ * pulls when the due rows are at least dense_fraction x the nonempty rows;
+ * dense_fraction <= 0 always pulls, > 1 never does.  Both give identical sums.
+ * has_pull reports whether the layout was built. */
+int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction);
+int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_pull);
 
 /* Execution mode of szsnn_run.  GRAPH: each step is a gather launch and an
  * update launch, replayed from a captured HIP graph.  PERSISTENT: one

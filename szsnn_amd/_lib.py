@@ -57,6 +57,7 @@ class NetDesc(ctypes.Structure):
         ("drive_cdf", c_u64p),
         ("drive_w", ctypes.c_float),
         ("queue_capacity", ctypes.c_uint32),
+        ("pull_layout", ctypes.c_int32),
     ]
 
 
@@ -121,6 +122,8 @@ PROTOTYPES = [
     ("szsnn_get_tiling", ctypes.c_int, [_vp, c_u32p, c_u32p, c_u32p]),
     ("szsnn_autotune", ctypes.c_int, [_vp, ctypes.c_uint32]),
     ("szsnn_set_mode", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("szsnn_set_gather_policy", ctypes.c_int, [_vp, ctypes.c_float]),
+    ("szsnn_get_gather_policy", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]),
     ("szsnn_adj_forward_segment", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                  _vp, _vp, _vp, _vp, _vp, _vp]),
     ("szsnn_adj_backward_segment", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,

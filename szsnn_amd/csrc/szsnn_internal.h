@@ -65,7 +65,26 @@ struct DevNet {
   const uint64_t* t_base;   // device cycle counter
   uint32_t volley;          // ids [0, volley) are forced every step
   const uint64_t* forced_bits;   // [W] forced ids this step, or null
+  uint64_t* hist;           // [R][W] spike bitmaps of the last R steps (slot t % R)
+  // pull layout (dense gather): target blocks of kPullTB neurons; synapses of
+  // block b and delay k at pull_syn[pull_off[b*D+k] .. pull_off[b*D+k+1]),
+  // sorted by target, each (source << kPullShift | target % kPullTB, weight);
+  // segments padded to even length with inert (weight 0) entries.
+  const uint32_t* pull_off; // [NB*D + 1] or null (no pull layout)
+  const uint2* pull_syn;
+  const uint32_t* pull_lo;  // [NB] first raster word of the block's sources
+  uint32_t pull_nb;         // NB
+  uint32_t pull_words;      // LDS words staged per block (max over blocks)
+  uint32_t dense_rows;      // gather pulls when the due rows >= this (0xffffffff: never)
 };
+
+#ifndef SZSNN_PULL_CONSTANTS
+#define SZSNN_PULL_CONSTANTS
+constexpr uint32_t kPullShift = 8;
+constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block
+constexpr uint32_t kPullMaxWords = 6144;         // 48 KiB of staged bitmap (sources spanning 393k ids)
+constexpr uint32_t kPullRun = 8;                 // consecutive synapses per lane
+#endif
 
 // Counter slots.
 enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntRows = 4, kCntSlots = 5 };
@@ -77,7 +96,9 @@ hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_
 // not exceed persistent_max_blocks_per_cu(model, block) * CUs).
 hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint32_t block, uint32_t grid,
                              hipStream_t s);
-int persistent_max_blocks_per_cu(int model, uint32_t block);
+int persistent_max_blocks_per_cu(const DevNet& n, uint32_t block);
+// dynamic LDS bytes of a gather launch (pull path)
+size_t pull_dyn_bytes(const DevNet& n);
 hipError_t launch_advance(uint64_t* t_base, uint64_t k, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
 

@@ -154,6 +154,7 @@ __device__ inline void update_one(const DevNet& n, uint64_t t, uint32_t id) {
   const uint32_t lane = __lane_id();
   const uint32_t word = id >> 6;
   if (lane == 0 && word < n.W) {
+    n.hist[(t % n.R) * n.W + word] = mask;
     if (n.raster && t >= n.rec_first && t - n.rec_first < n.rec_steps)
       n.raster[(t - n.rec_first) * n.W + word] = mask;
     if (mask) atomicAdd(&n.counters[kCntSpikes], (unsigned long long)__popcll(mask));
@@ -235,7 +236,9 @@ template <int BLK>
 __device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, uint32_t nblocks,
                                    GatherSmem<BLK>& sm) {
   const uint32_t slot = (uint32_t)(t % n.R);
-  const uint32_t nrows = n.q_count[slot];
+  // after an overflow the count runs past the slot's capacity (entries beyond
+  // it were never written): read only what is there
+  const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[slot], n.q_cap);
   const uint32_t* rows = n.q_rows + (uint64_t)slot * n.q_cap;
   unsigned long long delivered = 0, rows_done = 0;
   for (uint32_t chunk = block * BLK; chunk < nrows; chunk += nblocks * BLK) {
@@ -269,10 +272,82 @@ __device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, 
   }
 }
 
+// Dense gather (pull) for target block `b`: for each delay k the spike
+// bitmap of step t-k-1 (the block's source range) is staged in LDS; each lane
+// walks kPullRun consecutive synapses of the (b, k) segment, keeps a running
+// sum while the target stays the same, and adds it into the block's LDS
+// accumulators when the target changes.  The block then stores its targets'
+// complete sums: no global atomics.  dyn: kPullWords u64 bitmap + 2*kPullTB i64.
+template <int BLK>
+__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint64_t* dyn) {
+  uint64_t* bits = dyn;
+  long long* acc = reinterpret_cast<long long*>(dyn + n.pull_words);
+  for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) acc[i] = 0;
+  const uint32_t lo = n.pull_lo[b];
+  const uint32_t nw = min(n.pull_words, n.W - lo);
+  unsigned long long events = 0;
+  for (uint32_t k = 0; k < n.D; ++k) {
+    if (t < (uint64_t)k + 1) break;
+    const uint32_t s0 = n.pull_off[b * n.D + k], s1 = n.pull_off[b * n.D + k + 1];
+    if (s0 == s1) continue;
+    const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
+    __syncthreads();  // the previous delay's readers are done with `bits`
+    for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = src[w];
+    __syncthreads();
+    const uint32_t base_id = lo * 64;
+    for (uint32_t e0 = s0 + threadIdx.x * kPullRun; e0 < s1; e0 += BLK * kPullRun) {
+      const uint32_t e1 = min(e0 + kPullRun, s1);
+      uint32_t cur = 0xffffffffu;
+      long long se = 0, si = 0;
+      for (uint32_t e = e0; e < e1; ++e) {
+        const uint2 p = n.pull_syn[e];
+        const uint32_t tl = p.x & (kPullTB - 1);
+        if (tl != cur) {
+          if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
+          if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
+          se = si = 0;
+          cur = tl;
+        }
+        const uint32_t rel = (p.x >> kPullShift) - base_id;
+        const int32_t w = (int32_t)p.y;
+        if (((bits[rel >> 6] >> (rel & 63)) & 1ull) && w != 0) {
+          if (w >= 0) se += w; else si += w;
+          ++events;
+        }
+      }
+      if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
+      if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
+    const uint32_t tgt = b * kPullTB + i;
+    if (tgt < n.N) {
+      n.acc[tgt] = acc[i];
+      n.acc[n.npad + tgt] = acc[kPullTB + i];
+    }
+  }
+  // event count: one atomic per wave
+  unsigned long long ev = events;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) ev += __shfl_xor(ev, off);
+  if (__lane_id() == 0 && ev) atomicAdd(&n.counters[kCntEvents], ev);
+}
+
 template <int BLK>
 __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   __shared__ GatherSmem<BLK> sm;
-  gather_body<BLK>(n, *n.t_base + j, blockIdx.x, gridDim.x, sm);
+  extern __shared__ uint64_t dyn[];
+  const uint64_t t = *n.t_base + j;
+  if (n.pull_off) {
+    const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[t % n.R], n.q_cap);
+    if (nrows >= n.dense_rows) {
+      if (blockIdx.x < n.pull_nb) pull_body<BLK>(n, t, blockIdx.x, dyn);
+      if (blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
+      return;
+    }
+  }
+  gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
 }
 
 // Persistent step loop: one cooperative launch runs `nsteps` steps; grid
@@ -282,9 +357,19 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   __shared__ GatherSmem<BLK> sm;
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const uint64_t t0 = *t_base;
+  extern __shared__ uint64_t dyn[];
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
-    gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
+    const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[t % n.R], n.q_cap);
+    if (n.pull_off && nrows >= n.dense_rows) {
+      for (uint32_t b = blockIdx.x; b < n.pull_nb; b += gridDim.x) {
+        pull_body<BLK>(n, t, b, dyn);
+        __syncthreads();
+      }
+      if (blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
+    } else {
+      gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
+    }
     grid.sync();
     for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
       update_one<MODEL>(n, t, base + threadIdx.x);
@@ -315,13 +400,19 @@ hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_
   return hipGetLastError();
 }
 
+size_t pull_dyn_bytes(const DevNet& n) {
+  return n.pull_off ? (size_t)n.pull_words * 8 + (size_t)2 * kPullTB * 8 : 0;
+}
+
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {
   if (grid == 0) return hipErrorInvalidValue;
+  if (n.pull_off && grid < n.pull_nb) grid = n.pull_nb;
+  const size_t dyn = pull_dyn_bytes(n);
   switch (block) {
-    case 128: gather_kernel<128><<<grid, 128, 0, s>>>(n, j); break;
-    case 256: gather_kernel<256><<<grid, 256, 0, s>>>(n, j); break;
-    case 512: gather_kernel<512><<<grid, 512, 0, s>>>(n, j); break;
-    case 1024: gather_kernel<1024><<<grid, 1024, 0, s>>>(n, j); break;
+    case 128: gather_kernel<128><<<grid, 128, dyn, s>>>(n, j); break;
+    case 256: gather_kernel<256><<<grid, 256, dyn, s>>>(n, j); break;
+    case 512: gather_kernel<512><<<grid, 512, dyn, s>>>(n, j); break;
+    case 1024: gather_kernel<1024><<<grid, 1024, dyn, s>>>(n, j); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -335,13 +426,13 @@ hipError_t coop_launch(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint3
   uint64_t* tb = t_base;
   void* args[] = {&nn, &ns, &tb};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&persistent_kernel<MODEL, BLK>), dim3(grid),
-                                    dim3(BLK), args, 0, s);
+                                    dim3(BLK), args, pull_dyn_bytes(n), s);
 }
 template <int MODEL, int BLK>
-int coop_max_blocks() {
+int coop_max_blocks(size_t dyn) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&persistent_kernel<MODEL, BLK>),
-                                                   BLK, 0) != hipSuccess)
+                                                   BLK, dyn) != hipSuccess)
     return 0;
   return nb;
 }
@@ -365,12 +456,14 @@ hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base,
 #undef SZSNN_COOP
 }
 
-int persistent_max_blocks_per_cu(int model, uint32_t block) {
+int persistent_max_blocks_per_cu(const DevNet& n, uint32_t block) {
+  const size_t dyn = pull_dyn_bytes(n);
+  const int model = n.model;
 #define SZSNN_OCC(M)                                       \
   switch (block) {                                         \
-    case 256: return coop_max_blocks<M, 256>();            \
-    case 512: return coop_max_blocks<M, 512>();            \
-    case 1024: return coop_max_blocks<M, 1024>();          \
+    case 256: return coop_max_blocks<M, 256>(dyn);            \
+    case 512: return coop_max_blocks<M, 512>(dyn);            \
+    case 1024: return coop_max_blocks<M, 1024>(dyn);          \
     default: return 0;                                     \
   }
   switch (model) {

@@ -19,6 +19,7 @@
 
 #include "../../include/szsnn_hip.h"
 #include "szsnn_internal.h"
+#include "szsnn_layout.h"
 
 using namespace szsnn;
 
@@ -65,6 +66,8 @@ struct szsnn_net {
   bool graph_dirty = true;
   // autotune snapshot buffers
   std::vector<std::pair<void*, size_t>> snap;
+  uint64_t nonempty_rows = 0;
+  float dense_fraction = 0.3f;
   // batched networks: per-instance id ranges (empty for a plain network)
   std::vector<uint32_t> inst_noff, inst_soff, inst_n, inst_m;
 };
@@ -164,7 +167,7 @@ int build_graph(szsnn_net* net) {
 }
 
 uint32_t persistent_grid(szsnn_net* net, uint32_t block, uint32_t want) {
-  const int per_cu = persistent_max_blocks_per_cu(net->dn.model, block);
+  const int per_cu = persistent_max_blocks_per_cu(net->dn, block);
   const uint32_t maxg = (uint32_t)std::max(per_cu, 0) * (uint32_t)std::max(net->ctx->cu_count, 1);
   const uint32_t needed = (uint32_t)std::max<uint64_t>(1, ((uint64_t)net->dn.T + block - 1) / block);
   uint32_t g = want ? want : maxg;
@@ -338,56 +341,23 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   const double scale = std::ldexp(1.0, d->frac_bits);
   const uint64_t nrows = T * D;
 
-  // --- delay-bucketed CSR: row = id*D + k, stable in input order
-  std::vector<uint32_t> row_off(nrows + 1, 0);
-  std::vector<int32_t> wfix(d->n_syn);
-  for (uint64_t s = 0; s < d->n_syn; ++s) {
-    const uint32_t src = d->syn_src[s], tgt = d->syn_tgt[s], k = d->syn_delay[s];
-    if (src >= T || tgt >= N) {
-      std::ostringstream os;
-      os << "szsnn_net_upload: synapse " << s << " connects nonexisting ids source=" << src << " target=" << tgt;
-      return fail(ctx, SZSNN_ERR_INVALID, os.str());
-    }
-    if (k >= D) {
-      std::ostringstream os;
-      os << "szsnn_net_upload: synapse " << s << " delay " << k << " >= max_delay " << D;
-      return fail(ctx, SZSNN_ERR_INVALID, os.str());
-    }
-    const double q = std::nearbyint((double)d->syn_w[s] * scale);
-    if (!(std::fabs(q) < 2147483648.0)) {
-      std::ostringstream os;
-      os << "szsnn_net_upload: weight " << d->syn_w[s] << " does not fit int32 at frac_bits " << d->frac_bits;
-      return fail(ctx, SZSNN_ERR_INVALID, os.str());
-    }
-    wfix[s] = (int32_t)q;
-    if (wfix[s] != 0) row_off[(uint64_t)src * D + k + 1]++;
-  }
-  for (uint64_t r = 0; r < nrows; ++r) {
-    const uint64_t next = (uint64_t)row_off[r + 1] + row_off[r];
-    if (next > 0xffffffffull) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: more than 2^32-1 synapses");
-    row_off[r + 1] = (uint32_t)next;
-  }
-  const uint64_t S = row_off[nrows];
-  std::vector<uint2> syn(S);
-  {
-    std::vector<uint32_t> pos(row_off.begin(), row_off.end() - 1);
-    for (uint64_t s = 0; s < d->n_syn; ++s) {
-      if (wfix[s] == 0) continue;
-      const uint64_t r = (uint64_t)d->syn_src[s] * D + d->syn_delay[s];
-      syn[pos[r]++] = make_uint2(d->syn_tgt[s], (uint32_t)wfix[s]);
-    }
-  }
-  std::vector<int32_t>().swap(wfix);
-  std::vector<uint64_t> delay_bits(T, 0);
-  uint64_t nonempty = 0;
-  for (uint64_t id = 0; id < T; ++id)
-    for (uint64_t k = 0; k < D; ++k)
-      if (row_off[id * D + k + 1] != row_off[id * D + k]) {
-        delay_bits[id] |= 1ull << k;
-        ++nonempty;
-      }
+  // --- device formats: delay-bucketed CSR and the pull layout (szsnn_layout.h)
+  HostLayout L;
+  if (!build_layout(*d, L)) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: " + L.error);
+  const uint64_t S = L.row_off[nrows];
+  const uint64_t nonempty = L.nonempty;
   uint64_t cap = std::max<uint64_t>(nonempty, 1);
   if (d->queue_capacity) cap = std::min<uint64_t>(cap, d->queue_capacity);
+  const std::vector<uint32_t>& row_off = L.row_off;
+  const std::vector<uint64_t>& delay_bits = L.delay_bits;
+  const std::vector<uint32_t>& pull_off = L.pull_off;
+  const std::vector<uint32_t>& pull_lo = L.pull_lo;
+  const uint32_t pull_words = L.pull_words;
+  const uint64_t NB = L.NB;
+  static_assert(sizeof(LayoutPair) == sizeof(uint2), "LayoutPair must match uint2");
+  const uint2* syn_data = reinterpret_cast<const uint2*>(L.syn.data());
+  const uint2* pull_syn_data = reinterpret_cast<const uint2*>(L.pull_syn.data());
+  const size_t pull_syn_size = L.pull_syn.size();
 
   const double drive_q = std::nearbyint((double)d->drive_w * scale);
   if (!(std::fabs(drive_q) < 9.2e18)) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: drive_w too large");
@@ -439,6 +409,13 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     guard(dev_alloc(net, &src_rate, M)) && guard(dev_alloc(net, &src_last, M));
   if (rc == SZSNN_OK && M && n.source_model == kSrcPoisson) guard(dev_alloc(net, &src_thresh, M));
   if (rc == SZSNN_OK && d->drive_k) guard(dev_alloc(net, &drive_cdf, d->drive_k));
+  uint64_t* hist = nullptr;
+  uint32_t *d_pull_off = nullptr, *d_pull_lo = nullptr;
+  uint2* d_pull_syn = nullptr;
+  if (rc == SZSNN_OK) guard(dev_alloc(net, &hist, (size_t)n.R * n.W));
+  if (rc == SZSNN_OK && !pull_off.empty())
+    guard(dev_alloc(net, &d_pull_off, pull_off.size())) && guard(dev_alloc(net, &d_pull_syn, pull_syn_size)) &&
+        guard(dev_alloc(net, &d_pull_lo, pull_lo.size()));
   if (rc != SZSNN_OK) {
     szsnn_net_destroy(net);
     return rc;
@@ -457,11 +434,15 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   }
   guard(upload(net, params, hp.data(), hp.size())) && guard(upload(net, vars, hv.data(), hv.size())) &&
       guard(upload(net, keys, hk.data(), T)) && guard(upload(net, roff, row_off.data(), nrows + 1)) &&
-      guard(upload(net, dsyn, syn.data(), S)) && guard(upload(net, dbits, delay_bits.data(), T));
+      guard(upload(net, dsyn, syn_data, S)) && guard(upload(net, dbits, delay_bits.data(), T));
   if (rc == SZSNN_OK && noise) guard(upload(net, noise, d->noise_amp, N));
   if (rc == SZSNN_OK && src_rate) guard(upload(net, src_rate, d->src_rate, M)) && guard(upload(net, src_last, hsl.data(), M));
   if (rc == SZSNN_OK && src_thresh) guard(upload(net, src_thresh, d->src_thresh, M));
   if (rc == SZSNN_OK && drive_cdf) guard(upload(net, drive_cdf, d->drive_cdf, d->drive_k));
+  if (rc == SZSNN_OK && d_pull_off)
+    guard(upload(net, d_pull_off, pull_off.data(), pull_off.size())) &&
+        guard(upload(net, d_pull_syn, pull_syn_data, pull_syn_size)) &&
+        guard(upload(net, d_pull_lo, pull_lo.data(), pull_lo.size()));
   if (rc == SZSNN_OK) {
     hipStream_t s = ctx->stream;
     he = hipMemsetAsync(acc, 0, (size_t)2 * n.npad * 8, s);
@@ -469,6 +450,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     if (he == hipSuccess) he = hipMemsetAsync(counters, 0, kCntSlots * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->t_base, 0, 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->forced_bits, 0, (size_t)n.W * 8, s);
+    if (he == hipSuccess) he = hipMemsetAsync(hist, 0, (size_t)n.R * n.W * 8, s);
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     if (he != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_net_upload: ") + hipGetErrorString(he));
   }
@@ -497,6 +479,15 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.t_base = net->t_base;
   n.volley = 0;
   n.forced_bits = nullptr;
+  n.hist = hist;
+  n.pull_off = d_pull_off;
+  n.pull_syn = d_pull_syn;
+  n.pull_lo = d_pull_lo;
+  n.pull_nb = d_pull_off ? (uint32_t)NB : 0;
+  n.pull_words = pull_words;
+  net->nonempty_rows = nonempty;
+  n.dense_rows = (uint32_t)std::min<double>(0xffffffffu, std::ceil(net->dense_fraction * (double)nonempty));
+  if (n.dense_rows == 0) n.dense_rows = 1;
 
   net->info.n_neurons = (uint32_t)N;
   net->info.n_sources = (uint32_t)M;
@@ -826,6 +817,28 @@ int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_bloc
   return SZSNN_OK;
 }
 
+int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_gather_policy: net is NULL");
+  net->dense_fraction = dense_fraction;
+  if (dense_fraction <= 0.0f)
+    net->dn.dense_rows = 0;
+  else if (dense_fraction > 1.0f)
+    net->dn.dense_rows = 0xffffffffu;
+  else
+    net->dn.dense_rows =
+        (uint32_t)std::max<double>(1.0, std::ceil((double)dense_fraction * (double)net->nonempty_rows));
+  net->graph_dirty = true;
+  return SZSNN_OK;
+}
+
+int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_pull) {
+  if (!net || !dense_fraction || !has_pull)
+    return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_gather_policy: bad argument");
+  *dense_fraction = net->dense_fraction;
+  *has_pull = net->dn.pull_off != nullptr;
+  return SZSNN_OK;
+}
+
 int szsnn_set_mode(szsnn_net* net, int32_t mode, uint32_t block, uint32_t grid) {
   if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_mode: net is NULL");
   if (mode != SZSNN_MODE_GRAPH && mode != SZSNN_MODE_PERSISTENT)
@@ -879,6 +892,7 @@ std::vector<Region> state_regions(szsnn_net* net) {
       {n.q_rows, (size_t)n.R * n.q_cap * 4},
       {n.counters, (size_t)kCntSlots * 8},
       {net->t_base, 8},
+      {n.hist, (size_t)n.R * n.W * 8},
   };
   if (n.src_last) r.push_back({n.src_last, (size_t)n.M * 4});
   return r;
@@ -969,7 +983,7 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
   const uint32_t pblocks[] = {256, 512, 1024};
   const uint32_t pdiv[] = {4, 2, 1};
   for (uint32_t pb : pblocks) {
-    if (persistent_max_blocks_per_cu(net->dn.model, pb) <= 0) continue;
+    if (persistent_max_blocks_per_cu(net->dn, pb) <= 0) continue;
     for (int gi = 0; gi < 4; ++gi) {
       if (rc) break;
       Tiling tl = best;

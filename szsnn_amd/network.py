@@ -45,6 +45,7 @@ class Network:
     drive_cdf: np.ndarray = None  # uint64 [K]
     drive_w: float = 0.0
     queue_capacity: int = 0
+    pull_layout: int = 0          # 0: build the dense-gather layout when possible; -1: never
     meta: dict = field(default_factory=dict)
 
     @property

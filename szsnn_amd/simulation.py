@@ -87,6 +87,7 @@ def _desc(net):
     d.drive_cdf = ptr(cdf, ctypes.c_uint64)
     d.drive_w = net.drive_w
     d.queue_capacity = net.queue_capacity
+    d.pull_layout = getattr(net, "pull_layout", 0)
     return d, keep
 
 
@@ -189,6 +190,15 @@ class Simulation:
         check(self.lib.szsnn_get_tiling(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
               self.ctx.handle)
         return {"update_block": a.value, "gather_block": b.value, "gather_grid": c.value}
+
+    def set_gather_policy(self, dense_fraction):
+        """Pull when due rows >= dense_fraction x nonempty rows (<= 0: always, > 1: never)."""
+        check(self.lib.szsnn_set_gather_policy(self.handle, float(dense_fraction)), self.ctx.handle)
+
+    def gather_policy(self):
+        f, h = ctypes.c_float(), ctypes.c_int32()
+        check(self.lib.szsnn_get_gather_policy(self.handle, ctypes.byref(f), ctypes.byref(h)), self.ctx.handle)
+        return {"dense_fraction": f.value, "has_pull": bool(h.value)}
 
     def set_mode(self, mode, block=0, grid=0):
         check(self.lib.szsnn_set_mode(self.handle, int(mode), int(block), int(grid)), self.ctx.handle)

@@ -142,3 +142,16 @@ def test_batched_izhikevich_builder():
         amps = np.unique(n.noise_amp)
         assert len(amps) == 2 and np.all((amps >= 2) & (amps <= 8))
         assert n.meta["n_exc"] == round(0.8 * n.n_neurons)
+
+
+def test_layout_and_gather_index_check_cpu(tmp_path):
+    """Device formats built by the runtime's builder, and the push/pull gather
+    index arithmetic replayed on the CPU with bounds checks (ASan/UBSan)."""
+    import subprocess
+    exe = tmp_path / "layout_check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fsanitize=address,undefined", "-I",
+                    os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "layout_check.cpp"),
+                    "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "layout_check OK" in r.stdout
