@@ -69,9 +69,9 @@ def run_batched(args, ctx, log):
     ctx.synchronize()
     wall_b = time.perf_counter() - t1
     ev_b = bs.counters()["events"] - c0["events"]
-    rb = bs.raster()
-    per_inst = [bs.instance_raster(b, rb) for b in range(len(nets))]
-    del bs, rb
+    log(f"[bench] batched run: {wall_b * 1e3:.1f} ms for {T} steps, {ev_b / wall_b:.3e} events/s")
+    per_inst = bs.instance_rasters()
+    del bs
     wall_p = 0.0
     ev_p = 0
     same = True
@@ -86,6 +86,8 @@ def run_batched(args, ctx, log):
         ev_p += s.counters()["events"]
         same = same and np.array_equal(raster_to_bool(s.raster(), net.n_neurons), per_inst[b])
         del s
+        if b % 32 == 31:
+            log(f"[bench] plain runs: {b + 1}/{len(nets)}, {wall_p:.2f} s so far")
     out = {"instances": len(nets), "neurons": n_tot, "synapses": s_tot, "steps": T, "dt_ms": 1.0,
            "batched_events_per_s": ev_b / wall_b, "plain_sequential_events_per_s": ev_p / wall_p,
            "speedup": (ev_b / wall_b) / (ev_p / wall_p), "batched_wall_s": wall_b,

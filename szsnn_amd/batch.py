@@ -16,6 +16,17 @@ from .configs import DEFAULT_SEED, izhikevich2003
 from .simulation import Simulation, _desc, raster_to_bool
 
 
+def _bits(raster, first, count):
+    """bool [steps, count] of ids first..first+count-1 from uint64 bitmaps."""
+    if count == 0:
+        return np.zeros((raster.shape[0], 0), bool)
+    w0, w1 = first // 64, (first + count + 63) // 64
+    sub = np.ascontiguousarray(raster[:, w0:w1])
+    bits = np.unpackbits(sub.view(np.uint8).reshape(sub.shape[0], -1), axis=1, bitorder="little")
+    off = first - w0 * 64
+    return bits[:, off:off + count].astype(bool)
+
+
 def batched_izhikevich(seed=DEFAULT_SEED, n_instances=256, n_min=500, n_max=20000, fanout=100,
                        max_delay_steps=20, amp_lo=2.0, amp_hi=8.0):
     """Instances of the Izhikevich 2003 network with log-uniform sizes in
@@ -73,9 +84,13 @@ class BatchSimulation(Simulation):
     def instance_raster(self, b, raster=None):
         """bool [steps, n_b + m_b] for instance b (its neurons, then its sources)."""
         r = self.raster() if raster is None else raster
-        allb = raster_to_bool(r, self.info.n_neurons + self.info.n_sources)
         no, n, so, m = self.offsets[b]
-        return np.concatenate([allb[:, no:no + n], allb[:, so:so + m]], axis=1)
+        return np.concatenate([_bits(r, no, n), _bits(r, so, m)], axis=1)
+
+    def instance_rasters(self, raster=None):
+        """All instances' rasters, unpacking each only over its own id range."""
+        r = self.raster() if raster is None else raster
+        return [self.instance_raster(b, r) for b in range(len(self.networks))]
 
     def instance_vars(self, b, vars_=None):
         v = self.get_vars() if vars_ is None else vars_

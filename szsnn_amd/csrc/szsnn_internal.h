@@ -45,7 +45,10 @@ struct DevNet {
   long long drive_w; // fixed-point weight of one drive event
   const float* params;      // [np][npad]
   float* vars;              // [nv][npad]
-  long long* acc;           // [2][npad] fixed-point E/I accumulators
+  long long* acc;           // [acc_slots][2][npad] fixed-point E/I accumulators;
+                            // slot 0 always, slots 1.. only on steps that pulled
+  uint32_t acc_slots;       // G: delay groups of the pull gather (>= 1)
+  uint32_t* pulled;         // set by gather(t): 1 if it pulled, read by update(t)
   const uint64_t* keys;     // [T] random-stream keys
   const float* noise_amp;   // [N] or null
   const float* src_rate;    // [M] or null

@@ -102,29 +102,51 @@ __device__ inline bool izh_update(const float* __restrict__ p, float* __restrict
   return spike;
 }
 
-// update(t) for one id (one lane); every lane of the wave must call it
-// (ballots).  Resets the due-queue slot gathered at t when id == 0.
+// LDS of one update block: per-delay reservation counts and bases, the
+// waves' offsets inside the block's reservation, and counter partial sums.
+struct UpdateSmem {
+  uint32_t cnt[64];
+  uint32_t base[64];
+  uint32_t woff[16][64];
+  unsigned long long spikes, drive;
+};
+
+// update(t) for one id per lane; every thread of the block must call it (it
+// synchronises the block).  Resets the due-queue slot gathered at t when
+// id == 0.  Counters and due-queue reservations are aggregated per block:
+// one global atomic per block and delay.
 template <int MODEL>
-__device__ inline void update_one(const DevNet& n, uint64_t t, uint32_t id) {
+__device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem& sm) {
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  if (threadIdx.x < 64) sm.cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sm.spikes = sm.drive = 0;
   if (id == 0) n.q_count[t % n.R] = 0;  // slot gathered this step
+  __syncthreads();
   bool spike = false;
   uint64_t bits = 0;
+  uint32_t kdrive = 0;
   if (id < n.T) {
     bool forced = id < n.volley;
     if (n.forced_bits) forced = forced || ((n.forced_bits[id >> 6] >> (id & 63)) & 1ull);
     if (id < n.N) {
       long long ae = n.acc[id];
-      const long long ai = n.acc[n.npad + id];
+      long long ai = n.acc[n.npad + id];
       n.acc[id] = 0;
       n.acc[n.npad + id] = 0;
+      if (n.acc_slots > 1 && *n.pulled) {
+        // pulled step: the delay groups' partial sums (rewritten by every pull)
+        for (uint32_t g = 1; g < n.acc_slots; ++g) {
+          ae += n.acc[(size_t)g * 2 * n.npad + id];
+          ai += n.acc[(size_t)g * 2 * n.npad + n.npad + id];
+        }
+      }
       if (n.drive_k) {
         const uint64_t u = rng_draw(n.keys[id], t, 2);
         uint32_t k = 0;
         while (k < n.drive_k && u >= n.drive_cdf[k]) ++k;
-        if (k) {
-          ae += (long long)k * n.drive_w;
-          atomicAdd(&n.counters[kCntDrive], (unsigned long long)k);
-        }
+        ae += (long long)k * n.drive_w;
+        kdrive = k;
       }
       const float accE = (float)ae * n.acc_scale;
       const float accI = (float)ai * n.acc_scale;
@@ -148,36 +170,50 @@ __device__ inline void update_one(const DevNet& n, uint64_t t, uint32_t id) {
     }
     if (spike) bits = n.delay_bits[id];
   }
-  // dense spike output, one u64 word per wavefront (storeDenseSpikes,
-  // IF_curr_exp_update_kernel.cl:58-94)
+  // dense spike output and history, one u64 word per wavefront
+  // (storeDenseSpikes, IF_curr_exp_update_kernel.cl:58-94)
   const uint64_t mask = __ballot(spike);
-  const uint32_t lane = __lane_id();
   const uint32_t word = id >> 6;
   if (lane == 0 && word < n.W) {
     n.hist[(t % n.R) * n.W + word] = mask;
     if (n.raster && t >= n.rec_first && t - n.rec_first < n.rec_steps)
       n.raster[(t - n.rec_first) * n.W + word] = mask;
-    if (mask) atomicAdd(&n.counters[kCntSpikes], (unsigned long long)__popcll(mask));
+    if (mask) atomicAdd(&sm.spikes, (unsigned long long)__popcll(mask));
   }
-  if (mask == 0) return;
-  // scatter: for each delay some spiking lane of this wave has, one atomic
-  // reserves room for all of the wave's rows of that delay.
+  if (n.drive_k) {
+    uint32_t kd = kdrive;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) kd += __shfl_xor(kd, off);
+    if (lane == 0 && kd) atomicAdd(&sm.drive, (unsigned long long)kd);
+  }
+  // scatter, phase 1: per delay, the wave's rows get consecutive places
+  // inside the block's reservation
   uint64_t uni = bits;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
   const uint64_t lt = lanemask_lt();
-  while (uni) {
-    const uint32_t k = (uint32_t)__ffsll((unsigned long long)uni) - 1;
-    uni &= uni - 1;
+  for (uint64_t u = uni; u; u &= u - 1) {
+    const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
+    const uint64_t m = __ballot((bits >> k) & 1ull);
+    if (lane == 0) sm.woff[wave][k] = atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
+  }
+  __syncthreads();
+  // phase 2: one global reservation per (block, delay); block counters
+  if (threadIdx.x < n.D && sm.cnt[threadIdx.x])
+    sm.base[threadIdx.x] = atomicAdd(&n.q_count[(t + threadIdx.x + 1) % n.R], sm.cnt[threadIdx.x]);
+  if (threadIdx.x == 64 || (blockDim.x <= 64 && threadIdx.x == 0)) {
+    if (sm.spikes) atomicAdd(&n.counters[kCntSpikes], sm.spikes);
+    if (sm.drive) atomicAdd(&n.counters[kCntDrive], sm.drive);
+  }
+  __syncthreads();
+  // phase 3: write the rows
+  for (uint64_t u = uni; u; u &= u - 1) {
+    const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
     const bool mine = (bits >> k) & 1ull;
     const uint64_t m = __ballot(mine);
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-    const uint32_t slot = (uint32_t)((t + k + 1) % n.R);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&n.q_count[slot], (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
     if (mine) {
-      const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(m & lt);
+      const uint64_t pos = (uint64_t)sm.base[k] + sm.woff[wave][k] + (uint64_t)__popcll(m & lt);
+      const uint32_t slot = (uint32_t)((t + k + 1) % n.R);
       if (pos < n.q_cap)
         n.q_rows[(uint64_t)slot * n.q_cap + pos] = id * n.D + k;
       else
@@ -188,8 +224,9 @@ __device__ inline void update_one(const DevNet& n, uint64_t t, uint32_t id) {
 
 template <int MODEL>
 __global__ void __launch_bounds__(1024) update_kernel(DevNet n, uint32_t j) {
+  __shared__ UpdateSmem sm;
   const uint64_t t = *n.t_base + j;
-  update_one<MODEL>(n, t, blockIdx.x * blockDim.x + threadIdx.x);
+  update_block<MODEL>(n, t, blockIdx.x * blockDim.x + threadIdx.x, sm);
 }
 
 // Block-wide exclusive scan of one u32 per thread; returns the total.
@@ -272,21 +309,25 @@ __device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, 
   }
 }
 
-// Dense gather (pull) for target block `b`: for each delay k the spike
-// bitmap of step t-k-1 (the block's source range) is staged in LDS; each lane
-// walks kPullRun consecutive synapses of the (b, k) segment, keeps a running
-// sum while the target stays the same, and adds it into the block's LDS
-// accumulators when the target changes.  The block then stores its targets'
-// complete sums: no global atomics.  dyn: kPullWords u64 bitmap + 2*kPullTB i64.
+// Dense gather (pull) for target block `b` and delay group `g`: for each
+// delay k of the group the spike bitmap of step t-k-1 (the block's source
+// range) is staged in LDS; each lane walks kPullRun consecutive synapses of
+// the (b, k) segment (16-byte loads), keeps a running sum while the target
+// stays the same, and adds it into the block's LDS accumulators when the
+// target changes.  The block stores its targets' sums for the group into
+// accumulator slot g: plain stores, no global atomics.
+// dyn: pull_words u64 bitmap + 2*kPullTB i64.
 template <int BLK>
-__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint64_t* dyn) {
+__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t* dyn) {
   uint64_t* bits = dyn;
   long long* acc = reinterpret_cast<long long*>(dyn + n.pull_words);
   for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) acc[i] = 0;
   const uint32_t lo = n.pull_lo[b];
   const uint32_t nw = min(n.pull_words, n.W - lo);
+  const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.acc_slots);
+  const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.acc_slots);
   unsigned long long events = 0;
-  for (uint32_t k = 0; k < n.D; ++k) {
+  for (uint32_t k = k0; k < k1; ++k) {
     if (t < (uint64_t)k + 1) break;
     const uint32_t s0 = n.pull_off[b * n.D + k], s1 = n.pull_off[b * n.D + k + 1];
     if (s0 == s1) continue;
@@ -296,20 +337,28 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint64
     __syncthreads();
     const uint32_t base_id = lo * 64;
     for (uint32_t e0 = s0 + threadIdx.x * kPullRun; e0 < s1; e0 += BLK * kPullRun) {
-      const uint32_t e1 = min(e0 + kPullRun, s1);
+      // segments start and end at even indices: whole 16-byte pairs
+      const uint32_t npair = (min(e0 + kPullRun, s1) - e0) >> 1;
+      const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + e0);
+      uint4 q[kPullRun / 2];
+#pragma unroll
+      for (uint32_t j = 0; j < kPullRun / 2; ++j)
+        if (j < npair) q[j] = p4[j];
       uint32_t cur = 0xffffffffu;
       long long se = 0, si = 0;
-      for (uint32_t e = e0; e < e1; ++e) {
-        const uint2 p = n.pull_syn[e];
-        const uint32_t tl = p.x & (kPullTB - 1);
+#pragma unroll
+      for (uint32_t j = 0; j < kPullRun; ++j) {
+        if ((j >> 1) >= npair) break;
+        const uint32_t px = (j & 1) ? q[j >> 1].z : q[j >> 1].x;
+        const int32_t w = (int32_t)((j & 1) ? q[j >> 1].w : q[j >> 1].y);
+        const uint32_t tl = px & (kPullTB - 1);
         if (tl != cur) {
           if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
           if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
           se = si = 0;
           cur = tl;
         }
-        const uint32_t rel = (p.x >> kPullShift) - base_id;
-        const int32_t w = (int32_t)p.y;
+        const uint32_t rel = (px >> kPullShift) - base_id;
         if (((bits[rel >> 6] >> (rel & 63)) & 1ull) && w != 0) {
           if (w >= 0) se += w; else si += w;
           ++events;
@@ -320,11 +369,12 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint64
     }
   }
   __syncthreads();
+  long long* out = n.acc + (size_t)g * 2 * n.npad;
   for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
     const uint32_t tgt = b * kPullTB + i;
     if (tgt < n.N) {
-      n.acc[tgt] = acc[i];
-      n.acc[n.npad + tgt] = acc[kPullTB + i];
+      out[tgt] = acc[i];
+      out[n.npad + tgt] = acc[kPullTB + i];
     }
   }
   // event count: one atomic per wave
@@ -341,9 +391,14 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   const uint64_t t = *n.t_base + j;
   if (n.pull_off) {
     const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[t % n.R], n.q_cap);
-    if (nrows >= n.dense_rows) {
-      if (blockIdx.x < n.pull_nb) pull_body<BLK>(n, t, blockIdx.x, dyn);
-      if (blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
+    const bool pull = nrows >= n.dense_rows;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *n.pulled = pull ? 1u : 0u;
+      if (pull && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
+    }
+    if (pull) {
+      if (blockIdx.x < n.pull_nb * n.acc_slots)
+        pull_body<BLK>(n, t, blockIdx.x / n.acc_slots, blockIdx.x % n.acc_slots, dyn);
       return;
     }
   }
@@ -355,24 +410,27 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
 template <int MODEL, int BLK>
 __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nsteps, uint64_t* t_base) {
   __shared__ GatherSmem<BLK> sm;
+  __shared__ UpdateSmem usm;
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const uint64_t t0 = *t_base;
   extern __shared__ uint64_t dyn[];
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
     const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[t % n.R], n.q_cap);
-    if (n.pull_off && nrows >= n.dense_rows) {
-      for (uint32_t b = blockIdx.x; b < n.pull_nb; b += gridDim.x) {
-        pull_body<BLK>(n, t, b, dyn);
+    const bool pull = n.pull_off && nrows >= n.dense_rows;
+    if (pull) {
+      for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.acc_slots; bg += gridDim.x) {
+        pull_body<BLK>(n, t, bg / n.acc_slots, bg % n.acc_slots, dyn);
         __syncthreads();
       }
       if (blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
     } else {
       gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
     }
+    if (n.pull_off && blockIdx.x == 0 && threadIdx.x == 0) *n.pulled = pull ? 1u : 0u;
     grid.sync();
     for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
-      update_one<MODEL>(n, t, base + threadIdx.x);
+      update_block<MODEL>(n, t, base + threadIdx.x, usm);
     grid.sync();
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
@@ -406,7 +464,7 @@ size_t pull_dyn_bytes(const DevNet& n) {
 
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {
   if (grid == 0) return hipErrorInvalidValue;
-  if (n.pull_off && grid < n.pull_nb) grid = n.pull_nb;
+  if (n.pull_off && grid < n.pull_nb * n.acc_slots) grid = n.pull_nb * n.acc_slots;
   const size_t dyn = pull_dyn_bytes(n);
   switch (block) {
     case 128: gather_kernel<128><<<grid, 128, dyn, s>>>(n, j); break;

@@ -393,12 +393,19 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   };
   float *params = nullptr, *vars = nullptr, *noise = nullptr, *src_rate = nullptr, *src_last = nullptr;
   long long* acc = nullptr;
+  uint32_t* pulled = nullptr;
+  // delay groups of the pull gather: enough (block, group) work items to fill the chip
+  uint32_t acc_slots = 1;
+  if (!pull_off.empty()) {
+    const uint64_t want = ((uint64_t)4 * std::max(ctx->cu_count, 1) + NB - 1) / std::max<uint64_t>(NB, 1);
+    acc_slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, (uint64_t)D, (uint64_t)8}));
+  }
   uint64_t *keys = nullptr, *src_thresh = nullptr, *drive_cdf = nullptr, *dbits = nullptr, *raster = nullptr;
   uint32_t *roff = nullptr, *qc = nullptr, *qr = nullptr;
   uint2* dsyn = nullptr;
   unsigned long long* counters = nullptr;
   guard(dev_alloc(net, &params, (size_t)np * n.npad)) && guard(dev_alloc(net, &vars, (size_t)nv * n.npad)) &&
-      guard(dev_alloc(net, &acc, (size_t)2 * n.npad)) && guard(dev_alloc(net, &keys, T)) &&
+      guard(dev_alloc(net, &acc, (size_t)2 * n.npad * acc_slots)) && guard(dev_alloc(net, &pulled, 1)) && guard(dev_alloc(net, &keys, T)) &&
       guard(dev_alloc(net, &roff, nrows + 1)) && guard(dev_alloc(net, &dsyn, S)) &&
       guard(dev_alloc(net, &dbits, T)) && guard(dev_alloc(net, &qc, n.R)) &&
       guard(dev_alloc(net, &qr, (size_t)n.R * cap)) && guard(dev_alloc(net, &counters, kCntSlots)) &&
@@ -445,7 +452,8 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
         guard(upload(net, d_pull_lo, pull_lo.data(), pull_lo.size()));
   if (rc == SZSNN_OK) {
     hipStream_t s = ctx->stream;
-    he = hipMemsetAsync(acc, 0, (size_t)2 * n.npad * 8, s);
+    he = hipMemsetAsync(acc, 0, (size_t)2 * n.npad * 8 * acc_slots, s);
+    if (he == hipSuccess) he = hipMemsetAsync(pulled, 0, 4, s);
     if (he == hipSuccess) he = hipMemsetAsync(qc, 0, n.R * 4, s);
     if (he == hipSuccess) he = hipMemsetAsync(counters, 0, kCntSlots * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->t_base, 0, 8, s);
@@ -461,6 +469,8 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.params = params;
   n.vars = vars;
   n.acc = acc;
+  n.acc_slots = acc_slots;
+  n.pulled = pulled;
   n.keys = keys;
   n.noise_amp = noise;
   n.src_rate = src_rate;
@@ -887,7 +897,8 @@ std::vector<Region> state_regions(szsnn_net* net) {
   const DevNet& n = net->dn;
   std::vector<Region> r = {
       {n.vars, (size_t)net->info.n_vars * n.npad * 4},
-      {n.acc, (size_t)2 * n.npad * 8},
+      {n.acc, (size_t)2 * n.npad * 8 * n.acc_slots},
+      {n.pulled, 4},
       {n.q_count, (size_t)n.R * 4},
       {n.q_rows, (size_t)n.R * n.q_cap * 4},
       {n.counters, (size_t)kCntSlots * 8},

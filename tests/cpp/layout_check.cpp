@@ -110,19 +110,24 @@ static void push_replay(const HostLayout& L, uint32_t N, uint64_t S, const std::
 }
 
 // pull replay for step t with history bitmaps hist[R][W]
+// pull replay for step t with history bitmaps hist[R][W]; G delay groups, each
+// writing its slot of `out` ([G][2][N]); the caller sums the slots
 static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t t,
-                        const std::vector<uint64_t>& hist, uint32_t R, uint32_t W, std::vector<long long>& out,
-                        uint64_t& events) {
+                        const std::vector<uint64_t>& hist, uint32_t R, uint32_t W, uint32_t G,
+                        std::vector<long long>& out, uint64_t& events) {
   const uint32_t words = L.pull_words;
   std::vector<uint64_t> bits(words);
   std::vector<long long> acc(2 * kPullTB);
-  for (uint32_t b = 0; b < L.NB; ++b) {
+  for (uint32_t bg = 0; bg < L.NB * G; ++bg) {
+    const uint32_t b = bg / G, g = bg % G;
     std::fill(acc.begin(), acc.end(), 0);
     CHECK(b < L.pull_lo.size(), "pull_lo b=%u", b);
     const uint32_t lo = L.pull_lo[b];
     CHECK(lo <= W, "lo=%u W=%u", lo, W);
     const uint32_t nw = std::min(words, W - lo);
-    for (uint32_t k = 0; k < D; ++k) {
+    const uint32_t k0 = (uint32_t)(((uint64_t)g * D) / G), k1 = (uint32_t)(((uint64_t)(g + 1) * D) / G);
+    CHECK(k1 <= D, "group end");
+    for (uint32_t k = k0; k < k1; ++k) {
       if (t < (uint64_t)k + 1) break;
       CHECK((uint64_t)b * D + k + 1 < L.pull_off.size(), "pull_off");
       const uint32_t s0 = L.pull_off[b * D + k], s1 = L.pull_off[b * D + k + 1];
@@ -137,6 +142,7 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
       const uint32_t base_id = lo * 64;
       for (uint32_t e0 = s0; e0 < s1; e0 += kPullRun) {  // every lane's run, in order
         const uint32_t e1 = std::min(e0 + kPullRun, s1);
+        CHECK((e0 & 1) == 0 && ((e1 - e0) & 1) == 0, "16-byte pair loads need even run bounds");
         for (uint32_t e = e0; e < e1; ++e) {
           const LayoutPair p = L.pull_syn[e];
           const uint32_t tl = p.x & (kPullTB - 1);
@@ -157,8 +163,9 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
     for (uint32_t i = 0; i < kPullTB; ++i) {
       const uint32_t tgt = b * kPullTB + i;
       if (tgt < N) {
-        out[tgt] = acc[i];
-        out[N + tgt] = acc[kPullTB + i];
+        CHECK((size_t)g * 2 * N + N + tgt < out.size(), "slot write");
+        out[(size_t)g * 2 * N + tgt] = acc[i];
+        out[(size_t)g * 2 * N + N + tgt] = acc[kPullTB + i];
       } else {
         CHECK(acc[i] == 0 && acc[kPullTB + i] == 0, "sum for a target beyond N");
       }
@@ -228,11 +235,17 @@ static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D,
     std::vector<long long> push(2 * (size_t)N, 0);
     push_replay(L, N, Ss, part, 4, (uint32_t)rows.size() + 1000, push, 256, 3);
   }
-  std::vector<long long> pulled(2 * (size_t)N, -1);
-  uint64_t pev = 0;
-  pull_replay(L, N, T, D, t, hist, R, W, pulled, pev);
-  CHECK(pulled == naive, "pull sum differs");
-  CHECK(pev == nev, "pull events %llu vs %llu", (unsigned long long)pev, (unsigned long long)nev);
+  for (uint32_t G : {1u, 2u, 3u, 8u}) {
+    if (G > D) continue;
+    std::vector<long long> slots((size_t)G * 2 * N, -1);
+    uint64_t pev = 0;
+    pull_replay(L, N, T, D, t, hist, R, W, G, slots, pev);
+    std::vector<long long> pulled(2 * (size_t)N, 0);
+    for (uint32_t g = 0; g < G; ++g)
+      for (size_t i = 0; i < 2 * (size_t)N; ++i) pulled[i] += slots[(size_t)g * 2 * N + i];
+    CHECK(pulled == naive, "pull sum differs (G=%u)", G);
+    CHECK(pev == nev, "pull events %llu vs %llu", (unsigned long long)pev, (unsigned long long)nev);
+  }
 }
 
 int main() {

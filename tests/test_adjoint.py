@@ -91,7 +91,7 @@ def _gpu_grads(P, device, K=20):
     loss, y, bits = snn_loss(W1, W2, t["X"], t["target"], t["params"], P["alpha"], K=K)
     loss.backward()
     torch.cuda.synchronize()
-    return float(loss), W1.grad.cpu().double(), W2.grad.cpu().double(), bits.cpu()
+    return float(loss.detach()), W1.grad.cpu().double(), W2.grad.cpu().double(), bits.cpu()
 
 
 def _bits_to_spikes(bits, T, B, H):
@@ -142,8 +142,10 @@ def test_hip_adjoint_checkpoint_interval_invariant(cuda):
     _, a1, a2, _ = _gpu_grads(P, cuda, K=20)
     _, b1, b2, _ = _gpu_grads(P, cuda, K=7)
     _, c1, c2, _ = _gpu_grads(P, cuda, K=64)
-    assert torch.equal(a1, b1) and torch.equal(a1, c1)
-    assert torch.equal(a2, b2) and torch.equal(a2, c2)
+    # the recurrences are identical; only the grouping of the per-segment GEMM
+    # sums into dW changes with K, at FP32 rounding level
+    for x, y in ((a1, b1), (a1, c1), (a2, b2), (a2, c2)):
+        assert _rel(x, y) <= 1e-6, _rel(x, y)
 
 
 @pytest.mark.gpu

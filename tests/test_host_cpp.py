@@ -99,4 +99,4 @@ def test_cpp_driver_matches_oracle(tmp_path):
     raster, _ = Oracle(net, 32).run(steps, forced=forced)
     bits = np.unpackbits(raster.view(np.uint8).reshape(steps, -1), axis=1, bitorder="little")[:, :100]
     np.testing.assert_array_equal(got, bits.astype(bool))
-    assert got.sum() > 50
+    assert got.sum() > 20  # the network is active (forced volleys plus Const_rate drive)
