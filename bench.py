@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--autotune", action="store_true", help="autotune the plain network before timing")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tiling")
     ap.add_argument("--profile-pass", action="store_true",
                     help="counter pass: a 1 GiB device copy (calibration) then volley steps only")
     return ap.parse_args()
@@ -135,8 +135,8 @@ def main():
         g_ms, u_ms = sim.time_kernels(args.volley_steps)
         log(f"[bench] profile pass: volley gather {g_ms * 1e3 / args.volley_steps:.1f} us/launch")
         return
-    if args.autotune:
-        log(f"[bench] autotuned: {sim.autotune(100)}")
+    if not args.no_autotune:
+        log(f"[bench] autotuned for the natural dynamics: {sim.autotune(200)}")
     # warmup (also builds the step graph)
     sim.run(args.warmup)
     torch.cuda.synchronize()
@@ -173,10 +173,15 @@ def main():
     log(f"[bench] natural kernels: {natural}")
 
     # volley: every neuron forced every step -> every (source, delay) row is due
+    natural_tiling = (sim.tiling(), sim.mode())
+
     def volley(fraction):
         sim.set_gather_policy(fraction)
         sim.set_volley(net.n_neurons)
         sim.run(net.max_delay + 2)  # fill all delay slots
+        if not args.no_autotune:
+            log(f"[bench] autotuned for the volley (policy {fraction}): {sim.autotune(10)}")
+        sim.set_mode(0)  # time the kernels one launch at a time
         cv0 = sim.counters()
         gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
         cv1 = sim.counters()
@@ -193,6 +198,7 @@ def main():
     v_auto = volley(0.3)
     log(f"[bench] volley auto (pull): {v_auto}")
     sim.set_gather_policy(0.3)
+    v_auto["tiling"] = sim.tiling()
     per_launch = v_auto["alg_bytes_per_launch"]
     g_avg_s = v_auto["gather_us"] * 1e-6
     achieved = v_auto["gbs"]
@@ -234,6 +240,7 @@ def main():
         "firing_rate_hz": rate_hz,
         "events_per_step": events / args.steps,
         "natural_kernels": natural,
+        "natural_tiling": {**natural_tiling[0], **natural_tiling[1]},
         "volley_push_only": v_push,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -272,8 +272,8 @@ class Engine {
       Acc ae = accE[i];
       if (d_.drive_k) {
         const uint64_t u = draw(keys_[i], t_, 2);
-        uint32_t kk = 0;
-        while (kk < d_.drive_k && u >= d_.drive_cdf[kk]) ++kk;
+        uint32_t kk = 0;  // #{j : u >= cdf[j]}
+        for (uint32_t j = 0; j < d_.drive_k; ++j) kk += u >= d_.drive_cdf[j] ? 1u : 0u;
         ae += (Acc)kk * drive_acc_;
         drive_events_ += kk;
       }

@@ -142,9 +142,13 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
         }
       }
       if (n.drive_k) {
+        // events = #{j : u >= cdf[j]}; the thresholds ascend, so this equals the
+        // inverse-CDF index.  Counting over all j keeps the loads uniform
+        // across the wave (scalar loads) instead of a per-lane dependent walk.
         const uint64_t u = rng_draw(n.keys[id], t, 2);
         uint32_t k = 0;
-        while (k < n.drive_k && u >= n.drive_cdf[k]) ++k;
+#pragma unroll 8
+        for (uint32_t j = 0; j < n.drive_k; ++j) k += (u >= n.drive_cdf[j]) ? 1u : 0u;
         ae += (long long)k * n.drive_w;
         kdrive = k;
       }
@@ -397,8 +401,10 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
       if (pull && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
     }
     if (pull) {
-      if (blockIdx.x < n.pull_nb * n.acc_slots)
-        pull_body<BLK>(n, t, blockIdx.x / n.acc_slots, blockIdx.x % n.acc_slots, dyn);
+      for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.acc_slots; bg += gridDim.x) {
+        pull_body<BLK>(n, t, bg / n.acc_slots, bg % n.acc_slots, dyn);
+        __syncthreads();  // LDS reuse by the next work item
+      }
       return;
     }
   }
@@ -464,7 +470,6 @@ size_t pull_dyn_bytes(const DevNet& n) {
 
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {
   if (grid == 0) return hipErrorInvalidValue;
-  if (n.pull_off && grid < n.pull_nb * n.acc_slots) grid = n.pull_nb * n.acc_slots;
   const size_t dyn = pull_dyn_bytes(n);
   switch (block) {
     case 128: gather_kernel<128><<<grid, 128, dyn, s>>>(n, j); break;
