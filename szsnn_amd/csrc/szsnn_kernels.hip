@@ -340,19 +340,21 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = src[w];
     __syncthreads();
     const uint32_t base_id = lo * 64;
-    for (uint32_t e0 = s0 + threadIdx.x * kPullRun; e0 < s1; e0 += BLK * kPullRun) {
-      // segments start and end at even indices: whole 16-byte pairs
-      const uint32_t npair = (min(e0 + kPullRun, s1) - e0) >> 1;
-      const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + e0);
+    const uint32_t lane = __lane_id();
+    constexpr uint32_t kChunk = 64 * kPullRun;
+    // one lane-interleaved chunk per wave: the j-th 16-byte load of the wave
+    // is contiguous, and lane L receives its kPullRun consecutive entries
+    for (uint32_t cs = s0 + (threadIdx.x >> 6) * kChunk; cs < s1; cs += (BLK / 64) * kChunk) {
+      const uint32_t nl = min(kChunk, s1 - cs) / kPullRun;  // segments are whole lane runs
+      if (lane >= nl) continue;
+      const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + cs);
       uint4 q[kPullRun / 2];
 #pragma unroll
-      for (uint32_t j = 0; j < kPullRun / 2; ++j)
-        if (j < npair) q[j] = p4[j];
+      for (uint32_t j = 0; j < kPullRun / 2; ++j) q[j] = p4[j * nl + lane];
       uint32_t cur = 0xffffffffu;
       long long se = 0, si = 0;
 #pragma unroll
       for (uint32_t j = 0; j < kPullRun; ++j) {
-        if ((j >> 1) >= npair) break;
         const uint32_t px = (j & 1) ? q[j >> 1].z : q[j >> 1].x;
         const int32_t w = (int32_t)((j & 1) ? q[j >> 1].w : q[j >> 1].y);
         const uint32_t tl = px & (kPullTB - 1);

@@ -10,8 +10,14 @@
 //     reference's bitmap, SynapseData.ipp:70-83);
 //   * the pull layout for dense gathers: targets in blocks of kPullTB, one
 //     segment per (block, delay) sorted by target, entries
-//     (id << kPullShift | target % kPullTB, weight), segments padded to even
-//     length with inert entries (the block's first source word, weight 0).
+//     (id << kPullShift | target % kPullTB, weight), segments padded to a
+//     multiple of kPullRun entries with inert entries (the block's first
+//     source word, weight 0).  Lane-interleaved chunks: a segment is cut into
+//     chunks of 64 * kPullRun entries (the last one shorter); lane L of a wave
+//     owns the chunk's entries L*kPullRun .. L*kPullRun+kPullRun-1 in sorted
+//     order, and entry pair j of lane L is stored at pair j*nl + L of the
+//     chunk (nl = lanes in the chunk), so the wave's j-th 16-byte load is
+//     contiguous.
 // Shared by the runtime and by the CPU layout check (tests/cpp/layout_check.cpp).
 #pragma once
 
@@ -145,7 +151,7 @@ inline bool build_layout(const szsnn_net_desc& d, HostLayout& L) {
       kpos[sg * kPullTB + j] = (uint32_t)pos;
       pos += kcnt[sg * kPullTB + j + 1];
     }
-    if (pos & 1) ++pos;  // pad to even length
+    pos = (pos + kPullRun - 1) / kPullRun * kPullRun;  // pad to whole lane runs
     if (pos > 0xffffffffull) return true;
   }
   off[NB * D] = (uint32_t)pos;
@@ -161,6 +167,21 @@ inline bool build_layout(const szsnn_net_desc& d, HostLayout& L) {
         const uint64_t key = ((uint64_t)(tg >> kPullShift) * D + k) * kPullTB + (tg & (kPullTB - 1));
         ps[kpos[key]++] = LayoutPair{((uint32_t)id << kPullShift) | (tg & (kPullTB - 1)), L.syn[q].y};
       }
+  // interleave every chunk: logical pair q = lane*(kPullRun/2) + j goes to j*nl + lane
+  const uint32_t ppl = kPullRun / 2;  // pairs per lane
+  std::vector<LayoutPair> tmp(64 * kPullRun);
+  for (uint64_t sg = 0; sg < NB * D; ++sg)
+    for (uint64_t cs = off[sg]; cs < off[sg + 1]; cs += 64 * kPullRun) {
+      const uint64_t m = std::min<uint64_t>(64 * kPullRun, off[sg + 1] - cs);
+      const uint64_t nl = m / kPullRun;
+      std::copy(ps.begin() + cs, ps.begin() + cs + m, tmp.begin());
+      for (uint64_t q = 0; q < m / 2; ++q) {
+        const uint64_t lane = q / ppl, j = q % ppl;
+        const uint64_t dst = cs + 2 * (j * nl + lane);
+        ps[dst] = tmp[2 * q];
+        ps[dst + 1] = tmp[2 * q + 1];
+      }
+    }
   L.pull_off.swap(off);
   L.pull_lo.swap(lo);
   L.pull_syn.swap(ps);

@@ -132,7 +132,7 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
       CHECK((uint64_t)b * D + k + 1 < L.pull_off.size(), "pull_off");
       const uint32_t s0 = L.pull_off[b * D + k], s1 = L.pull_off[b * D + k + 1];
       CHECK(s0 <= s1 && s1 <= L.pull_syn.size(), "segment");
-      CHECK(((s1 - s0) & 1) == 0, "segment not padded to even length");
+      CHECK(((s1 - s0) % kPullRun) == 0 && (s0 % kPullRun) == 0, "segment not made of whole lane runs");
       if (s0 == s1) continue;
       const uint64_t hbase = ((t - k - 1) % R) * W + lo;
       for (uint32_t w = 0; w < nw; ++w) {
@@ -140,10 +140,16 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
         bits[w] = hist[hbase + w];
       }
       const uint32_t base_id = lo * 64;
-      for (uint32_t e0 = s0; e0 < s1; e0 += kPullRun) {  // every lane's run, in order
-        const uint32_t e1 = std::min(e0 + kPullRun, s1);
-        CHECK((e0 & 1) == 0 && ((e1 - e0) & 1) == 0, "16-byte pair loads need even run bounds");
-        for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t kChunk = 64 * kPullRun;
+      for (uint32_t cs = s0; cs < s1; cs += kChunk) {  // every wave's chunk, every lane's run
+        const uint32_t nl = std::min(kChunk, s1 - cs) / kPullRun;
+        CHECK(nl >= 1 && nl <= 64, "lanes per chunk");
+        for (uint32_t lane = 0; lane < nl; ++lane)
+        for (uint32_t jj = 0; jj < kPullRun; ++jj) {
+          const uint32_t pair = (jj / 2) * nl + lane;
+          CHECK(pair < std::min(kChunk, s1 - cs) / 2, "pair index beyond the chunk");
+          const uint32_t e = cs + 2 * pair + (jj & 1);
+          CHECK(e < s1, "entry beyond the segment");
           const LayoutPair p = L.pull_syn[e];
           const uint32_t tl = p.x & (kPullTB - 1);
           const uint32_t id = p.x >> kPullShift;
