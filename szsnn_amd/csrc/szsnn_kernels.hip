@@ -44,15 +44,18 @@ __device__ inline uint64_t lanemask_lt() {
 
 // IF_curr_exp (IF_curr_exp_update_kernel.cl:230-297) and IF_cond_exp (same
 // scheme, conductances in the current slots).  Returns spike.
+// The device parameter rows 1, 2, 4, 5 hold 1/cm, 1/tau_m, 1 - 1/tau_syn_E and
+// 1 - 1/tau_syn_I, formed once at upload with the same IEEE FP32 operations the
+// step would repeat (the reference evaluates them every step).
 template <int MODEL>
 __device__ inline bool if_update(const float* __restrict__ p, float* __restrict__ x, uint32_t i,
                                  uint32_t stride, float accE, float accI, float iinj, bool forced) {
-  const float v_rest = p[0 * stride + i], cm = p[1 * stride + i], tau_m = p[2 * stride + i];
-  const float tau_refrac = p[3 * stride + i], tau_e = p[4 * stride + i], tau_i = p[5 * stride + i];
+  const float v_rest = p[0 * stride + i], rcm = p[1 * stride + i], rtau_m = p[2 * stride + i];
+  const float tau_refrac = p[3 * stride + i], decay_e = p[4 * stride + i], decay_i = p[5 * stride + i];
   const float i_offset = p[6 * stride + i], v_reset = p[7 * stride + i], v_thresh = p[8 * stride + i];
   float v = x[0 * stride + i];
-  const float se = ((1.0f - 1.0f / tau_e) * x[1 * stride + i]) + accE;
-  const float si = ((1.0f - 1.0f / tau_i) * x[2 * stride + i]) + accI;
+  const float se = (decay_e * x[1 * stride + i]) + accE;
+  const float si = (decay_i * x[2 * stride + i]) + accI;
   float I;
   if (MODEL == kIfCurrExp) {
     I = ((se + si) + iinj) + i_offset;
@@ -66,7 +69,7 @@ __device__ inline bool if_update(const float* __restrict__ p, float* __restrict_
   if (!refr) {
     // x / y as x * (1/y), the reference's FP32 division (no correctly
     // rounded divide in its build, Simulation.ipp:13-21)
-    const float delta = (I * (1.0f / cm)) + ((v_rest - v) * (1.0f / tau_m));
+    const float delta = (I * rcm) + ((v_rest - v) * rtau_m);
     v = v + delta;
   }
   const bool spike = (!refr && v >= v_thresh) || forced;
@@ -342,36 +345,67 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     const uint32_t base_id = lo * 64;
     const uint32_t lane = __lane_id();
     constexpr uint32_t kChunk = 64 * kPullRun;
+    constexpr uint32_t kWaves = BLK / 64;
     // one lane-interleaved chunk per wave: the j-th 16-byte load of the wave
-    // is contiguous, and lane L receives its kPullRun consecutive entries
-    for (uint32_t cs = s0 + (threadIdx.x >> 6) * kChunk; cs < s1; cs += (BLK / 64) * kChunk) {
-      const uint32_t nl = min(kChunk, s1 - cs) / kPullRun;  // segments are whole lane runs
-      if (lane >= nl) continue;
-      const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + cs);
-      uint4 q[kPullRun / 2];
+    // is contiguous, and lane L receives its kPullRun consecutive entries.
+    // The next chunk's loads are issued before the current one is processed.
+    uint32_t cs = s0 + (threadIdx.x >> 6) * kChunk;
+    uint4 q[kPullRun / 2];
+    uint32_t nl = 0;
+    if (cs < s1) {
+      nl = min(kChunk, s1 - cs) / kPullRun;  // segments are whole lane runs
+      if (lane < nl) {
+        const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + cs);
 #pragma unroll
-      for (uint32_t j = 0; j < kPullRun / 2; ++j) q[j] = p4[j * nl + lane];
-      uint32_t cur = 0xffffffffu;
-      long long se = 0, si = 0;
+        for (uint32_t j = 0; j < kPullRun / 2; ++j) q[j] = p4[j * nl + lane];
+      }
+    }
+    while (cs < s1) {
+      const uint32_t ncs = cs + kWaves * kChunk;
+      uint4 qn[kPullRun / 2];
+      uint32_t nnl = 0;
+      if (ncs < s1) {
+        nnl = min(kChunk, s1 - ncs) / kPullRun;
+        if (lane < nnl) {
+          const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + ncs);
 #pragma unroll
-      for (uint32_t j = 0; j < kPullRun; ++j) {
-        const uint32_t px = (j & 1) ? q[j >> 1].z : q[j >> 1].x;
-        const int32_t w = (int32_t)((j & 1) ? q[j >> 1].w : q[j >> 1].y);
-        const uint32_t tl = px & (kPullTB - 1);
-        if (tl != cur) {
-          if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
-          if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
-          se = si = 0;
-          cur = tl;
-        }
-        const uint32_t rel = (px >> kPullShift) - base_id;
-        if (((bits[rel >> 6] >> (rel & 63)) & 1ull) && w != 0) {
-          if (w >= 0) se += w; else si += w;
-          ++events;
+          for (uint32_t j = 0; j < kPullRun / 2; ++j) qn[j] = p4[j * nnl + lane];
         }
       }
-      if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
-      if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
+      if (lane < nl) {
+        // spike bits of the run's sources, looked up before the sums
+        uint32_t sel = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kPullRun; ++j) {
+          const uint32_t px = (j & 1) ? q[j >> 1].z : q[j >> 1].x;
+          const uint32_t rel = (px >> kPullShift) - base_id;
+          sel |= (uint32_t)((bits[rel >> 6] >> (rel & 63)) & 1ull) << j;
+        }
+        uint32_t cur = 0xffffffffu;
+        long long se = 0, si = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kPullRun; ++j) {
+          const uint32_t px = (j & 1) ? q[j >> 1].z : q[j >> 1].x;
+          const int32_t w = (int32_t)((j & 1) ? q[j >> 1].w : q[j >> 1].y);
+          const uint32_t tl = px & (kPullTB - 1);
+          if (tl != cur) {
+            if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
+            if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
+            se = si = 0;
+            cur = tl;
+          }
+          if (((sel >> j) & 1u) && w != 0) {
+            if (w >= 0) se += w; else si += w;
+            ++events;
+          }
+        }
+        if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
+        if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kPullRun / 2; ++j) q[j] = qn[j];
+      nl = nnl;
+      cs = ncs;
     }
   }
   __syncthreads();

@@ -34,7 +34,7 @@ namespace szsnn {
 
 #ifndef SZSNN_PULL_CONSTANTS
 #define SZSNN_PULL_CONSTANTS
-constexpr uint32_t kPullShift = 8;
+constexpr uint32_t kPullShift = 9;
 constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block
 constexpr uint32_t kPullMaxWords = 6144;         // 48 KiB of staged bitmap (sources spanning 393k ids)
 constexpr uint32_t kPullRun = 8;                 // consecutive synapses per lane

@@ -431,6 +431,19 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   // host staging with the padded SoA stride
   std::vector<float> hp((size_t)np * n.npad, 0.0f), hv((size_t)nv * n.npad, 0.0f);
   for (int k = 0; k < np; ++k) std::memcpy(&hp[(size_t)k * n.npad], d->params + (size_t)k * N, N * 4);
+  if (d->model == kIfCurrExp || d->model == kIfCondExp) {
+    // rows 1, 2, 4, 5 -> 1/cm, 1/tau_m, 1 - 1/tau_syn_E, 1 - 1/tau_syn_I (FP32, as the step evaluates them)
+    for (uint64_t i = 0; i < N; ++i) {
+      float* c = &hp[1 * (size_t)n.npad + i];
+      float* m = &hp[2 * (size_t)n.npad + i];
+      float* e = &hp[4 * (size_t)n.npad + i];
+      float* q = &hp[5 * (size_t)n.npad + i];
+      *c = 1.0f / *c;
+      *m = 1.0f / *m;
+      *e = 1.0f - 1.0f / *e;
+      *q = 1.0f - 1.0f / *q;
+    }
+  }
   for (int k = 0; k < nv; ++k) std::memcpy(&hv[(size_t)k * n.npad], d->vars + (size_t)k * N, N * 4);
   std::vector<uint64_t> hk(T);
   for (uint64_t i = 0; i < T; ++i) hk[i] = id_keys ? id_keys[i] : splitmix_at(d->seed, i);
