@@ -64,7 +64,7 @@ struct DevNet {
   uint64_t* raster;         // [rec_steps][W] or null
   uint64_t rec_first;
   uint64_t rec_steps;
-  unsigned long long* counters;  // [0]=events [1]=spikes [2]=drive events [3]=overflow [4]=rows
+  unsigned long long* counters;  // striped, see counter(): events spikes drive overflow rows
   const uint64_t* t_base;   // device cycle counter
   uint32_t volley;          // ids [0, volley) are forced every step
   const uint64_t* forced_bits;   // [W] forced ids this step, or null
@@ -91,6 +91,16 @@ constexpr uint32_t kPullRun = 8;                 // consecutive synapses per lan
 
 // Counter slots.
 enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntRows = 4, kCntSlots = 5 };
+
+// Counters are striped over kCntStripes cache lines (128 B apart) so that
+// blocks adding to one counter do not serialise on a single address; the host
+// sums the stripes.  counters = [kCntSlots][kCntStripes][16] u64.
+constexpr uint32_t kCntStripes = 64;
+constexpr uint32_t kCntStride = 16;
+constexpr size_t kCntWords = (size_t)kCntSlots * kCntStripes * kCntStride;
+__host__ __device__ inline unsigned long long* counter(unsigned long long* base, int slot, uint32_t stripe) {
+  return base + ((size_t)slot * kCntStripes + (stripe % kCntStripes)) * kCntStride;
+}
 
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);

@@ -209,8 +209,8 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
   if (threadIdx.x < n.D && sm.cnt[threadIdx.x])
     sm.base[threadIdx.x] = atomicAdd(&n.q_count[(t + threadIdx.x + 1) % n.R], sm.cnt[threadIdx.x]);
   if (threadIdx.x == 64 || (blockDim.x <= 64 && threadIdx.x == 0)) {
-    if (sm.spikes) atomicAdd(&n.counters[kCntSpikes], sm.spikes);
-    if (sm.drive) atomicAdd(&n.counters[kCntDrive], sm.drive);
+    if (sm.spikes) atomicAdd(counter(n.counters, kCntSpikes, blockIdx.x), sm.spikes);
+    if (sm.drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), sm.drive);
   }
   __syncthreads();
   // phase 3: write the rows
@@ -224,7 +224,7 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
       if (pos < n.q_cap)
         n.q_rows[(uint64_t)slot * n.q_cap + pos] = id * n.D + k;
       else
-        atomicOr(&n.counters[kCntOverflow], 1ull);
+        atomicOr(counter(n.counters, kCntOverflow, blockIdx.x), 1ull);
     }
   }
 }
@@ -311,8 +311,8 @@ __device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, 
     __syncthreads();
   }
   if (threadIdx.x == 0 && rows_done) {
-    atomicAdd(&n.counters[kCntEvents], delivered);
-    atomicAdd(&n.counters[kCntRows], rows_done);
+    atomicAdd(counter(n.counters, kCntEvents, blockIdx.x), delivered);
+    atomicAdd(counter(n.counters, kCntRows, blockIdx.x), rows_done);
   }
 }
 
@@ -421,7 +421,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   unsigned long long ev = events;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) ev += __shfl_xor(ev, off);
-  if (__lane_id() == 0 && ev) atomicAdd(&n.counters[kCntEvents], ev);
+  if (__lane_id() == 0 && ev) atomicAdd(counter(n.counters, kCntEvents, blockIdx.x * 16 + (threadIdx.x >> 6)), ev);
 }
 
 template <int BLK>
@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
     const bool pull = nrows >= n.dense_rows;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       *n.pulled = pull ? 1u : 0u;
-      if (pull && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
+      if (pull && nrows) atomicAdd(counter(n.counters, kCntRows, 0), (unsigned long long)nrows);
     }
     if (pull) {
       for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.acc_slots; bg += gridDim.x) {
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
         pull_body<BLK>(n, t, bg / n.acc_slots, bg % n.acc_slots, dyn);
         __syncthreads();
       }
-      if (blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(&n.counters[kCntRows], (unsigned long long)nrows);
+      if (blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(counter(n.counters, kCntRows, 0), (unsigned long long)nrows);
     } else {
       gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
     }

@@ -205,10 +205,23 @@ int run_steps(szsnn_net* net, uint64_t n) {
   return SZSNN_OK;
 }
 
+// Sum of the stripes of every counter slot.
+int read_counters(szsnn_net* net, unsigned long long* c) {
+  std::vector<unsigned long long> raw(kCntWords);
+  HIP_TRY(net->ctx, hipMemcpyAsync(raw.data(), net->dn.counters, kCntWords * 8, hipMemcpyDeviceToHost,
+                                   net->ctx->stream));
+  HIP_TRY(net->ctx, hipStreamSynchronize(net->ctx->stream));
+  for (int k = 0; k < kCntSlots; ++k) {
+    c[k] = 0;
+    for (uint32_t st = 0; st < kCntStripes; ++st) c[k] += *counter(raw.data(), k, st);
+  }
+  return SZSNN_OK;
+}
+
 int check_overflow(szsnn_net* net) {
   unsigned long long c[kCntSlots];
-  HIP_TRY(net->ctx, hipMemcpyAsync(c, net->dn.counters, sizeof(c), hipMemcpyDeviceToHost, net->ctx->stream));
-  HIP_TRY(net->ctx, hipStreamSynchronize(net->ctx->stream));
+  int rc = read_counters(net, c);
+  if (rc) return rc;
   if (c[kCntOverflow])
     return fail(net->ctx, SZSNN_ERR_OVERFLOW,
                 "due-queue overflow: spikes were dropped (raise queue_capacity or use 0 for worst case)");
@@ -408,7 +421,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
       guard(dev_alloc(net, &acc, (size_t)2 * n.npad * acc_slots)) && guard(dev_alloc(net, &pulled, 1)) && guard(dev_alloc(net, &keys, T)) &&
       guard(dev_alloc(net, &roff, nrows + 1)) && guard(dev_alloc(net, &dsyn, S)) &&
       guard(dev_alloc(net, &dbits, T)) && guard(dev_alloc(net, &qc, n.R)) &&
-      guard(dev_alloc(net, &qr, (size_t)n.R * cap)) && guard(dev_alloc(net, &counters, kCntSlots)) &&
+      guard(dev_alloc(net, &qr, (size_t)n.R * cap)) && guard(dev_alloc(net, &counters, kCntWords)) &&
       guard(dev_alloc(net, &net->t_base, 1)) && guard(dev_alloc(net, &net->forced_bits, n.W)) &&
       guard(dev_alloc(net, &net->step_bitmap, n.W));
   if (rc == SZSNN_OK && d->noise_amp) guard(dev_alloc(net, &noise, N));
@@ -468,7 +481,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     he = hipMemsetAsync(acc, 0, (size_t)2 * n.npad * 8 * acc_slots, s);
     if (he == hipSuccess) he = hipMemsetAsync(pulled, 0, 4, s);
     if (he == hipSuccess) he = hipMemsetAsync(qc, 0, n.R * 4, s);
-    if (he == hipSuccess) he = hipMemsetAsync(counters, 0, kCntSlots * 8, s);
+    if (he == hipSuccess) he = hipMemsetAsync(counters, 0, kCntWords * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->t_base, 0, 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->forced_bits, 0, (size_t)n.W * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(hist, 0, (size_t)n.R * n.W * 8, s);
@@ -772,9 +785,8 @@ int szsnn_set_vars(szsnn_net* net, const float* host_in) {
 int szsnn_get_counters(szsnn_net* net, szsnn_counters* out) {
   if (!net || !out) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_counters: bad argument");
   unsigned long long c[kCntSlots];
-  szsnn_ctx* ctx = net->ctx;
-  HIP_TRY(ctx, hipMemcpyAsync(c, net->dn.counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  int rc = read_counters(net, c);
+  if (rc) return rc;
   out->cycle = net->t_host;
   out->events = c[kCntEvents];
   out->spikes = c[kCntSpikes];
@@ -786,7 +798,7 @@ int szsnn_get_counters(szsnn_net* net, szsnn_counters* out) {
 
 int szsnn_reset_counters(szsnn_net* net) {
   if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_reset_counters: net is NULL");
-  HIP_TRY(net->ctx, hipMemsetAsync(net->dn.counters, 0, kCntSlots * 8, net->ctx->stream));
+  HIP_TRY(net->ctx, hipMemsetAsync(net->dn.counters, 0, kCntWords * 8, net->ctx->stream));
   HIP_TRY(net->ctx, hipStreamSynchronize(net->ctx->stream));
   return SZSNN_OK;
 }
@@ -914,7 +926,7 @@ std::vector<Region> state_regions(szsnn_net* net) {
       {n.pulled, 4},
       {n.q_count, (size_t)n.R * 4},
       {n.q_rows, (size_t)n.R * n.q_cap * 4},
-      {n.counters, (size_t)kCntSlots * 8},
+      {n.counters, kCntWords * 8},
       {net->t_base, 8},
       {n.hist, (size_t)n.R * n.W * 8},
   };
