@@ -318,3 +318,26 @@ def test_pull_gather_equals_push_gather(ctx, which):
         o, ro, _ = _oracle_run(net, steps, 32)
         np.testing.assert_array_equal(rq, ro)
         assert cq["events"] == o.events
+
+
+def test_persistent_mode_pull_volley_equals_graph(ctx):
+    """The persistent kernel's pull path (dense gather inside the cooperative
+    loop) against the graph mode, under a volley that makes every step pull."""
+    from szsnn_amd._lib import MODE_PERSISTENT
+    from szsnn_amd.configs import if_cond_exp_100k
+    from szsnn_amd.simulation import Simulation
+    net = if_cond_exp_100k(n_exc=4000, n_inh=1000, k_exc=200, k_inh=50)
+    steps = 80
+    runs = []
+    for mode in (0, MODE_PERSISTENT):
+        s = Simulation(ctx, net)
+        if mode:
+            s.set_mode(MODE_PERSISTENT, 256, 0)
+        s.set_gather_policy(0.0)   # pull every step
+        s.set_volley(2500)
+        s.set_recording(0, steps)
+        s.run(steps)
+        runs.append((s.raster(), s.get_vars(), s.counters()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2]["events"] == runs[1][2]["events"] > 0
