@@ -437,7 +437,7 @@ class SpikeRecorder {
   SpikeRecorder(const std::string& filename, size_t recordStart = 0, size_t recordStop = 0)
       : out_(filename), start_(recordStart), stop_(recordStop) {
     out_ << "#spikes by cycles\n#record start at: " << start_;
-    if (stop_) out_ << " stop at: " << stop_;
+    if (stop_) out_ << "   stop at: " << stop_;
     out_ << "\n";
   }
   void recordCycleSpikes(size_t cycle, const std::vector<unsigned>& spikes) {
