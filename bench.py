@@ -32,6 +32,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="scaled-down network (smoke use only)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tiling")
@@ -42,6 +43,57 @@ def parse():
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def run_adjoint(args, log):
+    """Differentiable feedforward network (B=64, 1000 -> 2000 IF_curr_exp -> 100
+    readouts, T=200, checkpoint every 20 steps): HIP forward+backward per batch
+    next to the FP64 torch autograd restatement on the host CPU."""
+    import numpy as np
+    import torch
+    from szsnn_amd.adjoint import make_problem, snn_loss
+    P = make_problem()
+    dev = torch.device("cuda:0")
+    t = {k: torch.tensor(P[k], dtype=torch.float32, device=dev).contiguous()
+         for k in ("W1", "W2", "X", "target", "params")}
+
+    def once():
+        W1 = t["W1"].clone().requires_grad_(True)
+        W2 = t["W2"].clone().requires_grad_(True)
+        loss, _, bits = snn_loss(W1, W2, t["X"], t["target"], t["params"], P["alpha"], K=20)
+        loss.backward()
+        return W1.grad, W2.grad, bits
+
+    for _ in range(2):
+        once()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g1, g2, bits = once()
+    torch.cuda.synchronize()
+    gpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out = {"batch": 64, "steps": 200, "n_in": 1000, "n_hidden": 2000, "n_out": 100, "checkpoint_every": 20,
+           "gpu_ms_per_batch": gpu_ms}
+    if not args.no_cpu:
+        from oracle import adjoint_ref
+        T, B, H = 200, 64, 2000
+        b = bits.cpu().numpy().view(np.uint64)
+        u = np.unpackbits(b.view(np.uint8).reshape(T, -1), axis=1, bitorder="little")[:, :B * H]
+        spikes = torch.tensor(u.reshape(T, B, H).astype(bool))
+        d = {k: torch.tensor(P[k], dtype=torch.float64) for k in ("W1", "W2", "X", "target", "params")}
+        W1 = d["W1"].clone().requires_grad_(True)
+        W2 = d["W2"].clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        loss, _ = adjoint_ref.forward(W1, W2, d["X"], d["target"], d["params"], P["alpha"], spikes=spikes)
+        loss.backward()
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        rel1 = float((g1.cpu().double() - W1.grad).norm() / W1.grad.norm())
+        rel2 = float((g2.cpu().double() - W2.grad).norm() / W2.grad.norm())
+        out.update({"cpu_fp64_autograd_ms_per_batch": cpu_ms, "cpu_threads": torch.get_num_threads(),
+                    "rel_err_dW1": rel1, "rel_err_dW2": rel2})
+    log(f"[bench] adjoint: {out}")
+    return out
 
 
 def run_batched(args, ctx, log):
@@ -217,6 +269,7 @@ def main():
 
     del sim
     batched = None if args.no_batched else run_batched(args, ctx, log)
+    adjoint = None if args.no_adjoint else run_adjoint(args, log)
 
     cpu = None
     if not args.no_cpu:
@@ -260,6 +313,7 @@ def main():
                      "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6},
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
 
