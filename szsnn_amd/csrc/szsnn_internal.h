@@ -49,6 +49,7 @@ struct DevNet {
                             // slot 0 always, slots 1.. only on steps that pulled
   uint32_t acc_slots;       // G: delay groups of the pull gather (>= 1)
   uint32_t* pulled;         // set by gather(t): 1 if it pulled, read by update(t)
+  uint32_t* barrier;        // grid barrier words of the persistent loop (kBarWords)
   const uint64_t* keys;     // [T] random-stream keys
   const float* noise_amp;   // [N] or null
   const float* src_rate;    // [M] or null
@@ -95,6 +96,32 @@ enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kC
 // Counters are striped over kCntStripes cache lines (128 B apart) so that
 // blocks adding to one counter do not serialise on a single address; the host
 // sums the stripes.  counters = [kCntSlots][kCntStripes][16] u64.
+// Agent-coherent loads and stores (relaxed agent-scope atomics, `sc1` on
+// gfx950) for the state that one workgroup writes and another reads within a
+// launch: due-queue counts and rows, spike history words, accumulators and
+// the pull flag.  They are not served from the per-CU caches nor left dirty
+// in a per-XCD L2, so the persistent loop's grid barrier needs no L2
+// write-back or invalidate, only each wave's outstanding accesses completed.
+template <class T>
+__device__ inline T ld_agent(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void st_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier of the persistent loop: kBarLeaves leaf counters, a root
+// counter and a generation word, each on its own 128-byte line.
+constexpr uint32_t kBarLeaves = 16;
+constexpr uint32_t kBarStride = 32;  // u32 words per line
+constexpr uint32_t kBarWords = (kBarLeaves + 2) * kBarStride;
+// Spin limit of a barrier wait (about a second); reaching it (never with a
+// co-resident grid) adds kBarTimeout to the overflow counter and ends the
+// kernel.  Queue overflows add at most kCntStripes, far below kBarTimeout.
+constexpr uint32_t kBarSpinLimit = 1u << 22;
+constexpr unsigned long long kBarTimeout = 1ull << 32;
+
 constexpr uint32_t kCntStripes = 64;
 constexpr uint32_t kCntStride = 16;
 constexpr size_t kCntWords = (size_t)kCntSlots * kCntStripes * kCntStride;

@@ -14,8 +14,6 @@
 // Arithmetic is FP32 with one written evaluation order; the library is built
 // with -ffp-contract=off and IEEE division, so results are bit-identical to
 // the FP32 CPU restatement of the same equations.
-#include <hip/hip_cooperative_groups.h>
-
 #include "szsnn_internal.h"
 
 namespace szsnn {
@@ -124,24 +122,26 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
   const uint32_t wave = threadIdx.x >> 6;
   if (threadIdx.x < 64) sm.cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) sm.spikes = sm.drive = 0;
-  if (id == 0) n.q_count[t % n.R] = 0;  // slot gathered this step
+  if (id == 0) st_agent(&n.q_count[t % n.R], 0u);  // slot gathered this step
   __syncthreads();
   bool spike = false;
   uint64_t bits = 0;
   uint32_t kdrive = 0;
   if (id < n.T) {
+    // loaded up front so a spiking lane does not wait for it after the update
+    const uint64_t dbits = n.delay_bits[id];
     bool forced = id < n.volley;
     if (n.forced_bits) forced = forced || ((n.forced_bits[id >> 6] >> (id & 63)) & 1ull);
     if (id < n.N) {
-      long long ae = n.acc[id];
-      long long ai = n.acc[n.npad + id];
-      n.acc[id] = 0;
-      n.acc[n.npad + id] = 0;
-      if (n.acc_slots > 1 && *n.pulled) {
+      long long ae = ld_agent(&n.acc[id]);
+      long long ai = ld_agent(&n.acc[n.npad + id]);
+      st_agent(&n.acc[id], 0ll);
+      st_agent(&n.acc[n.npad + id], 0ll);
+      if (n.acc_slots > 1 && ld_agent(n.pulled)) {
         // pulled step: the delay groups' partial sums (rewritten by every pull)
         for (uint32_t g = 1; g < n.acc_slots; ++g) {
-          ae += n.acc[(size_t)g * 2 * n.npad + id];
-          ai += n.acc[(size_t)g * 2 * n.npad + n.npad + id];
+          ae += ld_agent(&n.acc[(size_t)g * 2 * n.npad + id]);
+          ai += ld_agent(&n.acc[(size_t)g * 2 * n.npad + n.npad + id]);
         }
       }
       if (n.drive_k) {
@@ -175,14 +175,14 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
         spike = spike || (rng_draw(n.keys[id], t, 3) < n.src_thresh[s]);
       }
     }
-    if (spike) bits = n.delay_bits[id];
+    bits = spike ? dbits : 0;
   }
   // dense spike output and history, one u64 word per wavefront
   // (storeDenseSpikes, IF_curr_exp_update_kernel.cl:58-94)
   const uint64_t mask = __ballot(spike);
   const uint32_t word = id >> 6;
   if (lane == 0 && word < n.W) {
-    n.hist[(t % n.R) * n.W + word] = mask;
+    st_agent(&n.hist[(t % n.R) * n.W + word], mask);
     if (n.raster && t >= n.rec_first && t - n.rec_first < n.rec_steps)
       n.raster[(t - n.rec_first) * n.W + word] = mask;
     if (mask) atomicAdd(&sm.spikes, (unsigned long long)__popcll(mask));
@@ -222,7 +222,7 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
       const uint64_t pos = (uint64_t)sm.base[k] + sm.woff[wave][k] + (uint64_t)__popcll(m & lt);
       const uint32_t slot = (uint32_t)((t + k + 1) % n.R);
       if (pos < n.q_cap)
-        n.q_rows[(uint64_t)slot * n.q_cap + pos] = id * n.D + k;
+        st_agent(&n.q_rows[(uint64_t)slot * n.q_cap + pos], id * n.D + k);
       else
         atomicOr(counter(n.counters, kCntOverflow, blockIdx.x), 1ull);
     }
@@ -272,24 +272,27 @@ struct GatherSmem {
   uint32_t wave[BLK / 64 + 1];
 };
 
-// gather(t) for blocks `block` of `nblocks`: BLK due rows per chunk; their
-// (start, length) are staged in LDS and scanned, then the block walks the
-// chunk's synapses contiguously so each row segment is read with coalesced
-// 8-byte loads.
+// gather(t) for blocks `block` of `nblocks`: chunks of up to BLK due rows;
+// their (start, length) are staged in LDS and scanned, then the block walks
+// the chunk's synapses contiguously so each row segment is read with
+// coalesced 8-byte loads.  The chunk shrinks to ceil(rows / nblocks) when
+// there are few due rows, so a sparse step is spread over all blocks instead
+// of queueing on the first ones.
 template <int BLK>
 __device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, uint32_t nblocks,
                                    GatherSmem<BLK>& sm) {
   const uint32_t slot = (uint32_t)(t % n.R);
   // after an overflow the count runs past the slot's capacity (entries beyond
   // it were never written): read only what is there
-  const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[slot], n.q_cap);
+  const uint32_t nrows = (uint32_t)min((uint64_t)ld_agent(&n.q_count[slot]), n.q_cap);
   const uint32_t* rows = n.q_rows + (uint64_t)slot * n.q_cap;
   unsigned long long delivered = 0, rows_done = 0;
-  for (uint32_t chunk = block * BLK; chunk < nrows; chunk += nblocks * BLK) {
+  const uint32_t per = min((uint32_t)BLK, max(1u, (nrows + nblocks - 1) / nblocks));
+  for (uint32_t chunk = block * per; chunk < nrows; chunk += nblocks * per) {
     const uint32_t r = chunk + threadIdx.x;
     uint32_t start = 0, len = 0;
-    if (r < nrows) {
-      const uint32_t row = rows[r];
+    if (threadIdx.x < per && r < nrows) {
+      const uint32_t row = ld_agent(&rows[r]);
       start = n.row_off[row];
       len = n.row_off[row + 1] - start;
     }
@@ -307,7 +310,7 @@ __device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, 
       atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)w);
     }
     delivered += total;
-    rows_done += min((uint32_t)BLK, nrows - chunk);
+    rows_done += min(per, nrows - chunk);
     __syncthreads();
   }
   if (threadIdx.x == 0 && rows_done) {
@@ -340,7 +343,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     if (s0 == s1) continue;
     const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
     __syncthreads();  // the previous delay's readers are done with `bits`
-    for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = src[w];
+    for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = ld_agent(&src[w]);
     __syncthreads();
     const uint32_t base_id = lo * 64;
     const uint32_t lane = __lane_id();
@@ -413,8 +416,8 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
     const uint32_t tgt = b * kPullTB + i;
     if (tgt < n.N) {
-      out[tgt] = acc[i];
-      out[n.npad + tgt] = acc[kPullTB + i];
+      st_agent(&out[tgt], acc[i]);
+      st_agent(&out[n.npad + tgt], acc[kPullTB + i]);
     }
   }
   // event count: one atomic per wave
@@ -430,10 +433,10 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   extern __shared__ uint64_t dyn[];
   const uint64_t t = *n.t_base + j;
   if (n.pull_off) {
-    const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[t % n.R], n.q_cap);
+    const uint32_t nrows = (uint32_t)min((uint64_t)ld_agent(&n.q_count[t % n.R]), n.q_cap);
     const bool pull = nrows >= n.dense_rows;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-      *n.pulled = pull ? 1u : 0u;
+      st_agent(n.pulled, pull ? 1u : 0u);
       if (pull && nrows) atomicAdd(counter(n.counters, kCntRows, 0), (unsigned long long)nrows);
     }
     if (pull) {
@@ -447,18 +450,69 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
 }
 
+// Grid barrier for a co-resident (cooperative) grid.  Arrival: block b adds
+// one to leaf counter b % kBarLeaves; the leaf's last block resets it and
+// adds one to the root; the root's last block resets it and advances the
+// generation word.  Blocks wait for the generation to change.  Spreading
+// arrivals over the leaves avoids funnelling every block through one
+// contended address.  State shared across blocks is accessed agent-coherently
+// (ld_agent / st_agent / device atomics), so the barrier only has to make
+// every wave's accesses complete before its block arrives (s_waitcnt 0); an
+// agent-scope fence here would write back and invalidate the XCD's L2 once per
+// block and barrier, which costs microseconds.  The wait is bounded: at the
+// limit (never reached by a co-resident grid) it records kBarTimeout in the
+// overflow counter and returns false, and the caller leaves its step loop.
+// sh[0] is the block's copy of the generation, sh[1] its failure flag.
+__device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long long* err) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t nb = gridDim.x;
+    const uint32_t leaf = blockIdx.x % kBarLeaves;
+    const uint32_t leaves = min(nb, kBarLeaves);
+    const uint32_t leaf_size = (nb - leaf + kBarLeaves - 1) / kBarLeaves;
+    uint32_t* gen = bar + (kBarLeaves + 1) * kBarStride;
+    const uint32_t g = sh[0];
+    if (atomicAdd(bar + leaf * kBarStride, 1u) == leaf_size - 1) {
+      atomicExch(bar + leaf * kBarStride, 0u);
+      if (atomicAdd(bar + kBarLeaves * kBarStride, 1u) == leaves - 1) {
+        atomicExch(bar + kBarLeaves * kBarStride, 0u);
+        atomicAdd(gen, 1u);
+      }
+    }
+    uint32_t spins = 0;
+    while (ld_agent(gen) == g) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kBarSpinLimit) {
+        atomicOr(err, kBarTimeout);
+        sh[1] = 1;
+        break;
+      }
+    }
+    sh[0] = g + 1;
+  }
+  __syncthreads();
+  return sh[1] == 0;
+}
+
 // Persistent step loop: one cooperative launch runs `nsteps` steps; grid
 // barriers separate gather(t) from update(t) and update(t) from gather(t+1).
 template <int MODEL, int BLK>
 __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nsteps, uint64_t* t_base) {
   __shared__ GatherSmem<BLK> sm;
   __shared__ UpdateSmem usm;
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  __shared__ uint32_t bar_sh[2];
+  if (threadIdx.x == 0) {
+    bar_sh[0] = ld_agent(n.barrier + (kBarLeaves + 1) * kBarStride);
+    bar_sh[1] = 0;
+  }
+  unsigned long long* err = counter(n.counters, kCntOverflow, 0);
   const uint64_t t0 = *t_base;
   extern __shared__ uint64_t dyn[];
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
-    const uint32_t nrows = (uint32_t)min((uint64_t)n.q_count[t % n.R], n.q_cap);
+    const uint32_t nrows = (uint32_t)min((uint64_t)ld_agent(&n.q_count[t % n.R]), n.q_cap);
     const bool pull = n.pull_off && nrows >= n.dense_rows;
     if (pull) {
       for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.acc_slots; bg += gridDim.x) {
@@ -469,11 +523,11 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
     } else {
       gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
     }
-    if (n.pull_off && blockIdx.x == 0 && threadIdx.x == 0) *n.pulled = pull ? 1u : 0u;
-    grid.sync();
+    if (n.pull_off && blockIdx.x == 0 && threadIdx.x == 0) st_agent(n.pulled, pull ? 1u : 0u);
+    if (!grid_barrier(n.barrier, bar_sh, err)) return;
     for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
       update_block<MODEL>(n, t, base + threadIdx.x, usm);
-    grid.sync();
+    if (!grid_barrier(n.barrier, bar_sh, err)) return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
 }

@@ -222,6 +222,10 @@ int check_overflow(szsnn_net* net) {
   unsigned long long c[kCntSlots];
   int rc = read_counters(net, c);
   if (rc) return rc;
+  if (c[kCntOverflow] >= kBarTimeout) {
+    (void)hipMemset(net->dn.barrier, 0, kBarWords * 4);
+    return fail(net->ctx, SZSNN_ERR_STATE, "persistent loop: a grid barrier wait timed out (grid not co-resident)");
+  }
   if (c[kCntOverflow])
     return fail(net->ctx, SZSNN_ERR_OVERFLOW,
                 "due-queue overflow: spikes were dropped (raise queue_capacity or use 0 for worst case)");
@@ -407,6 +411,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   float *params = nullptr, *vars = nullptr, *noise = nullptr, *src_rate = nullptr, *src_last = nullptr;
   long long* acc = nullptr;
   uint32_t* pulled = nullptr;
+  uint32_t* barrier = nullptr;
   // delay groups of the pull gather: enough (block, group) work items to fill the chip
   uint32_t acc_slots = 1;
   if (!pull_off.empty()) {
@@ -418,7 +423,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint2* dsyn = nullptr;
   unsigned long long* counters = nullptr;
   guard(dev_alloc(net, &params, (size_t)np * n.npad)) && guard(dev_alloc(net, &vars, (size_t)nv * n.npad)) &&
-      guard(dev_alloc(net, &acc, (size_t)2 * n.npad * acc_slots)) && guard(dev_alloc(net, &pulled, 1)) && guard(dev_alloc(net, &keys, T)) &&
+      guard(dev_alloc(net, &acc, (size_t)2 * n.npad * acc_slots)) && guard(dev_alloc(net, &pulled, 1)) && guard(dev_alloc(net, &barrier, kBarWords)) && guard(dev_alloc(net, &keys, T)) &&
       guard(dev_alloc(net, &roff, nrows + 1)) && guard(dev_alloc(net, &dsyn, S)) &&
       guard(dev_alloc(net, &dbits, T)) && guard(dev_alloc(net, &qc, n.R)) &&
       guard(dev_alloc(net, &qr, (size_t)n.R * cap)) && guard(dev_alloc(net, &counters, kCntWords)) &&
@@ -480,6 +485,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     hipStream_t s = ctx->stream;
     he = hipMemsetAsync(acc, 0, (size_t)2 * n.npad * 8 * acc_slots, s);
     if (he == hipSuccess) he = hipMemsetAsync(pulled, 0, 4, s);
+    if (he == hipSuccess) he = hipMemsetAsync(barrier, 0, kBarWords * 4, s);
     if (he == hipSuccess) he = hipMemsetAsync(qc, 0, n.R * 4, s);
     if (he == hipSuccess) he = hipMemsetAsync(counters, 0, kCntWords * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->t_base, 0, 8, s);
@@ -497,6 +503,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.acc = acc;
   n.acc_slots = acc_slots;
   n.pulled = pulled;
+  n.barrier = barrier;
   n.keys = keys;
   n.noise_amp = noise;
   n.src_rate = src_rate;
@@ -966,7 +973,7 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
   const int cus = std::max(ctx->cu_count, 1);
   const uint32_t ublocks[] = {64, 128, 256, 512, 1024};
   const uint32_t gblocks[] = {128, 256, 512, 1024};
-  const uint32_t gmult[] = {2, 4, 8, 16};
+  const uint32_t gmult[] = {1, 2, 4, 8, 16};
   Tiling best = net->tiling;
   float best_ms = 1e30f;
   hipEvent_t e0, e1;

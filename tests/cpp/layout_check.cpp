@@ -64,18 +64,20 @@ static szsnn_net_desc desc_of(const Net& n) {
   return d;
 }
 
-// push replay: rows due = list of row ids; BLK-row chunks as gather_body
+// push replay: rows due = list of row ids; chunks of `per` <= BLK rows as
+// gather_body
 static void push_replay(const HostLayout& L, uint32_t N, uint64_t S, const std::vector<uint32_t>& rows,
                         uint64_t q_cap, uint32_t count_in_slot, std::vector<long long>& acc, uint32_t BLK,
                         uint32_t nblocks) {
   const uint32_t nrows = (uint32_t)std::min<uint64_t>(count_in_slot, q_cap);
+  const uint32_t per = std::min(BLK, std::max(1u, (nrows + nblocks - 1) / nblocks));
   std::vector<uint32_t> sstart(BLK), spref(BLK + 1);
   for (uint32_t block = 0; block < nblocks; ++block)
-    for (uint32_t chunk = block * BLK; chunk < nrows; chunk += nblocks * BLK) {
+    for (uint32_t chunk = block * per; chunk < nrows; chunk += nblocks * per) {
       for (uint32_t tid = 0; tid < BLK; ++tid) {
         const uint32_t r = chunk + tid;
         uint32_t start = 0, len = 0;
-        if (r < nrows) {
+        if (tid < per && r < nrows) {
           CHECK(r < rows.size() && r < q_cap, "queue read r=%u", r);
           const uint32_t row = rows[r];
           CHECK((uint64_t)row + 1 < L.row_off.size(), "row_off read row=%u", row);
@@ -230,11 +232,12 @@ static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D,
       if (fired[t - k - 1][id] && ((L.delay_bits[id] >> k) & 1)) rows.push_back(id * D + k);
   }
   CHECK(rows.size() <= L.nonempty, "due rows exceed the worst-case capacity");
-  for (uint32_t BLK : {128u, 256u, 1024u}) {
-    std::vector<long long> push(2 * (size_t)N, 0);
-    push_replay(L, N, Ss, rows, std::max<uint64_t>(L.nonempty, 1), (uint32_t)rows.size(), push, BLK, 7);
-    CHECK(push == naive, "push sum differs (BLK=%u)", BLK);
-  }
+  for (uint32_t BLK : {128u, 256u, 1024u})
+    for (uint32_t nb : {1u, 7u, 300u}) {
+      std::vector<long long> push(2 * (size_t)N, 0);
+      push_replay(L, N, Ss, rows, std::max<uint64_t>(L.nonempty, 1), (uint32_t)rows.size(), push, BLK, nb);
+      CHECK(push == naive, "push sum differs (BLK=%u, blocks=%u)", BLK, nb);
+    }
   // overflowed slot: count beyond capacity is clamped, reads stay in bounds
   if (rows.size() > 4) {
     std::vector<uint32_t> part(rows.begin(), rows.begin() + 4);
