@@ -5,8 +5,10 @@ A "step" is one simulation step (gather -> update -> scatter) of the whole
 network.  Inputs (network, state) are resident in HBM before the timed region.
 Prints one JSON line (the driver's contract) with:
   value          synaptic events/s over the timed steps (natural dynamics)
-  roofline       gather kernel at the volley load (all neurons forced every
-                 step: every synapse row due each step), HIP-event timed
+  roofline       the pull gather kernel at the volley load (all neurons forced
+                 every step: every synapse delivered each step), HIP-event
+                 timed; algorithmic bytes = 8 B per delivered synapse entry +
+                 16 B per target and delay group written
   cpu_baseline   the FP32 CPU restatement (one thread) on a bounded sample
 """
 import argparse
@@ -222,22 +224,28 @@ def main():
     log(f"[bench] natural: {ms_per_step * 1e3:.2f} us/step, {ev_per_s:.3e} events/s, RTF {rtf:.2f}, "
         f"{rate_hz:.2f} Hz, {events / args.steps:.0f} events/step")
 
-    # per-kernel time on the natural dynamics (HIP events, same stream)
+    # per-kernel time on the natural dynamics (HIP events, same stream); the
+    # update kernel also pushes the step's spikes
     kn = 500
     cg0 = sim.counters()
     g_ms, u_ms = sim.time_kernels(kn)
     cg1 = sim.counters()
-    nat_g_bytes = 8 * (cg1["events"] - cg0["events"]) + 8 * (cg1["rows"] - cg0["rows"])
-    nat_u_bytes = 48 * net.n_neurons * kn
+    info = sim.info
+    # per neuron: params, vars read+written, its accumulator slot, key,
+    # synapse range and delay bits; per pushed synapse: its 8-byte entry
+    nat_u_bytes = ((4 * info.n_params + 8 * info.n_vars + 16 + 24) * net.n_neurons * kn
+                   + 8 * (cg1["events"] - cg0["events"]))
     natural = {
         "gather_us": g_ms * 1e3 / kn, "update_us": u_ms * 1e3 / kn,
-        "gather_gbs": nat_g_bytes / (g_ms * 1e-3) / 1e9 if g_ms > 0 else None,
         "update_gbs": nat_u_bytes / (u_ms * 1e-3) / 1e9 if u_ms > 0 else None,
     }
     log(f"[bench] natural kernels: {natural}")
 
     # volley: every neuron forced every step -> every (source, delay) row is due
     natural_tiling = (sim.tiling(), sim.mode())
+
+    default_policy = sim.gather_policy()["dense_fraction"]
+    groups = int(sim.info.pull_groups)
 
     def volley(fraction):
         sim.set_gather_policy(fraction)
@@ -250,18 +258,22 @@ def main():
         gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
         cv1 = sim.counters()
         sim.set_volley(0)
-        ev = cv1["events"] - cv0["events"]
-        rw = cv1["rows"] - cv0["rows"]
-        per = (8 * ev + 8 * rw) / args.volley_steps
+        ev = (cv1["events"] - cv0["events"]) / args.volley_steps
+        rw = (cv1["rows"] - cv0["rows"]) / args.volley_steps
         g_s = gv_ms * 1e-3 / args.volley_steps
-        return {"gather_us": g_s * 1e6, "update_us": uv_ms * 1e3 / args.volley_steps,
-                "events_per_launch": ev / args.volley_steps, "rows_per_launch": rw / args.volley_steps,
-                "alg_bytes_per_launch": per, "gbs": per / g_s / 1e9}
+        u_s = uv_ms * 1e-3 / args.volley_steps
+        return {"gather_us": g_s * 1e6, "update_us": u_s * 1e6, "events_per_launch": ev, "rows_per_launch": rw}
+    # push only: the update kernel pushes 1e8 synapses per step with atomics
     v_push = volley(2.0)
+    v_push["push_events_per_s"] = v_push["events_per_launch"] / (v_push["update_us"] * 1e-6)
     log(f"[bench] volley push-only: {v_push}")
-    v_auto = volley(0.3)
-    log(f"[bench] volley auto (pull): {v_auto}")
-    sim.set_gather_policy(0.3)
+    # default policy: every step is deferred and pulled
+    v_auto = volley(default_policy)
+    per = 8 * v_auto["events_per_launch"] + 16 * net.n_neurons * groups
+    v_auto["alg_bytes_per_launch"] = per
+    v_auto["gbs"] = per / (v_auto["gather_us"] * 1e-6) / 1e9
+    log(f"[bench] volley pull (policy {default_policy}): {v_auto}")
+    sim.set_gather_policy(default_policy)
     v_auto["tiling"] = sim.tiling()
     per_launch = v_auto["alg_bytes_per_launch"]
     g_avg_s = v_auto["gather_us"] * 1e-6
@@ -307,9 +319,10 @@ def main():
         "natural_kernels": natural,
         "natural_tiling": {**natural_tiling[0], **natural_tiling[1]},
         "volley_push_only": v_push,
+        "volley_pull": v_auto,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "gather_kernel (volley: all 100k neurons forced each step)",
+                     "kernel": "gather_kernel pull (volley: all 100k neurons forced each step)",
                      "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6},
         "cpu_baseline": cpu,
         "batched": batched,

@@ -79,7 +79,8 @@ typedef struct szsnn_net_desc {
   uint32_t drive_k;           /* binomial drive: number of CDF thresholds (0 = none, <= 32) */
   const uint64_t* drive_cdf;  /* [drive_k] events = #{j : draw(key, t, 2) >= cdf[j]} */
   float drive_w;              /* weight of one drive event, E channel */
-  uint32_t queue_capacity;    /* due-queue entries per slot; 0 = worst case (never overflows) */
+  uint32_t queue_capacity;    /* ignored: delivery goes through a ring of per-step accumulator
+                                 slots and has no queue that could overflow (reference D4) */
   int32_t pull_layout;        /* 0: also build the target-blocked layout for dense gathers when the
                                  network allows it (ids < 2^24, bounded source spans); -1: never */
 } szsnn_net_desc;
@@ -89,7 +90,7 @@ typedef struct szsnn_net_info {
   int32_t frac_bits;
   uint64_t n_syn_stored;      /* nonzero-weight synapses in the CSR */
   uint64_t n_rows;            /* (source, delay) rows = (N+M) * D */
-  uint64_t queue_capacity;    /* entries per due-queue slot */
+  uint64_t pull_groups;       /* delay groups of the pull gather (0: no pull layout) */
   uint64_t device_bytes;      /* device memory held by the network */
   uint64_t raster_words;      /* u64 words per raster row = ceil((N+M)/64) */
 } szsnn_net_info;
@@ -97,11 +98,11 @@ typedef struct szsnn_net_info {
 /* Counters since upload (or the last szsnn_reset_counters). */
 typedef struct szsnn_counters {
   uint64_t cycle;         /* completed steps (szsnn1.1/src/SzsnnTimer.cpp:66-71) */
-  uint64_t events;        /* synaptic events delivered by gather */
+  uint64_t events;        /* synaptic events delivered (counted at the consuming step) */
   uint64_t spikes;        /* spikes of neurons and sources */
   uint64_t drive_events;  /* binomial drive events (not synaptic) */
-  uint64_t overflow;      /* nonzero if a due queue overflowed */
-  uint64_t rows;          /* (source, delay) rows drained by gather */
+  uint64_t overflow;      /* >= 2^32 after a persistent-loop barrier timed out; else 0 */
+  uint64_t rows;          /* (source, delay) rows delivered */
 } szsnn_counters;
 
 /* ---- context / device runtime (replaces Context.hpp:77-303) ---- */
@@ -145,7 +146,7 @@ int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, ui
 /* Run n steps with no host input: HIP-graph-captured step loop.  When a raster
  * buffer is set (szsnn_set_recording), steps with cycle in
  * [rec_first, rec_first + rec_steps) write their bitmap to it.  Returns after
- * completion; the due-queue overflow flag is checked at the end. */
+ * completion; the error flag of the persistent loop is checked at the end. */
 int szsnn_run(szsnn_net* net, uint64_t n_steps);
 /* Same, but launched asynchronously: returns once queued on the stream. */
 int szsnn_run_async(szsnn_net* net, uint64_t n_steps);
@@ -174,18 +175,22 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
 int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_block, uint32_t gather_grid);
 int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_block, uint32_t* gather_grid);
 
-/* Gather policy: each step gathers by pushing the due rows (cost ~ events)
- * or, with the pull layout, by pulling every synapse of each target block
- * against the staged spike bitmaps (cost ~ synapses, no global atomics).  It
- * pulls when the due rows are at least dense_fraction x the nonempty rows;
- * dense_fraction <= 0 always pulls, > 1 never does.  Both give identical sums.
- * has_pull reports whether the layout was built. */
+/* Gather policy: the spikes of a step are either pushed by the update that
+ * produced them, each synapse into the accumulator slot of its delivery step
+ * (cost ~ events, global atomics), or, with the pull layout, deferred: each
+ * later step then pulls, per target block, the synapses of the deferred steps
+ * due at it against their staged spike bitmaps (cost ~ synapses, no global
+ * atomics).  A step is deferred when its predicted spikes (the previous
+ * step's plus the forced ids) reach dense_fraction x the ids with synapses;
+ * dense_fraction <= 0 always defers, > 1 never does.  Both give identical
+ * sums.  has_pull reports whether the layout was built. */
 int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction);
 int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_pull);
 
-/* Execution mode of szsnn_run.  GRAPH: each step is a gather launch and an
- * update launch, replayed from a captured HIP graph.  PERSISTENT: one
- * cooperative launch runs all steps, grid barriers between the phases; block
+/* Execution mode of szsnn_run.  GRAPH: each step is a gather launch (a no-op
+ * unless something is due for the pull) and an update launch, replayed from a
+ * captured HIP graph.  PERSISTENT: one cooperative launch runs all steps, a
+ * grid barrier after each update and one after each pull; block
  * 256/512/1024, grid 0 = the co-resident maximum (clamped to it). */
 #define SZSNN_MODE_GRAPH 0
 #define SZSNN_MODE_PERSISTENT 1

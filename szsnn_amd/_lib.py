@@ -71,7 +71,7 @@ class NetInfo(ctypes.Structure):
         ("frac_bits", ctypes.c_int32),
         ("n_syn_stored", ctypes.c_uint64),
         ("n_rows", ctypes.c_uint64),
-        ("queue_capacity", ctypes.c_uint64),
+        ("pull_groups", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
         ("raster_words", ctypes.c_uint64),
     ]

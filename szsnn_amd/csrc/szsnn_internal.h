@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "szsnn_constants.h"
+
 namespace szsnn {
 
 constexpr uint64_t kGolden = 0x9e3779b97f4a7c15ULL;
@@ -29,6 +31,13 @@ enum : int { kIfCurrExp = 0, kIzhikevich = 1, kIfCondExp = 2 };
 enum : int { kSrcNone = 0, kSrcConstRate = 1, kSrcPoisson = 2 };
 
 // Everything a step kernel reads, passed by value as one kernel argument.
+//
+// Delivery: update(t) pushes the synapses of the ids spiking at t straight
+// into a ring of accumulators, slot (t + k + 1) % R for API delay k, and
+// update(t') consumes slot t' % R.  A step whose spikes are predicted to be
+// dense is deferred instead: its spike bitmap stays in the history ring, the
+// step is stamped, and the pull gather of each later step t' sums, per target
+// block, the synapses of delay k = t' - s - 1 of every deferred step s.
 struct DevNet {
   int32_t model;
   int32_t source_model;
@@ -36,7 +45,7 @@ struct DevNet {
   uint32_t M;        // input sources
   uint32_t T;        // N + M ids
   uint32_t D;        // delay slots (API delay k < D)
-  uint32_t R;        // due-queue ring length = D + 1
+  uint32_t R;        // ring length = D + 1
   uint32_t W;        // raster words per step = ceil(T / 64)
   uint32_t npad;     // stride of the SoA arrays (N rounded up to 64)
   uint32_t drive_k;
@@ -45,10 +54,9 @@ struct DevNet {
   long long drive_w; // fixed-point weight of one drive event
   const float* params;      // [np][npad]
   float* vars;              // [nv][npad]
-  long long* acc;           // [acc_slots][2][npad] fixed-point E/I accumulators;
-                            // slot 0 always, slots 1.. only on steps that pulled
-  uint32_t acc_slots;       // G: delay groups of the pull gather (>= 1)
-  uint32_t* pulled;         // set by gather(t): 1 if it pulled, read by update(t)
+  long long* acc;           // [R][2][npad] fixed-point E/I input due at the slot's step
+  long long* acc_pull;      // [G][2][npad] or null: a pulling step's delay-group sums
+  uint32_t pull_groups;     // G: delay groups of the pull gather (>= 1)
   uint32_t* barrier;        // grid barrier words of the persistent loop (kBarWords)
   const uint64_t* keys;     // [T] random-stream keys
   const float* noise_amp;   // [N] or null
@@ -56,12 +64,14 @@ struct DevNet {
   float* src_last;          // [M] or null
   const uint64_t* src_thresh;  // [M] or null
   const uint64_t* drive_cdf;   // [drive_k] or null
-  const uint32_t* row_off;  // [T*D + 1] CSR row offsets, row = id*D + k
-  const uint2* syn;         // [S] (target, fixed-point weight as int32 bits)
-  const uint64_t* delay_bits;  // [T] bit k set iff row (id, k) is nonempty
-  uint32_t* q_count;        // [R] entries in each due-queue slot
-  uint32_t* q_rows;         // [R][q_cap] row ids due at the slot's step
-  uint64_t q_cap;
+  const uint32_t* syn_off;  // [T + 1] synapses of id at syn[syn_off[id] .. syn_off[id + 1])
+  const uint2* syn;         // [S] (target | k << kSynDelayShift, fixed-point weight as int32 bits)
+  const uint64_t* delay_bits;  // [T] bit k set iff id has a synapse of delay k
+  uint64_t* dstamp;         // [R] s + 1 in slot s % R when step s was deferred
+  unsigned long long* ring_cnt;  // [R][2][kRingStripes] events and rows due at the slot's step
+  unsigned long long* spk_ring;  // [4][kRingStripes] spikes of step s in slot s % 4
+  uint32_t defer_spikes;    // defer step t when its predicted spikes >= this (0xffffffff: never)
+  uint32_t forced_count;    // ids forced by a host step (for the prediction)
   uint64_t* raster;         // [rec_steps][W] or null
   uint64_t rec_first;
   uint64_t rec_steps;
@@ -73,35 +83,24 @@ struct DevNet {
   // pull layout (dense gather): target blocks of kPullTB neurons; synapses of
   // block b and delay k at pull_syn[pull_off[b*D+k] .. pull_off[b*D+k+1]),
   // sorted by target, each (source << kPullShift | target % kPullTB, weight);
-  // segments padded to even length with inert (weight 0) entries.
+  // segments padded to a multiple of kPullRun with inert (weight 0) entries.
   const uint32_t* pull_off; // [NB*D + 1] or null (no pull layout)
   const uint2* pull_syn;
   const uint32_t* pull_lo;  // [NB] first raster word of the block's sources
   uint32_t pull_nb;         // NB
   uint32_t pull_words;      // LDS words staged per block (max over blocks)
-  uint32_t dense_rows;      // gather pulls when the due rows >= this (0xffffffff: never)
 };
-
-#ifndef SZSNN_PULL_CONSTANTS
-#define SZSNN_PULL_CONSTANTS
-constexpr uint32_t kPullShift = 9;
-constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block
-constexpr uint32_t kPullMaxWords = 6144;         // 48 KiB of staged bitmap (sources spanning 393k ids)
-constexpr uint32_t kPullRun = 8;                 // consecutive synapses per lane
-#endif
 
 // Counter slots.
 enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntRows = 4, kCntSlots = 5 };
 
-// Counters are striped over kCntStripes cache lines (128 B apart) so that
-// blocks adding to one counter do not serialise on a single address; the host
-// sums the stripes.  counters = [kCntSlots][kCntStripes][16] u64.
 // Agent-coherent loads and stores (relaxed agent-scope atomics, `sc1` on
 // gfx950) for the state that one workgroup writes and another reads within a
-// launch: due-queue counts and rows, spike history words, accumulators and
-// the pull flag.  They are not served from the per-CU caches nor left dirty
-// in a per-XCD L2, so the persistent loop's grid barrier needs no L2
-// write-back or invalidate, only each wave's outstanding accesses completed.
+// launch: accumulators, spike history words, deferral stamps and the
+// delivery-time counter rings.  They are not served from the per-CU caches
+// nor left dirty in a per-XCD L2, so the persistent loop's grid barrier needs
+// no L2 write-back or invalidate, only each wave's outstanding accesses
+// completed.
 template <class T>
 __device__ inline T ld_agent(const T* p) {
   return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -118,10 +117,14 @@ constexpr uint32_t kBarStride = 32;  // u32 words per line
 constexpr uint32_t kBarWords = (kBarLeaves + 2) * kBarStride;
 // Spin limit of a barrier wait (about a second); reaching it (never with a
 // co-resident grid) adds kBarTimeout to the overflow counter and ends the
-// kernel.  Queue overflows add at most kCntStripes, far below kBarTimeout.
+// kernel.
 constexpr uint32_t kBarSpinLimit = 1u << 22;
 constexpr unsigned long long kBarTimeout = 1ull << 32;
 
+// Counters are striped over kCntStripes cache lines (128 B apart) so that
+// blocks adding to one counter do not serialise on a single address; the host
+// sums the stripes.  counters = [kCntSlots][kCntStripes][16] u64.  Events and
+// rows are counted when delivered (at the step that consumes them).
 constexpr uint32_t kCntStripes = 64;
 constexpr uint32_t kCntStride = 16;
 constexpr size_t kCntWords = (size_t)kCntSlots * kCntStripes * kCntStride;
@@ -129,8 +132,23 @@ __host__ __device__ inline unsigned long long* counter(unsigned long long* base,
   return base + ((size_t)slot * kCntStripes + (stripe % kCntStripes)) * kCntStride;
 }
 
+// Delivery-time counter rings: which = 0 events, 1 rows; each entry on its own
+// 128-byte line.
+constexpr uint32_t kRingStripes = 16;
+constexpr size_t kRingCntWords(uint32_t R) { return (size_t)R * 2 * kRingStripes * kCntStride; }
+constexpr size_t kSpkRingWords = (size_t)4 * kRingStripes * kCntStride;
+__host__ __device__ inline unsigned long long* ring_counter(unsigned long long* base, uint32_t slot, uint32_t which,
+                                                            uint32_t stripe) {
+  return base + (((size_t)slot * 2 + which) * kRingStripes + (stripe % kRingStripes)) * kCntStride;
+}
+__host__ __device__ inline unsigned long long* spk_counter(unsigned long long* base, uint64_t step, uint32_t stripe) {
+  return base + ((size_t)(step & 3) * kRingStripes + (stripe % kRingStripes)) * kCntStride;
+}
+
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
+// gather: the pull of step t (launches nothing without a pull layout).
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);
+// update: block 64, 128, 256, 512 or 1024.
 hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s);
 // One cooperative launch running nsteps steps (block 256/512/1024; grid must
 // not exceed persistent_max_blocks_per_cu(model, block) * CUs).

@@ -1,15 +1,18 @@
 // szsnn_kernels.hip -- per-step CDNA4 kernels of the SZSNN core (gfx950, wave64).
 //
 // Per step t (reference order, szsnn1.1/src/Simulation.ipp:471-483):
-//   gather(t)  : drain due-queue slot t % R; expand every due (source, delay)
-//                CSR row; add each synapse's fixed-point weight into the
-//                target's E or I accumulator.  Replaces gather_kernel.cl:67-200
-//                and the global half of scatter_kernel.cl:194-355.
-//   update(t)  : one id per lane, SoA state; neuron model or input-source model;
-//                spikes compacted per wavefront with ballot; each spiking id
-//                appends its nonempty delay rows to the due-queue slot of the
-//                step that will gather them (wave-aggregated atomics).  Replaces
-//                *_update_kernel.cl and the local half of scatter_kernel.cl:97-190.
+//   gather(t)  : only when a deferred (dense) step has synapses due at t: per
+//                target block, sum the due delays' synapses of the deferred
+//                steps' spiking sources (the pull layout), no atomics.
+//   update(t)  : one id per lane, SoA state; consume accumulator slot t % R;
+//                neuron model or input-source model; spike words by wavefront
+//                ballot into the history ring and raster; then the block
+//                pushes its spiking ids' synapses into the accumulator slots
+//                of their delivery steps t + k + 1 (fixed-point atomics), or,
+//                on a step predicted dense, stamps the step for the pull.
+//                Replaces *_update_kernel.cl, scatter_kernel.cl:97-355 and
+//                gather_kernel.cl:67-200 (their per-delay queues become the
+//                ring of per-delivery-step accumulator slots).
 //
 // Arithmetic is FP32 with one written evaluation order; the library is built
 // with -ffp-contract=off and IEEE division, so results are bit-identical to
@@ -34,11 +37,6 @@ __device__ inline int32_t gauss_int(uint64_t key, uint64_t t) {
 
 // Unit-variance scale of the centred Irwin-Hall sum: 0.5/sqrt(8*(2^32-1)/12).
 constexpr float kGaussScale = 0x1.3988e2p-17f;
-
-__device__ inline uint64_t lanemask_lt() {
-  const uint32_t lane = __lane_id();
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
 
 // IF_curr_exp (IF_curr_exp_update_kernel.cl:230-297) and IF_cond_exp (same
 // scheme, conductances in the current slots).  Returns spike.
@@ -103,45 +101,154 @@ __device__ inline bool izh_update(const float* __restrict__ p, float* __restrict
   return spike;
 }
 
-// LDS of one update block: per-delay reservation counts and bases, the
-// waves' offsets inside the block's reservation, and counter partial sums.
-struct UpdateSmem {
-  uint32_t cnt[64];
-  uint32_t base[64];
-  uint32_t woff[16][64];
-  unsigned long long spikes, drive;
-};
-
-// update(t) for one id per lane; every thread of the block must call it (it
-// synchronises the block).  Resets the due-queue slot gathered at t when
-// id == 0.  Counters and due-queue reservations are aggregated per block:
-// one global atomic per block and delay.
-template <int MODEL>
-__device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem& sm) {
+// Block-wide exclusive scan of one u32 per thread; returns the total.
+template <int BLK>
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, uint32_t* s_wave) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = threadIdx.x >> 6;
-  if (threadIdx.x < 64) sm.cnt[threadIdx.x] = 0;
-  if (threadIdx.x == 0) sm.spikes = sm.drive = 0;
-  if (id == 0) st_agent(&n.q_count[t % n.R], 0u);  // slot gathered this step
+  uint32_t inc = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off);
+    if (lane >= (uint32_t)off) inc += y;
+  }
+  if (lane == 63) s_wave[wave] = inc;
   __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int w = 0; w < BLK / 64; ++w) {
+      const uint32_t v = s_wave[w];
+      s_wave[w] = acc;
+      acc += v;
+    }
+    s_wave[BLK / 64] = acc;
+  }
+  __syncthreads();
+  s_pref[threadIdx.x] = s_wave[wave] + inc - x;
+  if (threadIdx.x == 0) s_pref[BLK] = s_wave[BLK / 64];
+  __syncthreads();
+  return s_pref[BLK];
+}
+
+// Bit k set iff step t - k - 1 was deferred, i.e. its synapses of delay k
+// are pulled at t.  Every lane of the calling wave gets the mask.
+__device__ inline uint64_t due_mask(const DevNet& n, uint64_t t) {
+  if (!n.pull_off) return 0;
+  const uint32_t k = __lane_id();
+  bool due = false;
+  if (k < n.D && t >= (uint64_t)k + 1) {
+    const uint64_t s = t - k - 1;
+    due = ld_agent(&n.dstamp[s % n.R]) == s + 1;
+  }
+  return __ballot(due);
+}
+
+// Spikes expected at step t: those of step t - 1 plus the forced ids.  Every
+// lane of the calling wave gets the sum.
+__device__ inline unsigned long long predicted_spikes(const DevNet& n, uint64_t t) {
+  unsigned long long c = 0;
+  const uint32_t lane = __lane_id();
+  if (t >= 1 && lane < kRingStripes) c = ld_agent(spk_counter(n.spk_ring, t - 1, lane));
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+  return c + n.volley + n.forced_count;
+}
+
+// LDS of one update block.
+template <int BLK>
+struct UpdateSmem {
+  uint32_t cnt[64];          // the block's rows per delay (spiking ids with that delay)
+  uint32_t ev[64];           // the block's pushed synapses per delay
+  uint32_t start[BLK];       // push: first synapse of each lane's id
+  uint32_t pref[BLK + 1];    // push: exclusive scan of the lanes' synapse counts
+  uint32_t wave[BLK / 64 + 1];
+  unsigned long long spikes, drive;
+  uint64_t due;              // due_mask(t)
+  uint32_t defer;            // step t's spikes go to the pull gather
+};
+
+// Block 0's bookkeeping at step t (one wave): move the events and rows due at
+// t from the delivery ring into the counters, and clear the spike-count slot
+// that step t + 1 will fill.
+__device__ inline void fold_rings(const DevNet& n, uint64_t t, uint32_t slot) {
+  const uint32_t lane = __lane_id();
+  unsigned long long e = 0, r = 0;
+  if (lane < kRingStripes) {
+    unsigned long long* p = ring_counter(n.ring_cnt, slot, 0, lane);
+    e = ld_agent(p);
+    if (e) st_agent(p, 0ull);
+  } else if (lane < 2 * kRingStripes) {
+    unsigned long long* p = ring_counter(n.ring_cnt, slot, 1, lane - kRingStripes);
+    r = ld_agent(p);
+    if (r) st_agent(p, 0ull);
+  } else if (lane < 3 * kRingStripes) {
+    st_agent(spk_counter(n.spk_ring, t + 1, lane - 2 * kRingStripes), 0ull);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    e += __shfl_xor(e, off);
+    r += __shfl_xor(r, off);
+  }
+  if (lane == 0) {
+    if (e) atomicAdd(counter(n.counters, kCntEvents, 0), e);
+    if (r) atomicAdd(counter(n.counters, kCntRows, 0), r);
+  }
+}
+
+// update(t) for one id per lane; every thread of the block must call it (it
+// synchronises the block).  Consumes accumulator slot t % R (plus the pull
+// groups' sums when the step pulled), updates the neuron or source, writes
+// the spike word of each wave to the history ring and the raster, and then
+// either pushes the block's spiking ids' synapses into the accumulator slots
+// of their delivery steps or, on a deferred step, stamps the step for the
+// pull gather.  Rows and events are added to the delivery-time counter ring
+// with one atomic per (block, delay).
+template <int MODEL, int BLK>
+__device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
+  const uint32_t lane = __lane_id();
+  const uint32_t slot = (uint32_t)(t % n.R);
+  long long ring_e = 0, ring_i = 0;
+  uint32_t r0 = 0, r1 = 0;
+  uint64_t dbits = 0;
+  long long* ring = n.acc + (size_t)slot * 2 * n.npad;
+  if (id < n.N) {
+    ring_e = ld_agent(&ring[id]);
+    ring_i = ld_agent(&ring[n.npad + id]);
+  }
+  if (id < n.T) {
+    r0 = n.syn_off[id];
+    r1 = n.syn_off[id + 1];
+    dbits = n.delay_bits[id];
+  }
+  if (threadIdx.x < 64) {
+    sm.cnt[threadIdx.x] = 0;
+    sm.ev[threadIdx.x] = 0;
+    const uint64_t due = due_mask(n, t);
+    const bool defer = n.pull_off && predicted_spikes(n, t) >= n.defer_spikes;
+    if (threadIdx.x == 0) {
+      sm.due = due;
+      sm.defer = defer ? 1u : 0u;
+      sm.spikes = sm.drive = 0;
+    }
+    if (blockIdx.x == 0) fold_rings(n, t, slot);
+  }
+  __syncthreads();
+  const uint64_t due = sm.due;
+  const bool defer = sm.defer != 0;
   bool spike = false;
-  uint64_t bits = 0;
   uint32_t kdrive = 0;
   if (id < n.T) {
-    // loaded up front so a spiking lane does not wait for it after the update
-    const uint64_t dbits = n.delay_bits[id];
     bool forced = id < n.volley;
     if (n.forced_bits) forced = forced || ((n.forced_bits[id >> 6] >> (id & 63)) & 1ull);
     if (id < n.N) {
-      long long ae = ld_agent(&n.acc[id]);
-      long long ai = ld_agent(&n.acc[n.npad + id]);
-      st_agent(&n.acc[id], 0ll);
-      st_agent(&n.acc[n.npad + id], 0ll);
-      if (n.acc_slots > 1 && ld_agent(n.pulled)) {
-        // pulled step: the delay groups' partial sums (rewritten by every pull)
-        for (uint32_t g = 1; g < n.acc_slots; ++g) {
-          ae += ld_agent(&n.acc[(size_t)g * 2 * n.npad + id]);
-          ai += ld_agent(&n.acc[(size_t)g * 2 * n.npad + n.npad + id]);
+      if (ring_e) st_agent(&ring[id], 0ll);
+      if (ring_i) st_agent(&ring[n.npad + id], 0ll);
+      long long ae = ring_e, ai = ring_i;
+      if (due) {
+        // pulled step: the delay groups' sums (rewritten by every pull)
+        for (uint32_t g = 0; g < n.pull_groups; ++g) {
+          ae += ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
+          ai += ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
         }
       }
       if (n.drive_k) {
@@ -175,14 +282,14 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
         spike = spike || (rng_draw(n.keys[id], t, 3) < n.src_thresh[s]);
       }
     }
-    bits = spike ? dbits : 0;
   }
+  const uint64_t bits = spike ? dbits : 0;
   // dense spike output and history, one u64 word per wavefront
   // (storeDenseSpikes, IF_curr_exp_update_kernel.cl:58-94)
   const uint64_t mask = __ballot(spike);
   const uint32_t word = id >> 6;
   if (lane == 0 && word < n.W) {
-    st_agent(&n.hist[(t % n.R) * n.W + word], mask);
+    st_agent(&n.hist[(size_t)slot * n.W + word], mask);
     if (n.raster && t >= n.rec_first && t - n.rec_first < n.rec_steps)
       n.raster[(t - n.rec_first) * n.W + word] = mask;
     if (mask) atomicAdd(&sm.spikes, (unsigned long long)__popcll(mask));
@@ -193,152 +300,101 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
     for (int off = 32; off >= 1; off >>= 1) kd += __shfl_xor(kd, off);
     if (lane == 0 && kd) atomicAdd(&sm.drive, (unsigned long long)kd);
   }
-  // scatter, phase 1: per delay, the wave's rows get consecutive places
-  // inside the block's reservation
+  // rows per delay: the wave's spiking ids with a synapse of that delay
   uint64_t uni = bits;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
-  const uint64_t lt = lanemask_lt();
   for (uint64_t u = uni; u; u &= u - 1) {
     const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
     const uint64_t m = __ballot((bits >> k) & 1ull);
-    if (lane == 0) sm.woff[wave][k] = atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
+    if (lane == 0) atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
   }
   __syncthreads();
-  // phase 2: one global reservation per (block, delay); block counters
-  if (threadIdx.x < n.D && sm.cnt[threadIdx.x])
-    sm.base[threadIdx.x] = atomicAdd(&n.q_count[(t + threadIdx.x + 1) % n.R], sm.cnt[threadIdx.x]);
-  if (threadIdx.x == 64 || (blockDim.x <= 64 && threadIdx.x == 0)) {
-    if (sm.spikes) atomicAdd(counter(n.counters, kCntSpikes, blockIdx.x), sm.spikes);
-    if (sm.drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), sm.drive);
-  }
-  __syncthreads();
-  // phase 3: write the rows
-  for (uint64_t u = uni; u; u &= u - 1) {
-    const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
-    const bool mine = (bits >> k) & 1ull;
-    const uint64_t m = __ballot(mine);
-    if (mine) {
-      const uint64_t pos = (uint64_t)sm.base[k] + sm.woff[wave][k] + (uint64_t)__popcll(m & lt);
-      const uint32_t slot = (uint32_t)((t + k + 1) % n.R);
-      if (pos < n.q_cap)
-        st_agent(&n.q_rows[(uint64_t)slot * n.q_cap + pos], id * n.D + k);
-      else
-        atomicOr(counter(n.counters, kCntOverflow, blockIdx.x), 1ull);
-    }
-  }
-}
-
-template <int MODEL>
-__global__ void __launch_bounds__(1024) update_kernel(DevNet n, uint32_t j) {
-  __shared__ UpdateSmem sm;
-  const uint64_t t = *n.t_base + j;
-  update_block<MODEL>(n, t, blockIdx.x * blockDim.x + threadIdx.x, sm);
-}
-
-// Block-wide exclusive scan of one u32 per thread; returns the total.
-template <int BLK>
-__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, uint32_t* s_wave) {
-  const uint32_t lane = __lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
-  uint32_t inc = x;
+  if (!defer && sm.spikes) {
+    // push: the block's spiking ids' synapse ranges are scanned and walked
+    // contiguously; each synapse adds its weight into the slot of step
+    // t + k + 1 (scatter_kernel.cl:194-355 and gather_kernel.cl:67-200,
+    // folded into the spiking step)
+    sm.start[threadIdx.x] = r0;
+    const uint32_t total = block_exclusive_scan<BLK>(spike ? r1 - r0 : 0u, sm.pref, sm.wave);
+    const uint32_t first_slot = slot + 1 == n.R ? 0u : slot + 1;
+    constexpr uint32_t kBatch = 4;
+    for (uint32_t e0 = 0; e0 < total; e0 += BLK * kBatch) {
+      uint2 sw[kBatch];
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(inc, off);
-    if (lane >= (uint32_t)off) inc += y;
-  }
-  if (lane == 63) s_wave[wave] = inc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (int w = 0; w < BLK / 64; ++w) {
-      const uint32_t v = s_wave[w];
-      s_wave[w] = acc;
-      acc += v;
-    }
-    s_wave[BLK / 64] = acc;
-  }
-  __syncthreads();
-  s_pref[threadIdx.x] = s_wave[wave] + inc - x;
-  if (threadIdx.x == 0) s_pref[BLK] = s_wave[BLK / 64];
-  __syncthreads();
-  return s_pref[BLK];
-}
-
-template <int BLK>
-struct GatherSmem {
-  uint32_t start[BLK];
-  uint32_t pref[BLK + 1];
-  uint32_t wave[BLK / 64 + 1];
-};
-
-// gather(t) for blocks `block` of `nblocks`: chunks of up to BLK due rows;
-// their (start, length) are staged in LDS and scanned, then the block walks
-// the chunk's synapses contiguously so each row segment is read with
-// coalesced 8-byte loads.  The chunk shrinks to ceil(rows / nblocks) when
-// there are few due rows, so a sparse step is spread over all blocks instead
-// of queueing on the first ones.
-template <int BLK>
-__device__ inline void gather_body(const DevNet& n, uint64_t t, uint32_t block, uint32_t nblocks,
-                                   GatherSmem<BLK>& sm) {
-  const uint32_t slot = (uint32_t)(t % n.R);
-  // after an overflow the count runs past the slot's capacity (entries beyond
-  // it were never written): read only what is there
-  const uint32_t nrows = (uint32_t)min((uint64_t)ld_agent(&n.q_count[slot]), n.q_cap);
-  const uint32_t* rows = n.q_rows + (uint64_t)slot * n.q_cap;
-  unsigned long long delivered = 0, rows_done = 0;
-  const uint32_t per = min((uint32_t)BLK, max(1u, (nrows + nblocks - 1) / nblocks));
-  for (uint32_t chunk = block * per; chunk < nrows; chunk += nblocks * per) {
-    const uint32_t r = chunk + threadIdx.x;
-    uint32_t start = 0, len = 0;
-    if (threadIdx.x < per && r < nrows) {
-      const uint32_t row = ld_agent(&rows[r]);
-      start = n.row_off[row];
-      len = n.row_off[row + 1] - start;
-    }
-    sm.start[threadIdx.x] = start;
-    const uint32_t total = block_exclusive_scan<BLK>(len, sm.pref, sm.wave);
-    for (uint32_t e = threadIdx.x; e < total; e += BLK) {
-      uint32_t lo = 0, hi = BLK;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (sm.pref[mid] <= e) lo = mid; else hi = mid;
+      for (uint32_t u = 0; u < kBatch; ++u) {
+        const uint32_t e = e0 + u * BLK + threadIdx.x;
+        sw[u] = make_uint2(0u, 0u);
+        if (e < total) {
+          uint32_t lo = 0, hi = BLK;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sm.pref[mid] <= e) lo = mid; else hi = mid;
+          }
+          sw[u] = n.syn[sm.start[lo] + (e - sm.pref[lo])];
+        }
       }
-      const uint2 sw = n.syn[sm.start[lo] + (e - sm.pref[lo])];
-      const int32_t w = (int32_t)sw.y;
-      long long* dst = n.acc + (w < 0 ? n.npad : 0u) + sw.x;
-      atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)w);
+#pragma unroll
+      for (uint32_t u = 0; u < kBatch; ++u) {
+        if (e0 + u * BLK + threadIdx.x < total) {
+          const uint32_t k = sw[u].x >> kSynDelayShift;
+          const uint32_t tgt = sw[u].x & kSynTargetMask;
+          const int32_t w = (int32_t)sw[u].y;
+          uint32_t ds = first_slot + k;
+          if (ds >= n.R) ds -= n.R;
+          long long* dst = n.acc + (size_t)ds * 2 * n.npad + (w < 0 ? n.npad : 0u) + tgt;
+          atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)w);
+          atomicAdd(&sm.ev[k], 1u);
+        }
+      }
     }
-    delivered += total;
-    rows_done += min(per, nrows - chunk);
     __syncthreads();
   }
-  if (threadIdx.x == 0 && rows_done) {
-    atomicAdd(counter(n.counters, kCntEvents, blockIdx.x), delivered);
-    atomicAdd(counter(n.counters, kCntRows, blockIdx.x), rows_done);
+  // counters: one atomic per (block, delay) into the delivery ring
+  if (threadIdx.x < n.D) {
+    const uint32_t k = threadIdx.x;
+    uint32_t ds = slot + 1 + k;
+    if (ds >= n.R) ds -= n.R;
+    if (sm.cnt[k]) atomicAdd(ring_counter(n.ring_cnt, ds, 1, blockIdx.x), (unsigned long long)sm.cnt[k]);
+    if (sm.ev[k]) atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)sm.ev[k]);
   }
+  if (threadIdx.x == 0) {
+    if (sm.spikes) {
+      atomicAdd(counter(n.counters, kCntSpikes, blockIdx.x), sm.spikes);
+      atomicAdd(spk_counter(n.spk_ring, t, blockIdx.x), sm.spikes);
+      if (defer) st_agent(&n.dstamp[slot], t + 1);
+    }
+    if (sm.drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), sm.drive);
+  }
+}
+
+template <int MODEL, int BLK>
+__global__ void __launch_bounds__(BLK) update_kernel(DevNet n, uint32_t j) {
+  __shared__ UpdateSmem<BLK> sm;
+  const uint64_t t = *n.t_base + j;
+  update_block<MODEL, BLK>(n, t, blockIdx.x * BLK + threadIdx.x, sm);
 }
 
 // Dense gather (pull) for target block `b` and delay group `g`: for each
-// delay k of the group the spike bitmap of step t-k-1 (the block's source
+// due delay k of the group the spike bitmap of step t-k-1 (the block's source
 // range) is staged in LDS; each lane walks kPullRun consecutive synapses of
 // the (b, k) segment (16-byte loads), keeps a running sum while the target
 // stays the same, and adds it into the block's LDS accumulators when the
 // target changes.  The block stores its targets' sums for the group into
-// accumulator slot g: plain stores, no global atomics.
+// pull slot g: plain stores, no global atomics.
 // dyn: pull_words u64 bitmap + 2*kPullTB i64.
 template <int BLK>
-__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t* dyn) {
+__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
   uint64_t* bits = dyn;
   long long* acc = reinterpret_cast<long long*>(dyn + n.pull_words);
   for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) acc[i] = 0;
   const uint32_t lo = n.pull_lo[b];
   const uint32_t nw = min(n.pull_words, n.W - lo);
-  const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.acc_slots);
-  const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.acc_slots);
+  const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
+  const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
   unsigned long long events = 0;
   for (uint32_t k = k0; k < k1; ++k) {
-    if (t < (uint64_t)k + 1) break;
+    if (!((due >> k) & 1ull)) continue;
     const uint32_t s0 = n.pull_off[b * n.D + k], s1 = n.pull_off[b * n.D + k + 1];
     if (s0 == s1) continue;
     const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
@@ -412,7 +468,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     }
   }
   __syncthreads();
-  long long* out = n.acc + (size_t)g * 2 * n.npad;
+  long long* out = n.acc_pull + (size_t)g * 2 * n.npad;
   for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
     const uint32_t tgt = b * kPullTB + i;
     if (tgt < n.N) {
@@ -427,27 +483,19 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   if (__lane_id() == 0 && ev) atomicAdd(counter(n.counters, kCntEvents, blockIdx.x * 16 + (threadIdx.x >> 6)), ev);
 }
 
+// gather(t): the pull of the deferred steps whose synapses are due at t, one
+// (target block, delay group) work item per block iteration.  Exits at once
+// when nothing is due (every block reads the same stamps).
 template <int BLK>
 __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
-  __shared__ GatherSmem<BLK> sm;
   extern __shared__ uint64_t dyn[];
   const uint64_t t = *n.t_base + j;
-  if (n.pull_off) {
-    const uint32_t nrows = (uint32_t)min((uint64_t)ld_agent(&n.q_count[t % n.R]), n.q_cap);
-    const bool pull = nrows >= n.dense_rows;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st_agent(n.pulled, pull ? 1u : 0u);
-      if (pull && nrows) atomicAdd(counter(n.counters, kCntRows, 0), (unsigned long long)nrows);
-    }
-    if (pull) {
-      for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.acc_slots; bg += gridDim.x) {
-        pull_body<BLK>(n, t, bg / n.acc_slots, bg % n.acc_slots, dyn);
-        __syncthreads();  // LDS reuse by the next work item
-      }
-      return;
-    }
+  const uint64_t due = due_mask(n, t);
+  if (!due) return;
+  for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
+    pull_body<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
+    __syncthreads();  // LDS reuse by the next work item
   }
-  gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
 }
 
 // Grid barrier for a co-resident (cooperative) grid.  Arrival: block b adds
@@ -496,12 +544,13 @@ __device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long l
   return sh[1] == 0;
 }
 
-// Persistent step loop: one cooperative launch runs `nsteps` steps; grid
-// barriers separate gather(t) from update(t) and update(t) from gather(t+1).
+// Persistent step loop: one cooperative launch runs `nsteps` steps.  A grid
+// barrier ends every update(t); on steps that pull, another one separates
+// the pull from update(t).  Every block reads the same stamps, so all take
+// the same barriers.
 template <int MODEL, int BLK>
 __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nsteps, uint64_t* t_base) {
-  __shared__ GatherSmem<BLK> sm;
-  __shared__ UpdateSmem usm;
+  __shared__ UpdateSmem<BLK> usm;
   __shared__ uint32_t bar_sh[2];
   if (threadIdx.x == 0) {
     bar_sh[0] = ld_agent(n.barrier + (kBarLeaves + 1) * kBarStride);
@@ -512,21 +561,16 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   extern __shared__ uint64_t dyn[];
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
-    const uint32_t nrows = (uint32_t)min((uint64_t)ld_agent(&n.q_count[t % n.R]), n.q_cap);
-    const bool pull = n.pull_off && nrows >= n.dense_rows;
-    if (pull) {
-      for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.acc_slots; bg += gridDim.x) {
-        pull_body<BLK>(n, t, bg / n.acc_slots, bg % n.acc_slots, dyn);
+    const uint64_t due = due_mask(n, t);
+    if (due) {
+      for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
+        pull_body<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
         __syncthreads();
       }
-      if (blockIdx.x == 0 && threadIdx.x == 0 && nrows) atomicAdd(counter(n.counters, kCntRows, 0), (unsigned long long)nrows);
-    } else {
-      gather_body<BLK>(n, t, blockIdx.x, gridDim.x, sm);
+      if (!grid_barrier(n.barrier, bar_sh, err)) return;
     }
-    if (n.pull_off && blockIdx.x == 0 && threadIdx.x == 0) st_agent(n.pulled, pull ? 1u : 0u);
-    if (!grid_barrier(n.barrier, bar_sh, err)) return;
     for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
-      update_block<MODEL>(n, t, base + threadIdx.x, usm);
+      update_block<MODEL, BLK>(n, t, base + threadIdx.x, usm);
     if (!grid_barrier(n.barrier, bar_sh, err)) return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
@@ -543,14 +587,22 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t value, uint64_t count) {
 }  // namespace
 
 hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s) {
-  if (block == 0 || block % 64 || block > 1024) return hipErrorInvalidValue;
-  const uint32_t grid = (n.T + block - 1) / block;
+#define SZSNN_UPD(M)                                                                          \
+  switch (block) {                                                                            \
+    case 64: update_kernel<M, 64><<<(n.T + 63) / 64, 64, 0, s>>>(n, j); break;                 \
+    case 128: update_kernel<M, 128><<<(n.T + 127) / 128, 128, 0, s>>>(n, j); break;            \
+    case 256: update_kernel<M, 256><<<(n.T + 255) / 256, 256, 0, s>>>(n, j); break;            \
+    case 512: update_kernel<M, 512><<<(n.T + 511) / 512, 512, 0, s>>>(n, j); break;            \
+    case 1024: update_kernel<M, 1024><<<(n.T + 1023) / 1024, 1024, 0, s>>>(n, j); break;       \
+    default: return hipErrorInvalidValue;                                                     \
+  }
   switch (n.model) {
-    case kIfCurrExp: update_kernel<kIfCurrExp><<<grid, block, 0, s>>>(n, j); break;
-    case kIzhikevich: update_kernel<kIzhikevich><<<grid, block, 0, s>>>(n, j); break;
-    case kIfCondExp: update_kernel<kIfCondExp><<<grid, block, 0, s>>>(n, j); break;
+    case kIfCurrExp: SZSNN_UPD(kIfCurrExp) break;
+    case kIzhikevich: SZSNN_UPD(kIzhikevich) break;
+    case kIfCondExp: SZSNN_UPD(kIfCondExp) break;
     default: return hipErrorInvalidValue;
   }
+#undef SZSNN_UPD
   return hipGetLastError();
 }
 
@@ -559,6 +611,7 @@ size_t pull_dyn_bytes(const DevNet& n) {
 }
 
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {
+  if (!n.pull_off) return hipSuccess;  // no pull layout: every step pushes
   if (grid == 0) return hipErrorInvalidValue;
   const size_t dyn = pull_dyn_bytes(n);
   switch (block) {

@@ -5,7 +5,9 @@
 // SynapseData.ipp:41-111):
 //   * delay-bucketed CSR: rows keyed by (id, API delay k), row = id*D + k,
 //     synapses in input order within a row, each (target, int32 fixed-point
-//     weight); zero weights are inert and dropped;
+//     weight); zero weights are inert and dropped.  The device push format
+//     is the same array with per-id ranges (an id's rows are contiguous) and
+//     the delay k packed above the target (target | k << kSynDelayShift);
 //   * the delay bitmap per id (bit k set iff row (id, k) is nonempty, the
 //     reference's bitmap, SynapseData.ipp:70-83);
 //   * the pull layout for dense gathers: targets in blocks of kPullTB, one
@@ -29,16 +31,9 @@
 #include <vector>
 
 #include "../../include/szsnn_hip.h"
+#include "szsnn_constants.h"
 
 namespace szsnn {
-
-#ifndef SZSNN_PULL_CONSTANTS
-#define SZSNN_PULL_CONSTANTS
-constexpr uint32_t kPullShift = 9;
-constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block
-constexpr uint32_t kPullMaxWords = 6144;         // 48 KiB of staged bitmap (sources spanning 393k ids)
-constexpr uint32_t kPullRun = 8;                 // consecutive synapses per lane
-#endif
 
 struct LayoutPair {
   uint32_t x, y;
@@ -46,7 +41,9 @@ struct LayoutPair {
 
 struct HostLayout {
   std::vector<uint32_t> row_off;      // [T*D + 1]
-  std::vector<LayoutPair> syn;        // [S]
+  std::vector<LayoutPair> syn;        // [S] push entries (target | k << kSynDelayShift, weight)
+  std::vector<uint32_t> syn_off;      // [T + 1] per-id ranges of syn
+  uint64_t active = 0;                // ids with at least one synapse
   std::vector<uint64_t> delay_bits;   // [T]
   uint64_t nonempty = 0;              // nonempty rows
   std::vector<uint32_t> pull_off;     // [NB*D + 1] or empty
@@ -57,7 +54,8 @@ struct HostLayout {
   std::string error;
 };
 
-inline bool build_layout(const szsnn_net_desc& d, HostLayout& L) {
+// Rows, delay bits and the pull layout (targets still unpacked in L.syn).
+inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
   const uint64_t N = d.n_neurons, M = d.n_sources, T = N + M, D = d.max_delay;
   const uint64_t nrows = T * D;
   const double scale = std::ldexp(1.0, d.frac_bits);
@@ -186,6 +184,28 @@ inline bool build_layout(const szsnn_net_desc& d, HostLayout& L) {
   L.pull_lo.swap(lo);
   L.pull_syn.swap(ps);
   L.pull_words = words;
+  return true;
+}
+
+// All device formats.  After the rows and the pull layout, the push format:
+// per-id synapse ranges (rows are id-major, so syn_off[id] = row_off[id * D])
+// and the API delay packed into each entry's target word.
+inline bool build_layout(const szsnn_net_desc& d, HostLayout& L) {
+  if (!build_rows_and_pull(d, L)) return false;
+  const uint64_t T = (uint64_t)d.n_neurons + d.n_sources, D = d.max_delay;
+  if (d.n_neurons > kMaxNeurons) {
+    L.error = "more than 2^26 neurons";
+    return false;
+  }
+  L.syn_off.assign(T + 1, 0);
+  L.active = 0;
+  for (uint64_t id = 0; id <= T; ++id) L.syn_off[id] = L.row_off[id * D];
+  for (uint64_t id = 0; id < T; ++id) {
+    L.active += L.syn_off[id + 1] != L.syn_off[id];
+    for (uint64_t k = 0; k < D; ++k)
+      for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q)
+        L.syn[q].x |= (uint32_t)k << kSynDelayShift;
+  }
   return true;
 }
 

@@ -66,8 +66,8 @@ struct szsnn_net {
   bool graph_dirty = true;
   // autotune snapshot buffers
   std::vector<std::pair<void*, size_t>> snap;
-  uint64_t nonempty_rows = 0;
-  float dense_fraction = 0.3f;
+  uint64_t active_ids = 0;        // ids with at least one synapse
+  float dense_fraction = 0.04f;
   // batched networks: per-instance id ranges (empty for a plain network)
   std::vector<uint32_t> inst_noff, inst_soff, inst_n, inst_m;
 };
@@ -131,9 +131,18 @@ void free_graph(szsnn_net* net) {
 }
 
 uint32_t default_gather_grid(const szsnn_net* net, uint32_t block) {
-  const uint64_t chunks = (net->dn.q_cap + block - 1) / block;
+  (void)block;
+  const uint64_t items = (uint64_t)net->dn.pull_nb * net->dn.pull_groups;
   const uint64_t cap = (uint64_t)std::max(net->ctx->cu_count, 1) * 8;
-  return (uint32_t)std::max<uint64_t>(1, std::min(chunks, cap));
+  return (uint32_t)std::max<uint64_t>(1, std::min(items, cap));
+}
+
+// Deferral threshold of the gather policy: a step is deferred to the pull
+// when its predicted spikes reach dense_fraction x the ids with synapses.
+uint32_t defer_threshold(const szsnn_net* net, float dense_fraction) {
+  if (!net->dn.pull_off || dense_fraction > 1.0f) return 0xffffffffu;
+  if (dense_fraction <= 0.0f) return 0;
+  return (uint32_t)std::max<double>(1.0, std::ceil((double)dense_fraction * (double)net->active_ids));
 }
 
 int launch_step(szsnn_net* net, const DevNet& dn, uint32_t j) {
@@ -226,9 +235,6 @@ int check_overflow(szsnn_net* net) {
     (void)hipMemset(net->dn.barrier, 0, kBarWords * 4);
     return fail(net->ctx, SZSNN_ERR_STATE, "persistent loop: a grid barrier wait timed out (grid not co-resident)");
   }
-  if (c[kCntOverflow])
-    return fail(net->ctx, SZSNN_ERR_OVERFLOW,
-                "due-queue overflow: spikes were dropped (raise queue_capacity or use 0 for worst case)");
   return SZSNN_OK;
 }
 
@@ -358,14 +364,12 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   const double scale = std::ldexp(1.0, d->frac_bits);
   const uint64_t nrows = T * D;
 
-  // --- device formats: delay-bucketed CSR and the pull layout (szsnn_layout.h)
+  // --- device formats: the push entries with per-id ranges and the pull layout (szsnn_layout.h)
   HostLayout L;
   if (!build_layout(*d, L)) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: " + L.error);
   const uint64_t S = L.row_off[nrows];
-  const uint64_t nonempty = L.nonempty;
-  uint64_t cap = std::max<uint64_t>(nonempty, 1);
-  if (d->queue_capacity) cap = std::min<uint64_t>(cap, d->queue_capacity);
-  const std::vector<uint32_t>& row_off = L.row_off;
+  const std::vector<uint32_t>& syn_off = L.syn_off;
+  const uint64_t active = L.active;
   const std::vector<uint64_t>& delay_bits = L.delay_bits;
   const std::vector<uint32_t>& pull_off = L.pull_off;
   const std::vector<uint32_t>& pull_lo = L.pull_lo;
@@ -401,7 +405,6 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.frac_bits = d->frac_bits;
   n.acc_scale = (float)std::ldexp(1.0, -d->frac_bits);
   n.drive_w = (long long)drive_q;
-  n.q_cap = cap;
 
   int rc = SZSNN_OK;
   auto guard = [&](int r) {
@@ -409,26 +412,25 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     return rc == SZSNN_OK;
   };
   float *params = nullptr, *vars = nullptr, *noise = nullptr, *src_rate = nullptr, *src_last = nullptr;
-  long long* acc = nullptr;
-  uint32_t* pulled = nullptr;
+  long long *acc = nullptr, *acc_pull = nullptr;
   uint32_t* barrier = nullptr;
   // delay groups of the pull gather: enough (block, group) work items to fill the chip
-  uint32_t acc_slots = 1;
+  uint32_t groups = 1;
   if (!pull_off.empty()) {
     const uint64_t want = ((uint64_t)4 * std::max(ctx->cu_count, 1) + NB - 1) / std::max<uint64_t>(NB, 1);
-    acc_slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, (uint64_t)D, (uint64_t)8}));
+    groups = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, (uint64_t)D, (uint64_t)8}));
   }
-  uint64_t *keys = nullptr, *src_thresh = nullptr, *drive_cdf = nullptr, *dbits = nullptr, *raster = nullptr;
-  uint32_t *roff = nullptr, *qc = nullptr, *qr = nullptr;
+  uint64_t *keys = nullptr, *src_thresh = nullptr, *drive_cdf = nullptr, *dbits = nullptr, *dstamp = nullptr;
+  uint32_t* soff = nullptr;
   uint2* dsyn = nullptr;
-  unsigned long long* counters = nullptr;
+  unsigned long long *counters = nullptr, *ring_cnt = nullptr, *spk_ring = nullptr;
   guard(dev_alloc(net, &params, (size_t)np * n.npad)) && guard(dev_alloc(net, &vars, (size_t)nv * n.npad)) &&
-      guard(dev_alloc(net, &acc, (size_t)2 * n.npad * acc_slots)) && guard(dev_alloc(net, &pulled, 1)) && guard(dev_alloc(net, &barrier, kBarWords)) && guard(dev_alloc(net, &keys, T)) &&
-      guard(dev_alloc(net, &roff, nrows + 1)) && guard(dev_alloc(net, &dsyn, S)) &&
-      guard(dev_alloc(net, &dbits, T)) && guard(dev_alloc(net, &qc, n.R)) &&
-      guard(dev_alloc(net, &qr, (size_t)n.R * cap)) && guard(dev_alloc(net, &counters, kCntWords)) &&
-      guard(dev_alloc(net, &net->t_base, 1)) && guard(dev_alloc(net, &net->forced_bits, n.W)) &&
-      guard(dev_alloc(net, &net->step_bitmap, n.W));
+      guard(dev_alloc(net, &acc, (size_t)n.R * 2 * n.npad)) && guard(dev_alloc(net, &barrier, kBarWords)) &&
+      guard(dev_alloc(net, &keys, T)) && guard(dev_alloc(net, &soff, T + 1)) && guard(dev_alloc(net, &dsyn, S)) &&
+      guard(dev_alloc(net, &dbits, T)) && guard(dev_alloc(net, &dstamp, n.R)) &&
+      guard(dev_alloc(net, &ring_cnt, kRingCntWords(n.R))) && guard(dev_alloc(net, &spk_ring, kSpkRingWords)) &&
+      guard(dev_alloc(net, &counters, kCntWords)) && guard(dev_alloc(net, &net->t_base, 1)) &&
+      guard(dev_alloc(net, &net->forced_bits, n.W)) && guard(dev_alloc(net, &net->step_bitmap, n.W));
   if (rc == SZSNN_OK && d->noise_amp) guard(dev_alloc(net, &noise, N));
   if (rc == SZSNN_OK && M && n.source_model == kSrcConstRate)
     guard(dev_alloc(net, &src_rate, M)) && guard(dev_alloc(net, &src_last, M));
@@ -440,12 +442,12 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   if (rc == SZSNN_OK) guard(dev_alloc(net, &hist, (size_t)n.R * n.W));
   if (rc == SZSNN_OK && !pull_off.empty())
     guard(dev_alloc(net, &d_pull_off, pull_off.size())) && guard(dev_alloc(net, &d_pull_syn, pull_syn_size)) &&
-        guard(dev_alloc(net, &d_pull_lo, pull_lo.size()));
+        guard(dev_alloc(net, &d_pull_lo, pull_lo.size())) &&
+        guard(dev_alloc(net, &acc_pull, (size_t)groups * 2 * n.npad));
   if (rc != SZSNN_OK) {
     szsnn_net_destroy(net);
     return rc;
   }
-  (void)raster;
   // host staging with the padded SoA stride
   std::vector<float> hp((size_t)np * n.npad, 0.0f), hv((size_t)nv * n.npad, 0.0f);
   for (int k = 0; k < np; ++k) std::memcpy(&hp[(size_t)k * n.npad], d->params + (size_t)k * N, N * 4);
@@ -471,7 +473,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     if (d->src_last) std::memcpy(hsl.data(), d->src_last, M * 4);
   }
   guard(upload(net, params, hp.data(), hp.size())) && guard(upload(net, vars, hv.data(), hv.size())) &&
-      guard(upload(net, keys, hk.data(), T)) && guard(upload(net, roff, row_off.data(), nrows + 1)) &&
+      guard(upload(net, keys, hk.data(), T)) && guard(upload(net, soff, syn_off.data(), T + 1)) &&
       guard(upload(net, dsyn, syn_data, S)) && guard(upload(net, dbits, delay_bits.data(), T));
   if (rc == SZSNN_OK && noise) guard(upload(net, noise, d->noise_amp, N));
   if (rc == SZSNN_OK && src_rate) guard(upload(net, src_rate, d->src_rate, M)) && guard(upload(net, src_last, hsl.data(), M));
@@ -483,10 +485,12 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
         guard(upload(net, d_pull_lo, pull_lo.data(), pull_lo.size()));
   if (rc == SZSNN_OK) {
     hipStream_t s = ctx->stream;
-    he = hipMemsetAsync(acc, 0, (size_t)2 * n.npad * 8 * acc_slots, s);
-    if (he == hipSuccess) he = hipMemsetAsync(pulled, 0, 4, s);
+    he = hipMemsetAsync(acc, 0, (size_t)n.R * 2 * n.npad * 8, s);
+    if (he == hipSuccess && acc_pull) he = hipMemsetAsync(acc_pull, 0, (size_t)groups * 2 * n.npad * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(barrier, 0, kBarWords * 4, s);
-    if (he == hipSuccess) he = hipMemsetAsync(qc, 0, n.R * 4, s);
+    if (he == hipSuccess) he = hipMemsetAsync(dstamp, 0, (size_t)n.R * 8, s);
+    if (he == hipSuccess) he = hipMemsetAsync(ring_cnt, 0, kRingCntWords(n.R) * 8, s);
+    if (he == hipSuccess) he = hipMemsetAsync(spk_ring, 0, kSpkRingWords * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(counters, 0, kCntWords * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->t_base, 0, 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->forced_bits, 0, (size_t)n.W * 8, s);
@@ -501,8 +505,8 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.params = params;
   n.vars = vars;
   n.acc = acc;
-  n.acc_slots = acc_slots;
-  n.pulled = pulled;
+  n.acc_pull = acc_pull;
+  n.pull_groups = groups;
   n.barrier = barrier;
   n.keys = keys;
   n.noise_amp = noise;
@@ -510,11 +514,13 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.src_last = src_last;
   n.src_thresh = src_thresh;
   n.drive_cdf = drive_cdf;
-  n.row_off = roff;
+  n.syn_off = soff;
   n.syn = dsyn;
   n.delay_bits = dbits;
-  n.q_count = qc;
-  n.q_rows = qr;
+  n.dstamp = dstamp;
+  n.ring_cnt = ring_cnt;
+  n.spk_ring = spk_ring;
+  n.forced_count = 0;
   n.raster = nullptr;
   n.rec_first = 0;
   n.rec_steps = 0;
@@ -528,9 +534,8 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.pull_lo = d_pull_lo;
   n.pull_nb = d_pull_off ? (uint32_t)NB : 0;
   n.pull_words = pull_words;
-  net->nonempty_rows = nonempty;
-  n.dense_rows = (uint32_t)std::min<double>(0xffffffffu, std::ceil(net->dense_fraction * (double)nonempty));
-  if (n.dense_rows == 0) n.dense_rows = 1;
+  net->active_ids = active;
+  n.defer_spikes = defer_threshold(net, net->dense_fraction);
 
   net->info.n_neurons = (uint32_t)N;
   net->info.n_sources = (uint32_t)M;
@@ -540,7 +545,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   net->info.frac_bits = d->frac_bits;
   net->info.n_syn_stored = S;
   net->info.n_rows = nrows;
-  net->info.queue_capacity = cap;
+  net->info.pull_groups = d_pull_off ? groups : 0;
   net->info.raster_words = n.W;
   net->tiling.gather_grid = default_gather_grid(net, net->tiling.gather_block);
   *out = net;
@@ -639,11 +644,6 @@ int szsnn_batch_upload(szsnn_ctx* ctx, const szsnn_net_desc* inst, uint32_t n_in
   c.max_delay = D;
   c.noise_amp = noise ? amp.data() : nullptr;
   c.queue_capacity = 0;
-  for (uint32_t b = 0; b < n_inst; ++b) c.queue_capacity += inst[b].queue_capacity;
-  if (c.queue_capacity) {
-    for (uint32_t b = 0; b < n_inst; ++b)
-      if (!inst[b].queue_capacity) c.queue_capacity = 0;  // any worst-case instance: worst case overall
-  }
   int rc = upload_impl(ctx, &c, keys.data(), out);
   if (rc) return rc;
   (*out)->inst_noff = std::move(noff);
@@ -693,6 +693,7 @@ int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, ui
     }
     HIP_TRY(ctx, hipMemcpyAsync(net->forced_bits, bits.data(), dn.W * 8, hipMemcpyHostToDevice, ctx->stream));
     dn.forced_bits = net->forced_bits;
+    dn.forced_count = n_forced;
   }
   const uint64_t t = net->t_host;
   uint64_t* row = nullptr;
@@ -848,8 +849,9 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
 
 int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_block, uint32_t gather_grid) {
   if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_tiling: net is NULL");
-  if (update_block && (update_block % 64 || update_block > 1024))
-    return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling: update_block must be a multiple of 64, <= 1024");
+  if (update_block && update_block != 64 && update_block != 128 && update_block != 256 && update_block != 512 &&
+      update_block != 1024)
+    return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling: update_block must be 64, 128, 256, 512 or 1024");
   if (gather_block && gather_block != 128 && gather_block != 256 && gather_block != 512 && gather_block != 1024)
     return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling: gather_block must be 128, 256, 512 or 1024");
   if (update_block) net->tiling.update_block = update_block;
@@ -862,13 +864,7 @@ int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_bloc
 int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction) {
   if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_gather_policy: net is NULL");
   net->dense_fraction = dense_fraction;
-  if (dense_fraction <= 0.0f)
-    net->dn.dense_rows = 0;
-  else if (dense_fraction > 1.0f)
-    net->dn.dense_rows = 0xffffffffu;
-  else
-    net->dn.dense_rows =
-        (uint32_t)std::max<double>(1.0, std::ceil((double)dense_fraction * (double)net->nonempty_rows));
+  net->dn.defer_spikes = defer_threshold(net, dense_fraction);
   net->graph_dirty = true;
   return SZSNN_OK;
 }
@@ -929,15 +925,16 @@ std::vector<Region> state_regions(szsnn_net* net) {
   const DevNet& n = net->dn;
   std::vector<Region> r = {
       {n.vars, (size_t)net->info.n_vars * n.npad * 4},
-      {n.acc, (size_t)2 * n.npad * 8 * n.acc_slots},
-      {n.pulled, 4},
-      {n.q_count, (size_t)n.R * 4},
-      {n.q_rows, (size_t)n.R * n.q_cap * 4},
+      {n.acc, (size_t)n.R * 2 * n.npad * 8},
+      {n.dstamp, (size_t)n.R * 8},
+      {n.ring_cnt, kRingCntWords(n.R) * 8},
+      {n.spk_ring, kSpkRingWords * 8},
       {n.counters, kCntWords * 8},
       {net->t_base, 8},
       {n.hist, (size_t)n.R * n.W * 8},
   };
   if (n.src_last) r.push_back({n.src_last, (size_t)n.M * 4});
+  if (n.acc_pull) r.push_back({n.acc_pull, (size_t)n.pull_groups * 2 * n.npad * 8});
   return r;
 }
 

@@ -192,7 +192,9 @@ class Simulation:
         return {"update_block": a.value, "gather_block": b.value, "gather_grid": c.value}
 
     def set_gather_policy(self, dense_fraction):
-        """Pull when due rows >= dense_fraction x nonempty rows (<= 0: always, > 1: never)."""
+        """Defer a step's spikes to the pull gather when the spikes predicted
+        for it (the previous step's plus the forced ones) reach
+        dense_fraction x the ids with synapses (<= 0: always, > 1: never)."""
         check(self.lib.szsnn_set_gather_policy(self.handle, float(dense_fraction)), self.ctx.handle)
 
     def gather_policy(self):

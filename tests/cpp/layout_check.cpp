@@ -1,11 +1,13 @@
-// CPU check of the device formats and of the gather kernels' index arithmetic.
+// CPU check of the device formats and of the step kernels' index arithmetic.
 //
 // Builds layouts with the runtime's builder (szsnn_amd/csrc/szsnn_layout.h)
 // for random and edge-case networks, then replays, in plain C++, the index
-// computations of gather_body (push) and pull_body (pull) from
+// computations of the push in update_block and of pull_body from
 // szsnn_amd/csrc/szsnn_kernels.hip with an explicit bounds check on every
-// array access, and compares both sums with a naive sum over the synapse list.
-// Exit code 0 and "layout_check OK" on success; any failure aborts with a message.
+// array access: spike histories in which some steps push and the others are
+// deferred to the pull must deliver, at step t, the naive sum over the
+// synapse list.  Exit code 0 and "layout_check OK" on success; any failure
+// aborts with a message.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -64,57 +66,65 @@ static szsnn_net_desc desc_of(const Net& n) {
   return d;
 }
 
-// push replay: rows due = list of row ids; chunks of `per` <= BLK rows as
-// gather_body
-static void push_replay(const HostLayout& L, uint32_t N, uint64_t S, const std::vector<uint32_t>& rows,
-                        uint64_t q_cap, uint32_t count_in_slot, std::vector<long long>& acc, uint32_t BLK,
-                        uint32_t nblocks) {
-  const uint32_t nrows = (uint32_t)std::min<uint64_t>(count_in_slot, q_cap);
-  const uint32_t per = std::min(BLK, std::max(1u, (nrows + nblocks - 1) / nblocks));
-  std::vector<uint32_t> sstart(BLK), spref(BLK + 1);
-  for (uint32_t block = 0; block < nblocks; ++block)
-    for (uint32_t chunk = block * per; chunk < nrows; chunk += nblocks * per) {
-      for (uint32_t tid = 0; tid < BLK; ++tid) {
-        const uint32_t r = chunk + tid;
-        uint32_t start = 0, len = 0;
-        if (tid < per && r < nrows) {
-          CHECK(r < rows.size() && r < q_cap, "queue read r=%u", r);
-          const uint32_t row = rows[r];
-          CHECK((uint64_t)row + 1 < L.row_off.size(), "row_off read row=%u", row);
-          start = L.row_off[row];
-          len = L.row_off[row + 1] - start;
-        }
-        sstart[tid] = start;
-        spref[tid] = len;
+// push replay of update_block at step s: blocks of BLK ids; the spiking ids'
+// synapse ranges are scanned and walked in batches of kBatch per lane; each
+// entry adds into ring slot (s + k + 1) % R.  ring = [R][2][N], ev = [R].
+static void push_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t s,
+                        const std::vector<uint8_t>& fired_s, uint32_t BLK, std::vector<long long>& ring,
+                        std::vector<uint64_t>& ev) {
+  const uint32_t R = D + 1;
+  const uint64_t S = L.syn.size();
+  const uint32_t slot = (uint32_t)(s % R);
+  const uint32_t first_slot = slot + 1 == R ? 0u : slot + 1;
+  constexpr uint32_t kBatch = 4;
+  std::vector<uint32_t> start(BLK), pref(BLK + 1);
+  for (uint32_t b = 0; b < (T + BLK - 1) / BLK; ++b) {
+    uint32_t run = 0;
+    for (uint32_t tid = 0; tid < BLK; ++tid) {
+      const uint32_t id = b * BLK + tid;
+      uint32_t r0 = 0, r1 = 0;
+      if (id < T) {
+        CHECK((size_t)id + 1 < L.syn_off.size(), "syn_off read id=%u", id);
+        r0 = L.syn_off[id];
+        r1 = L.syn_off[id + 1];
       }
-      uint32_t acc_s = 0;
-      for (uint32_t tid = 0; tid < BLK; ++tid) {
-        const uint32_t v = spref[tid];
-        spref[tid] = acc_s;
-        acc_s += v;
-      }
-      spref[BLK] = acc_s;
-      const uint32_t total = acc_s;
-      for (uint32_t e = 0; e < total; ++e) {
-        uint32_t lo = 0, hi = BLK;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (spref[mid] <= e) lo = mid; else hi = mid;
-        }
-        const uint64_t q = (uint64_t)sstart[lo] + (e - spref[lo]);
-        CHECK(q < S, "syn read q=%llu S=%llu", (unsigned long long)q, (unsigned long long)S);
-        const LayoutPair p = L.syn[q];
-        const int32_t w = (int32_t)p.y;
-        CHECK(p.x < N, "target %u", p.x);
-        acc[(w < 0 ? (size_t)N : 0) + p.x] += w;
-      }
+      start[tid] = r0;
+      pref[tid] = run;
+      run += (id < T && fired_s[id]) ? r1 - r0 : 0u;
     }
+    pref[BLK] = run;
+    const uint32_t total = run;
+    for (uint32_t e0 = 0; e0 < total; e0 += BLK * kBatch)
+      for (uint32_t u = 0; u < kBatch; ++u)
+        for (uint32_t tid = 0; tid < BLK; ++tid) {
+          const uint32_t e = e0 + u * BLK + tid;
+          if (e >= total) continue;
+          uint32_t lo = 0, hi = BLK;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pref[mid] <= e) lo = mid; else hi = mid;
+          }
+          const uint64_t q = (uint64_t)start[lo] + (e - pref[lo]);
+          CHECK(q < S, "syn read q=%llu S=%llu", (unsigned long long)q, (unsigned long long)S);
+          const LayoutPair p = L.syn[q];
+          const uint32_t k = p.x >> kSynDelayShift, tgt = p.x & kSynTargetMask;
+          CHECK(k < D, "packed delay %u", k);
+          CHECK(tgt < N, "target %u", tgt);
+          uint32_t ds = first_slot + k;
+          if (ds >= R) ds -= R;
+          CHECK(ds < R && ds != slot, "ring slot %u (own %u)", ds, slot);
+          const int32_t w = (int32_t)p.y;
+          CHECK(w != 0, "stored zero weight");
+          ring[((size_t)ds * 2 + (w < 0 ? 1 : 0)) * N + tgt] += w;
+          ++ev[ds];
+        }
+  }
 }
 
-// pull replay for step t with history bitmaps hist[R][W]
-// pull replay for step t with history bitmaps hist[R][W]; G delay groups, each
-// writing its slot of `out` ([G][2][N]); the caller sums the slots
-static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t t,
+// pull replay for step t with history bitmaps hist[R][W] and due mask `due`
+// (bit k: step t-k-1 was deferred); G delay groups, each writing its slot of
+// `out` ([G][2][N]); the caller sums the slots
+static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t t, uint64_t due,
                         const std::vector<uint64_t>& hist, uint32_t R, uint32_t W, uint32_t G,
                         std::vector<long long>& out, uint64_t& events) {
   const uint32_t words = L.pull_words;
@@ -130,7 +140,8 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
     const uint32_t k0 = (uint32_t)(((uint64_t)g * D) / G), k1 = (uint32_t)(((uint64_t)(g + 1) * D) / G);
     CHECK(k1 <= D, "group end");
     for (uint32_t k = k0; k < k1; ++k) {
-      if (t < (uint64_t)k + 1) break;
+      if (!((due >> k) & 1ull)) continue;
+      CHECK(t >= (uint64_t)k + 1, "due delay before the first step");
       CHECK((uint64_t)b * D + k + 1 < L.pull_off.size(), "pull_off");
       const uint32_t s0 = L.pull_off[b * D + k], s1 = L.pull_off[b * D + k + 1];
       CHECK(s0 <= s1 && s1 <= L.pull_syn.size(), "segment");
@@ -190,28 +201,34 @@ static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D,
   const uint64_t Ss = L.row_off.back();
   CHECK(L.row_off.size() == (size_t)T * D + 1, "row_off size");
   CHECK(L.syn.size() == Ss, "syn size");
-  // delay bits agree with rows
-  for (uint32_t id = 0; id < T; ++id)
-    for (uint32_t k = 0; k < D; ++k)
+  CHECK(L.syn_off.size() == (size_t)T + 1 && L.syn_off[T] == Ss, "syn_off");
+  // delay bits agree with rows; packed delays agree with the rows they sit in
+  uint64_t active = 0;
+  for (uint32_t id = 0; id < T; ++id) {
+    active += L.syn_off[id + 1] != L.syn_off[id];
+    for (uint32_t k = 0; k < D; ++k) {
       CHECK((((L.delay_bits[id] >> k) & 1) != 0) == (L.row_off[id * D + k + 1] > L.row_off[id * D + k]), "bits");
+      for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q)
+        CHECK((L.syn[q].x >> kSynDelayShift) == k, "packed delay of entry %u", q);
+    }
+  }
+  CHECK(active == L.active, "active ids");
   const bool pull = !L.pull_off.empty();
   CHECK(pull, "pull layout expected for N=%u", N);
   CHECK(L.pull_off.size() == L.NB * D + 1, "pull_off size");
-  // random spike history; step t
+  // random spike history up to step t - 1
   const uint64_t t = 3 * D + 2;
   std::vector<uint64_t> hist((size_t)R * W);
   std::bernoulli_distribution sp(0.3);
   std::vector<std::vector<uint8_t>> fired(t, std::vector<uint8_t>(T));
   for (uint64_t s = 0; s < t; ++s) {
     for (uint32_t id = 0; id < T; ++id) fired[s][id] = sp(rng);
-    if (s + R >= t + 1 || true) {
-      uint64_t* row = &hist[(s % R) * W];
-      std::fill(row, row + W, 0);
-      for (uint32_t id = 0; id < T; ++id)
-        if (fired[s][id]) row[id >> 6] |= 1ull << (id & 63);
-    }
+    uint64_t* row = &hist[(s % R) * W];
+    std::fill(row, row + W, 0);
+    for (uint32_t id = 0; id < T; ++id)
+      if (fired[s][id]) row[id >> 6] |= 1ull << (id & 63);
   }
-  // naive
+  // naive input delivered at step t
   std::vector<long long> naive(2 * (size_t)N, 0);
   uint64_t nev = 0;
   for (size_t s = 0; s < net.src.size(); ++s) {
@@ -223,37 +240,34 @@ static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D,
     naive[(wq < 0 ? (size_t)N : 0) + net.tgt[s]] += wq;
     ++nev;
   }
-  // due rows of step t, in queue order of the update kernel: for each spiking id
-  // of step t-k-1 with a nonempty row k
-  std::vector<uint32_t> rows;
-  for (uint32_t k = 0; k < D; ++k) {
-    if (t < (uint64_t)k + 1) continue;
-    for (uint32_t id = 0; id < T; ++id)
-      if (fired[t - k - 1][id] && ((L.delay_bits[id] >> k) & 1)) rows.push_back(id * D + k);
-  }
-  CHECK(rows.size() <= L.nonempty, "due rows exceed the worst-case capacity");
-  for (uint32_t BLK : {128u, 256u, 1024u})
-    for (uint32_t nb : {1u, 7u, 300u}) {
-      std::vector<long long> push(2 * (size_t)N, 0);
-      push_replay(L, N, Ss, rows, std::max<uint64_t>(L.nonempty, 1), (uint32_t)rows.size(), push, BLK, nb);
-      CHECK(push == naive, "push sum differs (BLK=%u, blocks=%u)", BLK, nb);
+  // deferral patterns over the steps t-D .. t-1: none, all, random
+  for (int pattern = 0; pattern < 3; ++pattern) {
+    std::vector<uint8_t> deferred(t, 0);
+    std::bernoulli_distribution coin(0.4);
+    for (uint64_t s = 0; s < t; ++s) deferred[s] = pattern == 1 ? 1 : pattern == 2 ? coin(rng) : 0;
+    uint64_t due = 0;
+    for (uint32_t k = 0; k < D; ++k)
+      if (t >= (uint64_t)k + 1 && deferred[t - k - 1]) due |= 1ull << k;
+    for (uint32_t BLK : {64u, 256u, 1024u}) {
+      std::vector<long long> ring((size_t)R * 2 * N, 0);
+      std::vector<uint64_t> ev(R, 0);
+      for (uint64_t s = (t > D ? t - D : 0); s < t; ++s)
+        if (!deferred[s]) push_replay(L, N, T, D, s, fired[s], BLK, ring, ev);
+      const size_t slot = t % R;
+      for (uint32_t G : {1u, 2u, 3u, 8u}) {
+        if (G > D) continue;
+        std::vector<long long> slots((size_t)G * 2 * N, -1);
+        uint64_t pev = 0;
+        if (due) pull_replay(L, N, T, D, t, due, hist, R, W, G, slots, pev);
+        std::vector<long long> got(ring.begin() + slot * 2 * N, ring.begin() + (slot + 1) * 2 * N);
+        if (due)
+          for (uint32_t g = 0; g < G; ++g)
+            for (size_t i = 0; i < 2 * (size_t)N; ++i) got[i] += slots[(size_t)g * 2 * N + i];
+        CHECK(got == naive, "delivered sum differs (pattern %d, BLK=%u, G=%u)", pattern, BLK, G);
+        CHECK(ev[slot] + pev == nev, "events %llu + %llu vs %llu", (unsigned long long)ev[slot],
+              (unsigned long long)pev, (unsigned long long)nev);
+      }
     }
-  // overflowed slot: count beyond capacity is clamped, reads stay in bounds
-  if (rows.size() > 4) {
-    std::vector<uint32_t> part(rows.begin(), rows.begin() + 4);
-    std::vector<long long> push(2 * (size_t)N, 0);
-    push_replay(L, N, Ss, part, 4, (uint32_t)rows.size() + 1000, push, 256, 3);
-  }
-  for (uint32_t G : {1u, 2u, 3u, 8u}) {
-    if (G > D) continue;
-    std::vector<long long> slots((size_t)G * 2 * N, -1);
-    uint64_t pev = 0;
-    pull_replay(L, N, T, D, t, hist, R, W, G, slots, pev);
-    std::vector<long long> pulled(2 * (size_t)N, 0);
-    for (uint32_t g = 0; g < G; ++g)
-      for (size_t i = 0; i < 2 * (size_t)N; ++i) pulled[i] += slots[(size_t)g * 2 * N + i];
-    CHECK(pulled == naive, "pull sum differs (G=%u)", G);
-    CHECK(pev == nev, "pull events %llu vs %llu", (unsigned long long)pev, (unsigned long long)nev);
   }
 }
 

@@ -199,13 +199,27 @@ def test_errors_are_reported(ctx):
     bad.syn_delay[0] = bad.max_delay
     with pytest.raises(SzsnnError, match="max_delay"):
         Simulation(ctx, bad)
-    # a queue too small for the spikes: reported, not silent (reference D4)
-    tiny = izhikevich2003(n_exc=80, n_inh=20, fanout=10)
-    tiny.queue_capacity = 1
-    sim = Simulation(ctx, tiny)
-    sim.set_volley(100)
-    with pytest.raises(SzsnnError, match="overflow"):
+    with pytest.raises(SzsnnError, match="update_block"):
+        sim.set_tiling(update_block=192)
+    # the reference's queues could overflow (reported, reference D4); delivery
+    # through the ring of accumulator slots has no queue: a tiny
+    # queue_capacity is ignored and a full volley still delivers every event
+    runs = []
+    for cap in (0, 1):
+        tiny = izhikevich2003(n_exc=80, n_inh=20, fanout=10)
+        tiny.queue_capacity = cap
+        sim = Simulation(ctx, tiny)
+        sim.set_gather_policy(2.0)   # push every step
+        sim.set_volley(100)
         sim.run(3)
+        runs.append(sim.counters())
+    # every id fires at steps 0, 1, 2: a synapse of delay k is delivered at the
+    # steps c + k + 1 <= 2 that are inside the run, i.e. max(0, 2 - k) times
+    w_q = np.round(tiny.syn_w.astype(np.float64) * 2.0 ** tiny.frac_bits)
+    k = tiny.syn_delay.astype(np.int64)[w_q != 0]
+    expected = int(np.maximum(0, 2 - k).sum())
+    assert runs[0]["events"] == runs[1]["events"] == expected
+    assert runs[1]["overflow"] == 0
 
 
 def test_autotune_restores_state(ctx):
@@ -293,6 +307,19 @@ def _run_policy(ctx, net, steps, fraction, volley=0):
     return s.raster(), s.get_vars(), s.counters()
 
 
+def _mixed_fraction(net, raster, volley):
+    """A policy fraction under which about half of the steps are deferred:
+    step t is deferred when spikes(t-1) + volley >= ceil(fraction x ids with
+    synapses).  Returns (fraction, deferred steps)."""
+    from szsnn_amd.simulation import raster_to_bool
+    counts = raster_to_bool(raster, net.n_neurons + net.n_sources).sum(axis=1).astype(np.int64)
+    pred = np.concatenate([[0], counts[:-1]]) + volley
+    active = np.unique(net.syn_src[net.syn_w != 0]).size
+    thr = int(np.median(pred[1:])) + 1
+    frac = (thr - 0.5) / active
+    return frac, int((pred >= thr).sum())
+
+
 @pytest.mark.parametrize("which", ["izh", "curr", "cond_volley"])
 def test_pull_gather_equals_push_gather(ctx, which):
     """Dense (pull) and sparse (push) gathers give identical sums, so rasters,
@@ -309,7 +336,9 @@ def test_pull_gather_equals_push_gather(ctx, which):
     print(f"case: pull vs push {which}", flush=True)
     rp, vp, cp = _run_policy(ctx, net, steps, 2.0, volley)   # push only
     rq, vq, cq = _run_policy(ctx, net, steps, 0.0, volley)   # pull only
-    ra, va, ca = _run_policy(ctx, net, steps, 0.3, volley)   # per-step choice
+    frac, n_def = _mixed_fraction(net, rp, volley)
+    assert 0 < n_def < steps, (frac, n_def)
+    ra, va, ca = _run_policy(ctx, net, steps, frac, volley)  # some steps deferred
     for r, v, c in ((rq, vq, cq), (ra, va, ca)):
         np.testing.assert_array_equal(r, rp)
         np.testing.assert_array_equal(v, vp)
