@@ -38,28 +38,58 @@ __device__ inline int32_t gauss_int(uint64_t key, uint64_t t) {
 // Unit-variance scale of the centred Irwin-Hall sum: 0.5/sqrt(8*(2^32-1)/12).
 constexpr float kGaussScale = 0x1.3988e2p-17f;
 
+// Per-model counts of SoA parameter and state rows.
+template <int MODEL>
+struct ModelRows {
+  static constexpr int kParams = MODEL == kIfCurrExp ? 9 : MODEL == kIzhikevich ? 4 : 11;
+  static constexpr int kVars = MODEL == kIzhikevich ? 2 : 4;
+};
+
+// One neuron's parameters and state, loaded into registers at the top of the
+// update so every load of the step is in flight before the first barrier.
+template <int MODEL>
+struct NeuronRegs {
+  float p[ModelRows<MODEL>::kParams];
+  float x[ModelRows<MODEL>::kVars];
+};
+
+template <int MODEL>
+__device__ inline void load_neuron(const DevNet& n, uint32_t i, NeuronRegs<MODEL>& r) {
+#pragma unroll
+  for (int k = 0; k < ModelRows<MODEL>::kParams; ++k) r.p[k] = n.params[(size_t)k * n.npad + i];
+#pragma unroll
+  for (int k = 0; k < ModelRows<MODEL>::kVars; ++k) r.x[k] = n.vars[(size_t)k * n.npad + i];
+}
+
+template <int MODEL>
+__device__ inline void store_neuron(const DevNet& n, uint32_t i, const NeuronRegs<MODEL>& r) {
+#pragma unroll
+  for (int k = 0; k < ModelRows<MODEL>::kVars; ++k) n.vars[(size_t)k * n.npad + i] = r.x[k];
+}
+
 // IF_curr_exp (IF_curr_exp_update_kernel.cl:230-297) and IF_cond_exp (same
 // scheme, conductances in the current slots).  Returns spike.
 // The device parameter rows 1, 2, 4, 5 hold 1/cm, 1/tau_m, 1 - 1/tau_syn_E and
 // 1 - 1/tau_syn_I, formed once at upload with the same IEEE FP32 operations the
 // step would repeat (the reference evaluates them every step).
 template <int MODEL>
-__device__ inline bool if_update(const float* __restrict__ p, float* __restrict__ x, uint32_t i,
-                                 uint32_t stride, float accE, float accI, float iinj, bool forced) {
-  const float v_rest = p[0 * stride + i], rcm = p[1 * stride + i], rtau_m = p[2 * stride + i];
-  const float tau_refrac = p[3 * stride + i], decay_e = p[4 * stride + i], decay_i = p[5 * stride + i];
-  const float i_offset = p[6 * stride + i], v_reset = p[7 * stride + i], v_thresh = p[8 * stride + i];
-  float v = x[0 * stride + i];
-  const float se = (decay_e * x[1 * stride + i]) + accE;
-  const float si = (decay_i * x[2 * stride + i]) + accI;
+__device__ inline bool if_update(NeuronRegs<MODEL>& r, float accE, float accI, float iinj, bool forced) {
+  const float* p = r.p;
+  float* x = r.x;
+  const float v_rest = p[0], rcm = p[1], rtau_m = p[2];
+  const float tau_refrac = p[3], decay_e = p[4], decay_i = p[5];
+  const float i_offset = p[6], v_reset = p[7], v_thresh = p[8];
+  float v = x[0];
+  const float se = (decay_e * x[1]) + accE;
+  const float si = (decay_i * x[2]) + accI;
   float I;
-  if (MODEL == kIfCurrExp) {
+  if constexpr (MODEL == kIfCurrExp) {
     I = ((se + si) + iinj) + i_offset;
   } else {
-    const float e_rev_e = p[9 * stride + i], e_rev_i = p[10 * stride + i];
+    const float e_rev_e = p[9], e_rev_i = p[10];
     I = (((se * (e_rev_e - v)) - (si * (e_rev_i - v))) + iinj) + i_offset;
   }
-  float fired = x[3 * stride + i];
+  float fired = x[3];
   const bool refr = fired <= tau_refrac;
   fired = fired + 1.0f;
   if (!refr) {
@@ -73,19 +103,18 @@ __device__ inline bool if_update(const float* __restrict__ p, float* __restrict_
     v = v_reset;
     fired = 1.0f;
   }
-  x[0 * stride + i] = v;
-  x[1 * stride + i] = se;
-  x[2 * stride + i] = si;
-  x[3 * stride + i] = fired;
+  x[0] = v;
+  x[1] = se;
+  x[2] = si;
+  x[3] = fired;
   return spike;
 }
 
 // Izhikevich (Izhikevich_update_kernel.cl:219,247-251,264-268): one Euler step
 // of both equations from the old (v, u).
-__device__ inline bool izh_update(const float* __restrict__ p, float* __restrict__ x, uint32_t i,
-                                  uint32_t stride, float accE, float accI, float iinj, bool forced) {
-  const float a = p[0 * stride + i], b = p[1 * stride + i];
-  float v = x[0 * stride + i], u = x[1 * stride + i];
+__device__ inline bool izh_update(NeuronRegs<kIzhikevich>& r, float accE, float accI, float iinj, bool forced) {
+  const float a = r.p[0], b = r.p[1];
+  float v = r.x[0], u = r.x[1];
   const float I = (accE + accI) + iinj;
   const float dv = ((((0.04f * v) + 5.0f) * v + 140.0f) - u) + I;
   const float du = a * ((b * v) - u);
@@ -93,11 +122,11 @@ __device__ inline bool izh_update(const float* __restrict__ p, float* __restrict
   u = u + du;
   const bool spike = (v >= 30.0f) || forced;
   if (spike) {
-    v = p[2 * stride + i];
-    u = u + p[3 * stride + i];
+    v = r.p[2];
+    u = u + r.p[3];
   }
-  x[0 * stride + i] = v;
-  x[1 * stride + i] = u;
+  r.x[0] = v;
+  r.x[1] = u;
   return spike;
 }
 
@@ -143,17 +172,6 @@ __device__ inline uint64_t due_mask(const DevNet& n, uint64_t t) {
   return __ballot(due);
 }
 
-// Spikes expected at step t: those of step t - 1 plus the forced ids.  Every
-// lane of the calling wave gets the sum.
-__device__ inline unsigned long long predicted_spikes(const DevNet& n, uint64_t t) {
-  unsigned long long c = 0;
-  const uint32_t lane = __lane_id();
-  if (t >= 1 && lane < kRingStripes) c = ld_agent(spk_counter(n.spk_ring, t - 1, lane));
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
-  return c + n.volley + n.forced_count;
-}
-
 // LDS of one update block.
 template <int BLK>
 struct UpdateSmem {
@@ -167,31 +185,43 @@ struct UpdateSmem {
   uint32_t defer;            // step t's spikes go to the pull gather
 };
 
-// Block 0's bookkeeping at step t (one wave): move the events and rows due at
-// t from the delivery ring into the counters, and clear the spike-count slot
-// that step t + 1 will fill.
-__device__ inline void fold_rings(const DevNet& n, uint64_t t, uint32_t slot) {
+// Block-uniform facts of step t, computed by one wave with all its loads
+// issued together: the pull's due mask (as due_mask) and whether step t is
+// deferred (the spikes of step t - 1 plus the forced ids reach the
+// threshold).  With `fold` (block 0), the wave also moves the events and rows
+// due at t from the delivery ring into the counters and clears the
+// spike-count slot that step t + 1 will fill.
+__device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bool fold, uint64_t& due_out,
+                                  bool& defer_out) {
   const uint32_t lane = __lane_id();
-  unsigned long long e = 0, r = 0;
-  if (lane < kRingStripes) {
-    unsigned long long* p = ring_counter(n.ring_cnt, slot, 0, lane);
-    e = ld_agent(p);
-    if (e) st_agent(p, 0ull);
-  } else if (lane < 2 * kRingStripes) {
-    unsigned long long* p = ring_counter(n.ring_cnt, slot, 1, lane - kRingStripes);
-    r = ld_agent(p);
-    if (r) st_agent(p, 0ull);
-  } else if (lane < 3 * kRingStripes) {
-    st_agent(spk_counter(n.spk_ring, t + 1, lane - 2 * kRingStripes), 0ull);
-  }
+  // unconditional loads at always-valid addresses (no divergent branch between
+  // them, so they are in flight together); validity is applied afterwards
+  const uint32_t k = lane < n.D ? lane : 0u;
+  const uint64_t stamp = n.pull_off ? ld_agent(&n.dstamp[(t - k - 1) % n.R]) : 0ull;
+  unsigned long long spk = ld_agent(spk_counter(n.spk_ring, t - 1, lane));
+  unsigned long long* fp = ring_counter(n.ring_cnt, slot, (lane / kRingStripes) & 1u, lane % kRingStripes);
+  unsigned long long fc = fold ? ld_agent(fp) : 0ull;
+  const bool stamp_ok = n.pull_off && lane < n.D && t >= (uint64_t)lane + 1 && stamp == t - lane;
+  if (t == 0 || lane >= kRingStripes) spk = 0;
+  if (lane >= 2 * kRingStripes) fc = 0;
+  due_out = __ballot(stamp_ok);
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    e += __shfl_xor(e, off);
-    r += __shfl_xor(r, off);
-  }
-  if (lane == 0) {
-    if (e) atomicAdd(counter(n.counters, kCntEvents, 0), e);
-    if (r) atomicAdd(counter(n.counters, kCntRows, 0), r);
+  for (int off = 32; off >= 1; off >>= 1) spk += __shfl_xor(spk, off);
+  defer_out = n.pull_off && spk + n.volley + n.forced_count >= n.defer_spikes;
+  if (fold) {
+    if (fc) st_agent(fp, 0ull);
+    if (lane >= 2 * kRingStripes && lane < 3 * kRingStripes)
+      st_agent(spk_counter(n.spk_ring, t + 1, lane - 2 * kRingStripes), 0ull);
+    unsigned long long e = lane < kRingStripes ? fc : 0ull, r = lane < kRingStripes ? 0ull : fc;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      e += __shfl_xor(e, off);
+      r += __shfl_xor(r, off);
+    }
+    if (lane == 0) {
+      if (e) atomicAdd(counter(n.counters, kCntEvents, 0), e);
+      if (r) atomicAdd(counter(n.counters, kCntRows, 0), r);
+    }
   }
 }
 
@@ -207,30 +237,45 @@ template <int MODEL, int BLK>
 __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
   const uint32_t lane = __lane_id();
   const uint32_t slot = (uint32_t)(t % n.R);
+  // every load the lane needs, issued before the first barrier
   long long ring_e = 0, ring_i = 0;
   uint32_t r0 = 0, r1 = 0;
-  uint64_t dbits = 0;
+  uint64_t dbits = 0, key = 0, fbits = 0, src_thresh = 0;
+  float amp = 0.0f, src_last = 0.0f, src_rate = 0.0f;
+  NeuronRegs<MODEL> nr;
   long long* ring = n.acc + (size_t)slot * 2 * n.npad;
   if (id < n.N) {
     ring_e = ld_agent(&ring[id]);
     ring_i = ld_agent(&ring[n.npad + id]);
+    load_neuron<MODEL>(n, id, nr);
+    if (n.noise_amp) amp = n.noise_amp[id];
+  } else if (id < n.T) {
+    const uint32_t s = id - n.N;
+    if (n.source_model == kSrcConstRate) {
+      src_last = n.src_last[s];
+      src_rate = n.src_rate[s];
+    } else if (n.source_model == kSrcPoisson) {
+      src_thresh = n.src_thresh[s];
+    }
   }
   if (id < n.T) {
     r0 = n.syn_off[id];
     r1 = n.syn_off[id + 1];
     dbits = n.delay_bits[id];
+    key = n.keys[id];
+    if (n.forced_bits) fbits = n.forced_bits[id >> 6];
   }
   if (threadIdx.x < 64) {
     sm.cnt[threadIdx.x] = 0;
     sm.ev[threadIdx.x] = 0;
-    const uint64_t due = due_mask(n, t);
-    const bool defer = n.pull_off && predicted_spikes(n, t) >= n.defer_spikes;
+    uint64_t due;
+    bool defer;
+    step_facts(n, t, slot, blockIdx.x == 0, due, defer);
     if (threadIdx.x == 0) {
       sm.due = due;
       sm.defer = defer ? 1u : 0u;
       sm.spikes = sm.drive = 0;
     }
-    if (blockIdx.x == 0) fold_rings(n, t, slot);
   }
   __syncthreads();
   const uint64_t due = sm.due;
@@ -238,8 +283,7 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
   bool spike = false;
   uint32_t kdrive = 0;
   if (id < n.T) {
-    bool forced = id < n.volley;
-    if (n.forced_bits) forced = forced || ((n.forced_bits[id >> 6] >> (id & 63)) & 1ull);
+    const bool forced = id < n.volley || ((fbits >> (id & 63)) & 1ull);
     if (id < n.N) {
       if (ring_e) st_agent(&ring[id], 0ll);
       if (ring_i) st_agent(&ring[n.npad + id], 0ll);
@@ -255,7 +299,7 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
         // events = #{j : u >= cdf[j]}; the thresholds ascend, so this equals the
         // inverse-CDF index.  Counting over all j keeps the loads uniform
         // across the wave (scalar loads) instead of a per-lane dependent walk.
-        const uint64_t u = rng_draw(n.keys[id], t, 2);
+        const uint64_t u = rng_draw(key, t, 2);
         uint32_t k = 0;
 #pragma unroll 8
         for (uint32_t j = 0; j < n.drive_k; ++j) k += (u >= n.drive_cdf[j]) ? 1u : 0u;
@@ -264,22 +308,23 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
       }
       const float accE = (float)ae * n.acc_scale;
       const float accI = (float)ai * n.acc_scale;
-      const float iinj = n.noise_amp ? n.noise_amp[id] * ((float)gauss_int(n.keys[id], t) * kGaussScale) : 0.0f;
-      if (MODEL == kIzhikevich)
-        spike = izh_update(n.params, n.vars, id, n.npad, accE, accI, iinj, forced);
+      const float iinj = n.noise_amp ? amp * ((float)gauss_int(key, t) * kGaussScale) : 0.0f;
+      if constexpr (MODEL == kIzhikevich)
+        spike = izh_update(nr, accE, accI, iinj, forced);
       else
-        spike = if_update<MODEL>(n.params, n.vars, id, n.npad, accE, accI, iinj, forced);
+        spike = if_update<MODEL>(nr, accE, accI, iinj, forced);
+      store_neuron<MODEL>(n, id, nr);
     } else {
       const uint32_t s = id - n.N;
       spike = forced;
       if (n.source_model == kSrcConstRate) {
         // Const_rate_update_kernel.cl:213-227
-        float last = n.src_last[s] + 1.0f;
-        spike = spike || (last == n.src_rate[s]);
+        float last = src_last + 1.0f;
+        spike = spike || (last == src_rate);
         if (spike) last = 0.0f;
         n.src_last[s] = last;
       } else if (n.source_model == kSrcPoisson) {
-        spike = spike || (rng_draw(n.keys[id], t, 3) < n.src_thresh[s]);
+        spike = spike || (rng_draw(key, t, 3) < src_thresh);
       }
     }
   }
