@@ -12,6 +12,7 @@ constexpr uint32_t kPullShift = 9;
 constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block
 constexpr uint32_t kPullMaxWords = 6144;         // 48 KiB of staged bitmap (sources spanning 393k ids)
 constexpr uint32_t kPullRun = 8;                 // consecutive synapses per lane
+constexpr uint32_t kMaxPullGroups = 8;           // delay groups of the pull (work items per target block)
 
 // Push entries carry the API delay in the target word's top bits.
 constexpr uint32_t kSynDelayShift = 26;

@@ -56,7 +56,7 @@ struct DevNet {
   float* vars;              // [nv][npad]
   long long* acc;           // [R][2][npad] fixed-point E/I input due at the slot's step
   long long* acc_pull;      // [G][2][npad] or null: a pulling step's delay-group sums
-  uint32_t pull_groups;     // G: delay groups of the pull gather (>= 1)
+  uint32_t pull_groups;     // G: delay groups of the pull gather (1 .. kMaxPullGroups)
   uint32_t* barrier;        // grid barrier words of the persistent loop (kBarWords)
   const uint64_t* keys;     // [T] random-stream keys
   const float* noise_amp;   // [N] or null

@@ -175,54 +175,101 @@ __device__ inline uint64_t due_mask(const DevNet& n, uint64_t t) {
 // LDS of one update block.
 template <int BLK>
 struct UpdateSmem {
+  uint64_t cdf[32];          // the binomial drive's CDF thresholds
   uint32_t cnt[64];          // the block's rows per delay (spiking ids with that delay)
   uint32_t ev[64];           // the block's pushed synapses per delay
   uint32_t start[BLK];       // push: first synapse of each lane's id
   uint32_t pref[BLK + 1];    // push: exclusive scan of the lanes' synapse counts
   uint32_t wave[BLK / 64 + 1];
   unsigned long long spikes, drive;
+  unsigned long long spk_prev;  // spikes of step t - 1
   uint64_t due;              // due_mask(t)
-  uint32_t defer;            // step t's spikes go to the pull gather
 };
 
-// Block-uniform facts of step t, computed by one wave with all its loads
-// issued together: the pull's due mask (as due_mask) and whether step t is
-// deferred (the spikes of step t - 1 plus the forced ids reach the
-// threshold).  With `fold` (block 0), the wave also moves the events and rows
-// due at t from the delivery ring into the counters and clears the
-// spike-count slot that step t + 1 will fill.
-__device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bool fold, uint64_t& due_out,
-                                  bool& defer_out) {
+// Block-uniform facts of step t, gathered by one wave (the block's first)
+// with all its loads issued together: the pull's due mask (as due_mask)
+// into sm.due, and the spikes of step t - 1 into sm.spk_prev (LDS atomics of
+// the stripes, no wave reduction).  With `fold` (block 0), each of the
+// wave's first 2 * kRingStripes lanes also reads one stripe of the events /
+// rows due at t into `fold_val`; fold_finish moves them into the counters at
+// the end of the step.
+template <int BLK>
+__device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bool fold, UpdateSmem<BLK>& sm,
+                                  unsigned long long& fold_val) {
   const uint32_t lane = __lane_id();
+  if (lane == 0) sm.spk_prev = 0;
   // unconditional loads at always-valid addresses (no divergent branch between
   // them, so they are in flight together); validity is applied afterwards
   const uint32_t k = lane < n.D ? lane : 0u;
   const uint64_t stamp = n.pull_off ? ld_agent(&n.dstamp[(t - k - 1) % n.R]) : 0ull;
-  unsigned long long spk = ld_agent(spk_counter(n.spk_ring, t - 1, lane));
-  unsigned long long* fp = ring_counter(n.ring_cnt, slot, (lane / kRingStripes) & 1u, lane % kRingStripes);
-  unsigned long long fc = fold ? ld_agent(fp) : 0ull;
+  const unsigned long long spk = ld_agent(spk_counter(n.spk_ring, t - 1, lane));
+  const unsigned long long fc =
+      fold ? ld_agent(ring_counter(n.ring_cnt, slot, (lane / kRingStripes) & 1u, lane % kRingStripes)) : 0ull;
   const bool stamp_ok = n.pull_off && lane < n.D && t >= (uint64_t)lane + 1 && stamp == t - lane;
-  if (t == 0 || lane >= kRingStripes) spk = 0;
-  if (lane >= 2 * kRingStripes) fc = 0;
-  due_out = __ballot(stamp_ok);
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) spk += __shfl_xor(spk, off);
-  defer_out = n.pull_off && spk + n.volley + n.forced_count >= n.defer_spikes;
-  if (fold) {
-    if (fc) st_agent(fp, 0ull);
-    if (lane >= 2 * kRingStripes && lane < 3 * kRingStripes)
-      st_agent(spk_counter(n.spk_ring, t + 1, lane - 2 * kRingStripes), 0ull);
-    unsigned long long e = lane < kRingStripes ? fc : 0ull, r = lane < kRingStripes ? 0ull : fc;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      e += __shfl_xor(e, off);
-      r += __shfl_xor(r, off);
-    }
-    if (lane == 0) {
-      if (e) atomicAdd(counter(n.counters, kCntEvents, 0), e);
-      if (r) atomicAdd(counter(n.counters, kCntRows, 0), r);
+  fold_val = lane < 2 * kRingStripes ? fc : 0ull;
+  const uint64_t due = __ballot(stamp_ok);
+  if (lane == 0) sm.due = due;
+  if (t > 0 && lane < kRingStripes && spk) atomicAdd(&sm.spk_prev, spk);
+}
+
+// Block 0, one wave, end of step t: add the events and rows due at t (read
+// by step_facts) to the counters, stripe by stripe, clear their ring
+// stripes, and clear the spike-count slot that step t + 1 will fill.  Done
+// last so that these writes do not hold up the step's own loads.
+__device__ inline void fold_finish(const DevNet& n, uint64_t t, uint32_t slot, unsigned long long fold_val) {
+  const uint32_t lane = __lane_id();
+  if (fold_val) {
+    st_agent(ring_counter(n.ring_cnt, slot, lane / kRingStripes, lane % kRingStripes), 0ull);
+    atomicAdd(counter(n.counters, lane < kRingStripes ? kCntEvents : kCntRows, lane % kRingStripes), fold_val);
+  }
+  if (lane >= 2 * kRingStripes && lane < 3 * kRingStripes)
+    st_agent(spk_counter(n.spk_ring, t + 1, lane - 2 * kRingStripes), 0ull);
+}
+
+// What a lane keeps about its id across steps: the neuron's parameters and
+// state (or the source's), its random-stream key, synapse range and delay
+// bits.  Loaded from memory every step, or, in the persistent loop when each
+// lane owns one id, once per launch and carried in registers.
+template <int MODEL>
+struct LaneState {
+  NeuronRegs<MODEL> nr;
+  uint64_t key, dbits, src_thresh;
+  uint32_t r0, r1;
+  float amp, src_last, src_rate;
+};
+
+template <int MODEL>
+__device__ inline void load_lane(const DevNet& n, uint32_t id, LaneState<MODEL>& ls) {
+  ls.key = ls.dbits = ls.src_thresh = 0;
+  ls.r0 = ls.r1 = 0;
+  ls.amp = ls.src_last = ls.src_rate = 0.0f;
+  if (id < n.N) {
+    load_neuron<MODEL>(n, id, ls.nr);
+    if (n.noise_amp) ls.amp = n.noise_amp[id];
+  } else if (id < n.T) {
+    const uint32_t s = id - n.N;
+    if (n.source_model == kSrcConstRate) {
+      ls.src_last = n.src_last[s];
+      ls.src_rate = n.src_rate[s];
+    } else if (n.source_model == kSrcPoisson) {
+      ls.src_thresh = n.src_thresh[s];
     }
   }
+  if (id < n.T) {
+    ls.r0 = n.syn_off[id];
+    ls.r1 = n.syn_off[id + 1];
+    ls.dbits = n.delay_bits[id];
+    ls.key = n.keys[id];
+  }
+}
+
+// The state a step changed: neuron variables, a Const_rate source's counter.
+template <int MODEL>
+__device__ inline void store_lane(const DevNet& n, uint32_t id, const LaneState<MODEL>& ls) {
+  if (id < n.N)
+    store_neuron<MODEL>(n, id, ls.nr);
+  else if (id < n.T && n.source_model == kSrcConstRate)
+    n.src_last[id - n.N] = ls.src_last;
 }
 
 // update(t) for one id per lane; every thread of the block must call it (it
@@ -232,101 +279,99 @@ __device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bo
 // either pushes the block's spiking ids' synapses into the accumulator slots
 // of their delivery steps or, on a deferred step, stamps the step for the
 // pull gather.  Rows and events are added to the delivery-time counter ring
-// with one atomic per (block, delay).
-template <int MODEL, int BLK>
-__device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
+// with one atomic per (block, delay).  CARRY: `ls` already holds the lane's
+// state and keeps it (the caller stores it at the end); otherwise it is
+// loaded and stored here, and the drive table is staged in LDS here.
+template <int MODEL, int BLK, bool CARRY>
+__device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm,
+                                   LaneState<MODEL>& ls) {
   const uint32_t lane = __lane_id();
   const uint32_t slot = (uint32_t)(t % n.R);
   // every load the lane needs, issued before the first barrier
   long long ring_e = 0, ring_i = 0;
-  uint32_t r0 = 0, r1 = 0;
-  uint64_t dbits = 0, key = 0, fbits = 0, src_thresh = 0;
-  float amp = 0.0f, src_last = 0.0f, src_rate = 0.0f;
-  NeuronRegs<MODEL> nr;
+  uint64_t fbits = 0;
   long long* ring = n.acc + (size_t)slot * 2 * n.npad;
   if (id < n.N) {
     ring_e = ld_agent(&ring[id]);
     ring_i = ld_agent(&ring[n.npad + id]);
-    load_neuron<MODEL>(n, id, nr);
-    if (n.noise_amp) amp = n.noise_amp[id];
-  } else if (id < n.T) {
-    const uint32_t s = id - n.N;
-    if (n.source_model == kSrcConstRate) {
-      src_last = n.src_last[s];
-      src_rate = n.src_rate[s];
-    } else if (n.source_model == kSrcPoisson) {
-      src_thresh = n.src_thresh[s];
-    }
   }
-  if (id < n.T) {
-    r0 = n.syn_off[id];
-    r1 = n.syn_off[id + 1];
-    dbits = n.delay_bits[id];
-    key = n.keys[id];
-    if (n.forced_bits) fbits = n.forced_bits[id >> 6];
+  if (!CARRY) {
+    load_lane<MODEL>(n, id, ls);
+    if (threadIdx.x < n.drive_k) sm.cdf[threadIdx.x] = n.drive_cdf[threadIdx.x];
   }
+  if (id < n.T && n.forced_bits) fbits = n.forced_bits[id >> 6];
+  unsigned long long fold_val = 0;
   if (threadIdx.x < 64) {
     sm.cnt[threadIdx.x] = 0;
     sm.ev[threadIdx.x] = 0;
-    uint64_t due;
-    bool defer;
-    step_facts(n, t, slot, blockIdx.x == 0, due, defer);
-    if (threadIdx.x == 0) {
-      sm.due = due;
-      sm.defer = defer ? 1u : 0u;
-      sm.spikes = sm.drive = 0;
-    }
+    if (threadIdx.x == 0) sm.spikes = sm.drive = 0;
+    step_facts<BLK>(n, t, slot, blockIdx.x == 0, sm, fold_val);
   }
   __syncthreads();
   const uint64_t due = sm.due;
-  const bool defer = sm.defer != 0;
+  // step t is deferred when the spikes of step t - 1 plus the forced ids
+  // reach the threshold (the same decision in every block)
+  const bool defer = n.pull_off && sm.spk_prev + n.volley + n.forced_count >= n.defer_spikes;
+  const uint32_t r0 = ls.r0, r1 = ls.r1;
+  const uint64_t dbits = ls.dbits;
   bool spike = false;
   uint32_t kdrive = 0;
   if (id < n.T) {
     const bool forced = id < n.volley || ((fbits >> (id & 63)) & 1ull);
     if (id < n.N) {
-      if (ring_e) st_agent(&ring[id], 0ll);
-      if (ring_i) st_agent(&ring[n.npad + id], 0ll);
       long long ae = ring_e, ai = ring_i;
       if (due) {
-        // pulled step: the delay groups' sums (rewritten by every pull)
-        for (uint32_t g = 0; g < n.pull_groups; ++g) {
-          ae += ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
-          ai += ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
+        // pulled step: the delay groups' sums (rewritten by every pull), all
+        // loads issued before the sum
+        long long pe[kMaxPullGroups], pi[kMaxPullGroups];
+#pragma unroll
+        for (uint32_t g = 0; g < kMaxPullGroups; ++g) {
+          pe[g] = pi[g] = 0;
+          if (g < n.pull_groups) {
+            pe[g] = ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
+            pi[g] = ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
+          }
+        }
+#pragma unroll
+        for (uint32_t g = 0; g < kMaxPullGroups; ++g) {
+          ae += pe[g];
+          ai += pi[g];
         }
       }
       if (n.drive_k) {
-        // events = #{j : u >= cdf[j]}; the thresholds ascend, so this equals the
-        // inverse-CDF index.  Counting over all j keeps the loads uniform
-        // across the wave (scalar loads) instead of a per-lane dependent walk.
-        const uint64_t u = rng_draw(key, t, 2);
+        // events = #{j : u >= cdf[j]}, the inverse-CDF index.  The thresholds
+        // ascend, so the count stops at the first one no lane of the wave
+        // reaches (reads stay uniform LDS broadcasts).
+        const uint64_t u = rng_draw(ls.key, t, 2);
         uint32_t k = 0;
-#pragma unroll 8
-        for (uint32_t j = 0; j < n.drive_k; ++j) k += (u >= n.drive_cdf[j]) ? 1u : 0u;
+        for (uint32_t j = 0; j < n.drive_k; ++j) {
+          const bool ge = u >= sm.cdf[j];
+          if (!__any(ge)) break;
+          k += ge ? 1u : 0u;
+        }
         ae += (long long)k * n.drive_w;
         kdrive = k;
       }
       const float accE = (float)ae * n.acc_scale;
       const float accI = (float)ai * n.acc_scale;
-      const float iinj = n.noise_amp ? amp * ((float)gauss_int(key, t) * kGaussScale) : 0.0f;
+      const float iinj = n.noise_amp ? ls.amp * ((float)gauss_int(ls.key, t) * kGaussScale) : 0.0f;
       if constexpr (MODEL == kIzhikevich)
-        spike = izh_update(nr, accE, accI, iinj, forced);
+        spike = izh_update(ls.nr, accE, accI, iinj, forced);
       else
-        spike = if_update<MODEL>(nr, accE, accI, iinj, forced);
-      store_neuron<MODEL>(n, id, nr);
+        spike = if_update<MODEL>(ls.nr, accE, accI, iinj, forced);
     } else {
-      const uint32_t s = id - n.N;
       spike = forced;
       if (n.source_model == kSrcConstRate) {
         // Const_rate_update_kernel.cl:213-227
-        float last = src_last + 1.0f;
-        spike = spike || (last == src_rate);
+        float last = ls.src_last + 1.0f;
+        spike = spike || (last == ls.src_rate);
         if (spike) last = 0.0f;
-        n.src_last[s] = last;
+        ls.src_last = last;
       } else if (n.source_model == kSrcPoisson) {
-        spike = spike || (rng_draw(key, t, 3) < src_thresh);
+        spike = spike || (rng_draw(ls.key, t, 3) < ls.src_thresh);
       }
     }
+    if (!CARRY) store_lane<MODEL>(n, id, ls);
   }
   const uint64_t bits = spike ? dbits : 0;
   // dense spike output and history, one u64 word per wavefront
@@ -339,20 +384,17 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
       n.raster[(t - n.rec_first) * n.W + word] = mask;
     if (mask) atomicAdd(&sm.spikes, (unsigned long long)__popcll(mask));
   }
-  if (n.drive_k) {
-    uint32_t kd = kdrive;
+  if (kdrive) atomicAdd(&sm.drive, (unsigned long long)kdrive);
+  if (mask) {
+    // rows per delay: the wave's spiking ids with a synapse of that delay
+    uint64_t uni = bits;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) kd += __shfl_xor(kd, off);
-    if (lane == 0 && kd) atomicAdd(&sm.drive, (unsigned long long)kd);
-  }
-  // rows per delay: the wave's spiking ids with a synapse of that delay
-  uint64_t uni = bits;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
-  for (uint64_t u = uni; u; u &= u - 1) {
-    const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
-    const uint64_t m = __ballot((bits >> k) & 1ull);
-    if (lane == 0) atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
+    for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
+    for (uint64_t u = uni; u; u &= u - 1) {
+      const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
+      const uint64_t m = __ballot((bits >> k) & 1ull);
+      if (lane == 0) atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
+    }
   }
   __syncthreads();
   if (!defer && sm.spikes) {
@@ -411,6 +453,16 @@ __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, Up
     }
     if (sm.drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), sm.drive);
   }
+  // writes nothing later in the step waits for
+  if (ring_e) st_agent(&ring[id], 0ll);
+  if (ring_i) st_agent(&ring[n.npad + id], 0ll);
+  if (blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_val);
+}
+
+template <int MODEL, int BLK>
+__device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
+  LaneState<MODEL> ls;
+  update_step<MODEL, BLK, false>(n, t, id, sm, ls);
 }
 
 template <int MODEL, int BLK>
@@ -592,7 +644,9 @@ __device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long l
 // Persistent step loop: one cooperative launch runs `nsteps` steps.  A grid
 // barrier ends every update(t); on steps that pull, another one separates
 // the pull from update(t).  Every block reads the same stamps, so all take
-// the same barriers.
+// the same barriers.  When the grid covers every id with one lane, each lane
+// keeps its id's parameters and state in registers for the whole launch and
+// stores the state once at the end.
 template <int MODEL, int BLK>
 __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nsteps, uint64_t* t_base) {
   __shared__ UpdateSmem<BLK> usm;
@@ -604,6 +658,13 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   unsigned long long* err = counter(n.counters, kCntOverflow, 0);
   const uint64_t t0 = *t_base;
   extern __shared__ uint64_t dyn[];
+  const bool carry = (uint64_t)gridDim.x * BLK >= n.T;
+  const uint32_t id = blockIdx.x * BLK + threadIdx.x;
+  LaneState<MODEL> ls;
+  if (carry) {
+    load_lane<MODEL>(n, id, ls);
+    if (threadIdx.x < n.drive_k) usm.cdf[threadIdx.x] = n.drive_cdf[threadIdx.x];
+  }
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
     const uint64_t due = due_mask(n, t);
@@ -614,10 +675,15 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       }
       if (!grid_barrier(n.barrier, bar_sh, err)) return;
     }
-    for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
-      update_block<MODEL, BLK>(n, t, base + threadIdx.x, usm);
+    if (carry) {
+      update_step<MODEL, BLK, true>(n, t, id, usm, ls);
+    } else {
+      for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
+        update_block<MODEL, BLK>(n, t, base + threadIdx.x, usm);
+    }
     if (!grid_barrier(n.barrier, bar_sh, err)) return;
   }
+  if (carry) store_lane<MODEL>(n, id, ls);
   if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
 }
 

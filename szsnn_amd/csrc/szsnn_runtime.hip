@@ -418,7 +418,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint32_t groups = 1;
   if (!pull_off.empty()) {
     const uint64_t want = ((uint64_t)4 * std::max(ctx->cu_count, 1) + NB - 1) / std::max<uint64_t>(NB, 1);
-    groups = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, (uint64_t)D, (uint64_t)8}));
+    groups = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, (uint64_t)D, (uint64_t)kMaxPullGroups}));
   }
   uint64_t *keys = nullptr, *src_thresh = nullptr, *drive_cdf = nullptr, *dbits = nullptr, *dstamp = nullptr;
   uint32_t* soff = nullptr;
