@@ -7,8 +7,10 @@ Prints one JSON line (the driver's contract) with:
   value          synaptic events/s over the timed steps (natural dynamics)
   roofline       the pull gather kernel at the volley load (all neurons forced
                  every step: every synapse delivered each step), HIP-event
-                 timed; algorithmic bytes = 8 B per delivered synapse entry +
-                 16 B per target and delay group written
+                 timed; algorithmic bytes = 8 B per delivered synapse (f32
+                 weight + u32 target) + 8 B per delivered (source, delay)
+                 row; `traffic` = HBM bytes per launch from the rocprofv3
+                 FETCH_SIZE / WRITE_SIZE passes recorded in profiles/
   cpu_baseline   the FP32 CPU restatement (one thread) on a bounded sample
 """
 import argparse
@@ -39,8 +41,41 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tiling")
     ap.add_argument("--profile-pass", action="store_true",
-                    help="counter pass: a 1 GiB device copy (calibration) then volley steps only")
+                    help="profiler pass: a 1 GiB device copy (calibration), the volley's delay slots filled by "
+                         "one persistent launch, then exactly --volley-steps gather_kernel launches")
+    ap.add_argument("--gather-block", type=int, default=256, help="profile pass: pull gather block size")
+    ap.add_argument("--gather-grid", type=int, default=4096, help="profile pass: pull gather grid")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_pull_volley_traffic.json"),
+                    help="HBM bytes per pull gather launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)")
     return ap.parse_args()
+
+
+def gather_alg_bytes(events, rows):
+    """Algorithmic bytes of one gather launch: an f32 weight and a u32 target
+    per delivered synapse, and a (u32 start, u32 len) row pointer per
+    delivered (source, delay) row (SynapseData.hpp:51-52, types.h:38)."""
+    return 8 * events + 8 * rows
+
+
+def copy_bandwidth_gbs(torch, nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (read + write bytes / time): the practical
+    HBM ceiling next to the 8 TB/s peak."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def log(*a):
@@ -196,10 +231,24 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+        # fill every delay slot with one persistent launch (no gather_kernel
+        # dispatch), then launch gather_kernel exactly --volley-steps times
+        sim.set_tiling(gather_block=args.gather_block, gather_grid=args.gather_grid)
+        sim.set_mode(1)
         sim.set_volley(net.n_neurons)
         sim.run(net.max_delay + 2)
+        ctx.synchronize()
+        c0 = sim.counters()
         g_ms, u_ms = sim.time_kernels(args.volley_steps)
-        log(f"[bench] profile pass: volley gather {g_ms * 1e3 / args.volley_steps:.1f} us/launch")
+        c1 = sim.counters()
+        n = args.volley_steps
+        ev = (c1["events"] - c0["events"]) / n
+        rw = (c1["rows"] - c0["rows"]) / n
+        per = gather_alg_bytes(ev, rw)
+        print(json.dumps({"profile_pass": True, "gather_block": args.gather_block, "gather_grid": args.gather_grid,
+                          "launches": n, "gather_us": g_ms * 1e3 / n, "update_us": u_ms * 1e3 / n,
+                          "events_per_launch": ev, "rows_per_launch": rw, "alg_bytes_per_launch": per,
+                          "gbs": per / (g_ms * 1e-3 / n) / 1e9}), flush=True)
         return
     if not args.no_autotune:
         log(f"[bench] autotuned for the natural dynamics: {sim.autotune(200)}")
@@ -269,8 +318,11 @@ def main():
     log(f"[bench] volley push-only: {v_push}")
     # default policy: every step is deferred and pulled
     v_auto = volley(default_policy)
-    per = 8 * v_auto["events_per_launch"] + 16 * net.n_neurons * groups
+    per = gather_alg_bytes(v_auto["events_per_launch"], v_auto["rows_per_launch"])
     v_auto["alg_bytes_per_launch"] = per
+    # what the pull layout moves besides the entries: one 16-byte E/I sum per
+    # target and delay group, written once per launch
+    v_auto["pull_bytes_per_launch"] = 8 * v_auto["events_per_launch"] + 16 * net.n_neurons * groups
     v_auto["gbs"] = per / (v_auto["gather_us"] * 1e-6) / 1e9
     log(f"[bench] volley pull (policy {default_policy}): {v_auto}")
     sim.set_gather_policy(default_policy)
@@ -280,6 +332,12 @@ def main():
     achieved = v_auto["gbs"]
 
     del sim
+    copy_gbs = copy_bandwidth_gbs(torch)
+    log(f"[bench] device copy: {copy_gbs:.0f} GB/s (read + write)")
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        with open(args.traffic_file) as f:
+            traffic = json.load(f)
     batched = None if args.no_batched else run_batched(args, ctx, log)
     adjoint = None if args.no_adjoint else run_adjoint(args, log)
 
@@ -321,9 +379,12 @@ def main():
         "volley_push_only": v_push,
         "volley_pull": v_auto,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                     "traffic_detail": traffic,
                      "kernel": "gather_kernel pull (volley: all 100k neurons forced each step)",
-                     "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6},
+                     "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6,
+                     "copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
