@@ -130,6 +130,13 @@ __device__ inline bool izh_update(NeuronRegs<kIzhikevich>& r, float accE, float 
   return spike;
 }
 
+// x of lane j (j uniform across the wave).
+__device__ inline uint64_t readlane64(uint64_t x, uint32_t j) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Block-wide exclusive scan of one u32 per thread; returns the total.
 template <int BLK>
 __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, uint32_t* s_wave) {
@@ -172,6 +179,43 @@ __device__ inline uint64_t due_mask(const DevNet& n, uint64_t t) {
   return __ballot(due);
 }
 
+// A block pushing at most this many spiking ids walks them from an unordered
+// LDS list; more take the block-wide scan.
+constexpr uint32_t kSparseIds = 8;
+
+// Push entries loaded per thread and batch.
+constexpr uint32_t kPushBatch = 4;
+
+// Adds a thread's batch of loaded push entries into the accumulator ring and
+// counts them per delay in ev.  Every loaded entry is consumed before the
+// first atomic, so the atomics issue back to back.  Consecutive lanes hold
+// consecutive entries, sorted by delay within an id, so one LDS add per run
+// of equal delays counts the events.
+__device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[kPushBatch], uint32_t first_slot,
+                                    uint32_t* ev) {
+  const uint32_t lane = __lane_id();
+  long long* dst[kPushBatch];
+#pragma unroll
+  for (uint32_t u = 0; u < kPushBatch; ++u) {
+    const uint32_t k = sw[u].x >> kSynDelayShift;
+    const bool valid = sw[u].y != 0u;
+    const uint32_t kp = __shfl_up(k, 1);
+    const bool head = valid && (lane == 0 || kp != k);
+    const uint64_t bound = __ballot(head) | ~__ballot(valid);
+    if (head) {
+      const uint64_t rest = lane == 63 ? 0ull : bound >> (lane + 1);
+      atomicAdd(&ev[k], rest ? (uint32_t)__ffsll((unsigned long long)rest) : 64u - lane);
+    }
+    uint32_t ds = first_slot + k;
+    if (ds >= n.R) ds -= n.R;
+    const int32_t w = (int32_t)sw[u].y;
+    dst[u] = n.acc + (size_t)ds * 2 * n.npad + (w < 0 ? n.npad : 0u) + (sw[u].x & kSynTargetMask);
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < kPushBatch; ++u)
+    if (sw[u].y != 0u) atomicAdd((unsigned long long*)dst[u], (unsigned long long)(long long)(int32_t)sw[u].y);
+}
+
 // LDS of one update block.
 template <int BLK>
 struct UpdateSmem {
@@ -181,6 +225,8 @@ struct UpdateSmem {
   uint32_t start[BLK];       // push: first synapse of each lane's id
   uint32_t pref[BLK + 1];    // push: exclusive scan of the lanes' synapse counts
   uint32_t wave[BLK / 64 + 1];
+  uint2 list[kSparseIds];    // push, few spiking ids: (first synapse, count), any order
+  uint32_t nlist;            // spiking ids with synapses this step (list holds the first kSparseIds)
   unsigned long long spikes, drive;
   unsigned long long spk_prev;  // spikes of step t - 1
   uint64_t due;              // due_mask(t)
@@ -304,7 +350,10 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
   if (threadIdx.x < 64) {
     sm.cnt[threadIdx.x] = 0;
     sm.ev[threadIdx.x] = 0;
-    if (threadIdx.x == 0) sm.spikes = sm.drive = 0;
+    if (threadIdx.x == 0) {
+      sm.spikes = sm.drive = 0;
+      sm.nlist = 0;
+    }
     step_facts<BLK>(n, t, slot, blockIdx.x == 0, sm, fold_val);
   }
   __syncthreads();
@@ -387,52 +436,98 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
   if (kdrive) atomicAdd(&sm.drive, (unsigned long long)kdrive);
   if (mask) {
     // rows per delay: the wave's spiking ids with a synapse of that delay
-    uint64_t uni = bits;
+    if (__popcll(mask) <= 8) {
+      // few spikes: lane k counts bit k of each spiking lane's delay bits
+      uint32_t c = 0;
+      for (uint64_t m = mask; m; m &= m - 1) {
+        const uint64_t b = readlane64(bits, (uint32_t)__ffsll((unsigned long long)m) - 1);
+        c += (uint32_t)((b >> lane) & 1ull);
+      }
+      if (c) atomicAdd(&sm.cnt[lane], c);
+    } else {
+      uint64_t uni = bits;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
-    for (uint64_t u = uni; u; u &= u - 1) {
-      const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
-      const uint64_t m = __ballot((bits >> k) & 1ull);
-      if (lane == 0) atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
+      for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
+      for (uint64_t u = uni; u; u &= u - 1) {
+        const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
+        const uint64_t m = __ballot((bits >> k) & 1ull);
+        if (lane == 0) atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
+      }
     }
   }
+  // spiking ids with synapses, on a pushed step: appended to the block's list
+  // (one LDS atomic per wave; the order is free, the sums are exact)
+  const bool has = !defer && spike && r1 > r0;
+  const uint64_t hm = __ballot(has);
+  if (hm) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&sm.nlist, (uint32_t)__popcll(hm));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+    const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+    if (has && pos < kSparseIds) sm.list[pos] = make_uint2(r0, r1 - r0);
+  }
+  // the consumed slot is cleared now: nothing later in the step reads it,
+  // and no push of this step targets it (delays k < D land in the other
+  // R - 1 slots)
+  if (ring_e) st_agent(&ring[id], 0ll);
+  if (ring_i) st_agent(&ring[n.npad + id], 0ll);
   __syncthreads();
-  if (!defer && sm.spikes) {
-    // push: the block's spiking ids' synapse ranges are scanned and walked
-    // contiguously; each synapse adds its weight into the slot of step
-    // t + k + 1 (scatter_kernel.cl:194-355 and gather_kernel.cl:67-200,
-    // folded into the spiking step)
-    sm.start[threadIdx.x] = r0;
-    const uint32_t total = block_exclusive_scan<BLK>(spike ? r1 - r0 : 0u, sm.pref, sm.wave);
+  const uint32_t nlist = sm.nlist;
+  if (nlist) {
+    // push: the synapses of the block's spiking ids, each adding its weight
+    // into the slot of step t + k + 1 (scatter_kernel.cl:194-355 and
+    // gather_kernel.cl:67-200, folded into the spiking step).  Entry e of
+    // the concatenated ranges goes to thread e % BLK, batch e / BLK.
     const uint32_t first_slot = slot + 1 == n.R ? 0u : slot + 1;
-    constexpr uint32_t kBatch = 4;
-    for (uint32_t e0 = 0; e0 < total; e0 += BLK * kBatch) {
-      uint2 sw[kBatch];
+    if (nlist <= kSparseIds) {
+      // few ids: the ranges come from the list, located by a short linear scan
+      uint32_t st[kSparseIds], len[kSparseIds];
+      uint32_t total = 0;
 #pragma unroll
-      for (uint32_t u = 0; u < kBatch; ++u) {
-        const uint32_t e = e0 + u * BLK + threadIdx.x;
-        sw[u] = make_uint2(0u, 0u);
-        if (e < total) {
-          uint32_t lo = 0, hi = BLK;
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (sm.pref[mid] <= e) lo = mid; else hi = mid;
-          }
-          sw[u] = n.syn[sm.start[lo] + (e - sm.pref[lo])];
-        }
+      for (uint32_t q = 0; q < kSparseIds; ++q) {
+        const uint2 v = q < nlist ? sm.list[q] : make_uint2(0u, 0u);
+        st[q] = v.x;
+        len[q] = v.y;
+        total += v.y;
       }
+      for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
+        uint2 sw[kPushBatch];
 #pragma unroll
-      for (uint32_t u = 0; u < kBatch; ++u) {
-        if (e0 + u * BLK + threadIdx.x < total) {
-          const uint32_t k = sw[u].x >> kSynDelayShift;
-          const uint32_t tgt = sw[u].x & kSynTargetMask;
-          const int32_t w = (int32_t)sw[u].y;
-          uint32_t ds = first_slot + k;
-          if (ds >= n.R) ds -= n.R;
-          long long* dst = n.acc + (size_t)ds * 2 * n.npad + (w < 0 ? n.npad : 0u) + tgt;
-          atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)w);
-          atomicAdd(&sm.ev[k], 1u);
+        for (uint32_t u = 0; u < kPushBatch; ++u) {
+          const uint32_t e = e0 + u * BLK + threadIdx.x;
+          sw[u] = make_uint2(0u, 0u);  // weight 0: no synapse (stored weights are nonzero)
+          if (e < total) {
+            uint32_t src = 0, acc = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < kSparseIds; ++q) {
+              if (e >= acc && e < acc + len[q]) src = st[q] + (e - acc);
+              acc += len[q];
+            }
+            sw[u] = n.syn[src];
+          }
         }
+        push_entries(n, sw, first_slot, sm.ev);
+      }
+    } else {
+      // many ids: block-wide scan of the lanes' counts, binary search per entry
+      sm.start[threadIdx.x] = r0;
+      const uint32_t total = block_exclusive_scan<BLK>(spike ? r1 - r0 : 0u, sm.pref, sm.wave);
+      for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
+        uint2 sw[kPushBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kPushBatch; ++u) {
+          const uint32_t e = e0 + u * BLK + threadIdx.x;
+          sw[u] = make_uint2(0u, 0u);
+          if (e < total) {
+            uint32_t lo = 0, hi = BLK;
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (sm.pref[mid] <= e) lo = mid; else hi = mid;
+            }
+            sw[u] = n.syn[sm.start[lo] + (e - sm.pref[lo])];
+          }
+        }
+        push_entries(n, sw, first_slot, sm.ev);
       }
     }
     __syncthreads();
@@ -453,9 +548,6 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
     }
     if (sm.drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), sm.drive);
   }
-  // writes nothing later in the step waits for
-  if (ring_e) st_agent(&ring[id], 0ll);
-  if (ring_i) st_agent(&ring[n.npad + id], 0ll);
   if (blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_val);
 }
 
