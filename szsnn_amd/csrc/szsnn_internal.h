@@ -111,10 +111,13 @@ __device__ inline void st_agent(T* p, T v) {
 }
 
 // Grid barrier of the persistent loop: kBarLeaves leaf counters, a root
-// counter and a generation word, each on its own 128-byte line.
+// counter and a generation word (64-bit words), each on its own 128-byte line.
 constexpr uint32_t kBarLeaves = 16;
 constexpr uint32_t kBarStride = 32;  // u32 words per line
 constexpr uint32_t kBarWords = (kBarLeaves + 2) * kBarStride;
+// Leaf and root words: arrivals in the low kBarCountBits, the carried sum above.
+constexpr uint32_t kBarCountBits = 16;
+constexpr unsigned long long kBarCountMask = (1ull << kBarCountBits) - 1;
 // Spin limit of a barrier wait (about a second); reaching it (never with a
 // co-resident grid) adds kBarTimeout to the overflow counter and ends the
 // kernel.

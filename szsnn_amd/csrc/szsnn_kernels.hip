@@ -130,6 +130,10 @@ __device__ inline bool izh_update(NeuronRegs<kIzhikevich>& r, float accE, float 
   return spike;
 }
 
+// s_waitcnt immediate of gfx9-family targets waiting for vmcnt = 0 only
+// (expcnt 7 and lgkmcnt 15 mean no wait on those counters).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 // x of lane j (j uniform across the wave).
 __device__ inline uint64_t readlane64(uint64_t x, uint32_t j) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)j);
@@ -191,12 +195,12 @@ constexpr uint32_t kPushBatch = 4;
 // first atomic, so the atomics issue back to back.  Consecutive lanes hold
 // consecutive entries, sorted by delay within an id, so one LDS add per run
 // of equal delays counts the events.
-__device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[kPushBatch], uint32_t first_slot,
-                                    uint32_t* ev) {
+template <uint32_t NB>
+__device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[NB], uint32_t first_slot, uint32_t* ev) {
   const uint32_t lane = __lane_id();
-  long long* dst[kPushBatch];
+  long long* dst[NB];
 #pragma unroll
-  for (uint32_t u = 0; u < kPushBatch; ++u) {
+  for (uint32_t u = 0; u < NB; ++u) {
     const uint32_t k = sw[u].x >> kSynDelayShift;
     const bool valid = sw[u].y != 0u;
     const uint32_t kp = __shfl_up(k, 1);
@@ -212,7 +216,7 @@ __device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[kPushBatc
     dst[u] = n.acc + (size_t)ds * 2 * n.npad + (w < 0 ? n.npad : 0u) + (sw[u].x & kSynTargetMask);
   }
 #pragma unroll
-  for (uint32_t u = 0; u < kPushBatch; ++u)
+  for (uint32_t u = 0; u < NB; ++u)
     if (sw[u].y != 0u) atomicAdd((unsigned long long*)dst[u], (unsigned long long)(long long)(int32_t)sw[u].y);
 }
 
@@ -230,6 +234,15 @@ struct UpdateSmem {
   unsigned long long spikes, drive;
   unsigned long long spk_prev;  // spikes of step t - 1
   uint64_t due;              // due_mask(t)
+};
+
+// Block-uniform facts of step t known before it starts: the pull's due mask
+// (bit k: step t - k - 1 was deferred) and the spikes of step t - 1.  The
+// persistent loop carries them across steps (the grid barrier returns each
+// step's spike total); a single-step launch reads them with step_facts.
+struct StepFacts {
+  uint64_t due;
+  unsigned long long spk_prev;
 };
 
 // Block-uniform facts of step t, gathered by one wave (the block's first)
@@ -329,8 +342,8 @@ __device__ inline void store_lane(const DevNet& n, uint32_t id, const LaneState<
 // state and keeps it (the caller stores it at the end); otherwise it is
 // loaded and stored here, and the drive table is staged in LDS here.
 template <int MODEL, int BLK, bool CARRY>
-__device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm,
-                                   LaneState<MODEL>& ls) {
+__device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm,
+                                                 LaneState<MODEL>& ls, const StepFacts& facts) {
   const uint32_t lane = __lane_id();
   const uint32_t slot = (uint32_t)(t % n.R);
   // every load the lane needs, issued before the first barrier
@@ -354,13 +367,14 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
       sm.spikes = sm.drive = 0;
       sm.nlist = 0;
     }
-    step_facts<BLK>(n, t, slot, blockIdx.x == 0, sm, fold_val);
+    if (!CARRY) step_facts<BLK>(n, t, slot, blockIdx.x == 0, sm, fold_val);
   }
   __syncthreads();
-  const uint64_t due = sm.due;
+  const uint64_t due = CARRY ? facts.due : sm.due;
+  const unsigned long long spk_prev = CARRY ? facts.spk_prev : sm.spk_prev;
   // step t is deferred when the spikes of step t - 1 plus the forced ids
   // reach the threshold (the same decision in every block)
-  const bool defer = n.pull_off && sm.spk_prev + n.volley + n.forced_count >= n.defer_spikes;
+  const bool defer = n.pull_off && spk_prev + n.volley + n.forced_count >= n.defer_spikes;
   const uint32_t r0 = ls.r0, r1 = ls.r1;
   const uint64_t dbits = ls.dbits;
   bool spike = false;
@@ -422,6 +436,11 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
     }
     if (!CARRY) store_lane<MODEL>(n, id, ls);
   }
+  // Every load of the step so far has been consumed.  Saying so explicitly
+  // (vmcnt 0) keeps the compiler from waiting on the stores and atomics issued
+  // below at later branch joins, where it cannot tell which conditional loads
+  // are still in flight.
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   const uint64_t bits = spike ? dbits : 0;
   // dense spike output and history, one u64 word per wavefront
   // (storeDenseSpikes, IF_curr_exp_update_kernel.cl:58-94)
@@ -506,12 +525,12 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
             sw[u] = n.syn[src];
           }
         }
-        push_entries(n, sw, first_slot, sm.ev);
+        push_entries<kPushBatch>(n, sw, first_slot, sm.ev);
       }
     } else {
       // many ids: block-wide scan of the lanes' counts, binary search per entry
       sm.start[threadIdx.x] = r0;
-      const uint32_t total = block_exclusive_scan<BLK>(spike ? r1 - r0 : 0u, sm.pref, sm.wave);
+      const uint32_t total = block_exclusive_scan<BLK>(has ? r1 - r0 : 0u, sm.pref, sm.wave);
       for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
         uint2 sw[kPushBatch];
 #pragma unroll
@@ -527,7 +546,7 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
             sw[u] = n.syn[sm.start[lo] + (e - sm.pref[lo])];
           }
         }
-        push_entries(n, sw, first_slot, sm.ev);
+        push_entries<kPushBatch>(n, sw, first_slot, sm.ev);
       }
     }
     __syncthreads();
@@ -548,13 +567,14 @@ __device__ inline void update_step(const DevNet& n, uint64_t t, uint32_t id, Upd
     }
     if (sm.drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), sm.drive);
   }
-  if (blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_val);
+  if (!CARRY && blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_val);
+  return sm.spikes;
 }
 
 template <int MODEL, int BLK>
 __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
   LaneState<MODEL> ls;
-  update_step<MODEL, BLK, false>(n, t, id, sm, ls);
+  update_step<MODEL, BLK, false>(n, t, id, sm, ls, StepFacts{0, 0});
 }
 
 template <int MODEL, int BLK>
@@ -687,11 +707,13 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   }
 }
 
-// Grid barrier for a co-resident (cooperative) grid.  Arrival: block b adds
-// one to leaf counter b % kBarLeaves; the leaf's last block resets it and
-// adds one to the root; the root's last block resets it and advances the
-// generation word.  Blocks wait for the generation to change.  Spreading
-// arrivals over the leaves avoids funnelling every block through one
+// Grid barrier for a co-resident (cooperative) grid, carrying a sum.
+// Arrival: block b adds (1, its spikes) to leaf counter b % kBarLeaves (one
+// 64-bit word: arrivals in the low kBarCountBits, the sum above); the leaf's
+// last block resets it and adds (1, leaf sum) to the root; the root's last
+// block resets it and writes the generation word (generation + 1, grid sum).
+// Blocks wait for the generation to change and return the grid sum.
+// Spreading arrivals over the leaves avoids funnelling every block through one
 // contended address.  State shared across blocks is accessed agent-coherently
 // (ld_agent / st_agent / device atomics), so the barrier only has to make
 // every wave's accesses complete before its block arrives (s_waitcnt 0); an
@@ -699,8 +721,10 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
 // block and barrier, which costs microseconds.  The wait is bounded: at the
 // limit (never reached by a co-resident grid) it records kBarTimeout in the
 // overflow counter and returns false, and the caller leaves its step loop.
-// sh[0] is the block's copy of the generation, sh[1] its failure flag.
-__device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long long* err) {
+// sh[0] is the block's copy of the generation, sh[1] its failure flag,
+// sh[2..3] the returned sum.
+__device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long long* err, unsigned long long add,
+                                    unsigned long long* sum) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -709,17 +733,24 @@ __device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long l
     const uint32_t leaf = blockIdx.x % kBarLeaves;
     const uint32_t leaves = min(nb, kBarLeaves);
     const uint32_t leaf_size = (nb - leaf + kBarLeaves - 1) / kBarLeaves;
-    uint32_t* gen = bar + (kBarLeaves + 1) * kBarStride;
+    unsigned long long* leaf_w = reinterpret_cast<unsigned long long*>(bar + leaf * kBarStride);
+    unsigned long long* root_w = reinterpret_cast<unsigned long long*>(bar + kBarLeaves * kBarStride);
+    unsigned long long* gen_w = reinterpret_cast<unsigned long long*>(bar + (kBarLeaves + 1) * kBarStride);
     const uint32_t g = sh[0];
-    if (atomicAdd(bar + leaf * kBarStride, 1u) == leaf_size - 1) {
-      atomicExch(bar + leaf * kBarStride, 0u);
-      if (atomicAdd(bar + kBarLeaves * kBarStride, 1u) == leaves - 1) {
-        atomicExch(bar + kBarLeaves * kBarStride, 0u);
-        atomicAdd(gen, 1u);
+    const unsigned long long lo = atomicAdd(leaf_w, 1ull + (add << kBarCountBits));
+    if ((lo & kBarCountMask) == leaf_size - 1) {
+      atomicExch(leaf_w, 0ull);
+      const unsigned long long lsum = (lo >> kBarCountBits) + add;
+      const unsigned long long ro = atomicAdd(root_w, 1ull + (lsum << kBarCountBits));
+      if ((ro & kBarCountMask) == leaves - 1) {
+        atomicExch(root_w, 0ull);
+        const unsigned long long tot = (ro >> kBarCountBits) + lsum;
+        atomicExch(gen_w, (tot << 32) | (unsigned long long)(g + 1u));
       }
     }
+    unsigned long long v = 0;
     uint32_t spins = 0;
-    while (ld_agent(gen) == g) {
+    while ((uint32_t)(v = ld_agent(gen_w)) == g) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > kBarSpinLimit) {
         atomicOr(err, kBarTimeout);
@@ -728,53 +759,95 @@ __device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long l
       }
     }
     sh[0] = g + 1;
+    sh[2] = (uint32_t)(v >> 32);
   }
   __syncthreads();
+  *sum = sh[2];
   return sh[1] == 0;
 }
 
 // Persistent step loop: one cooperative launch runs `nsteps` steps.  A grid
 // barrier ends every update(t); on steps that pull, another one separates
-// the pull from update(t).  Every block reads the same stamps, so all take
-// the same barriers.  When the grid covers every id with one lane, each lane
-// keeps its id's parameters and state in registers for the whole launch and
-// stores the state once at the end.
+// the pull from update(t).  Every block takes the same barriers.  When the
+// grid covers every id with one lane, each lane keeps its id's parameters and
+// state in registers for the whole launch and stores the state once at the
+// end, and the step facts are carried in registers: the due mask as a shift
+// register of deferral bits (step s deferred with spikes: the same decision
+// in every block, from the barrier's spike total), so a step starts with no
+// loads but its accumulators.  Then the delivery-time counters are folded by
+// the first 2 * kRingStripes blocks (one stripe each: exchanged with 0 at the
+// end of the step, added to the counters after the next barrier) instead of
+// by block 0 alone.
 template <int MODEL, int BLK>
 __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nsteps, uint64_t* t_base) {
   __shared__ UpdateSmem<BLK> usm;
-  __shared__ uint32_t bar_sh[2];
-  if (threadIdx.x == 0) {
-    bar_sh[0] = ld_agent(n.barrier + (kBarLeaves + 1) * kBarStride);
-    bar_sh[1] = 0;
-  }
+  __shared__ uint32_t bar_sh[4];
+  __shared__ StepFacts f0;
   unsigned long long* err = counter(n.counters, kCntOverflow, 0);
   const uint64_t t0 = *t_base;
-  extern __shared__ uint64_t dyn[];
   const bool carry = (uint64_t)gridDim.x * BLK >= n.T;
+  if (threadIdx.x < 64) {
+    const uint32_t lane = __lane_id();
+    if (lane == 0) {
+      bar_sh[0] = (uint32_t)ld_agent(reinterpret_cast<unsigned long long*>(n.barrier + (kBarLeaves + 1) * kBarStride));
+      bar_sh[1] = 0;
+      f0.spk_prev = 0;
+    }
+    if (carry) {
+      // the facts of step t0, as step_facts reads them
+      const uint32_t k = lane < n.D ? lane : 0u;
+      const uint64_t stamp = n.pull_off ? ld_agent(&n.dstamp[(t0 - k - 1) % n.R]) : 0ull;
+      const unsigned long long spk = ld_agent(spk_counter(n.spk_ring, t0 - 1, lane));
+      const uint64_t due = __ballot(n.pull_off && lane < n.D && t0 >= (uint64_t)lane + 1 && stamp == t0 - lane);
+      if (lane == 0) f0.due = due;
+      if (t0 > 0 && lane < kRingStripes && spk) atomicAdd(&f0.spk_prev, spk);
+    }
+  }
+  extern __shared__ uint64_t dyn[];
   const uint32_t id = blockIdx.x * BLK + threadIdx.x;
   LaneState<MODEL> ls;
   if (carry) {
     load_lane<MODEL>(n, id, ls);
     if (threadIdx.x < n.drive_k) usm.cdf[threadIdx.x] = n.drive_cdf[threadIdx.x];
   }
+  __syncthreads();
+  StepFacts f = f0;
+  const uint64_t dmask = n.D >= 64 ? ~0ull : (1ull << n.D) - 1ull;
+  // counter fold of the carry path: lane L of wave 0 owns stripe q
+  const uint32_t q = blockIdx.x + __lane_id() * gridDim.x;
+  const bool folds = carry && threadIdx.x < 64 && q < 2 * kRingStripes;
+  unsigned long long fold_v = 0;
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
-    const uint64_t due = due_mask(n, t);
+    if (fold_v) {
+      atomicAdd(counter(n.counters, q < kRingStripes ? kCntEvents : kCntRows, q % kRingStripes), fold_v);
+      fold_v = 0;
+    }
+    unsigned long long sum = 0;
+    const uint64_t due = carry ? f.due : due_mask(n, t);
     if (due) {
       for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
         pull_body<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
         __syncthreads();
       }
-      if (!grid_barrier(n.barrier, bar_sh, err)) return;
+      if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
     }
     if (carry) {
-      update_step<MODEL, BLK, true>(n, t, id, usm, ls);
+      const unsigned long long spikes = update_step<MODEL, BLK, true>(n, t, id, usm, ls, f);
+      const uint32_t slot = (uint32_t)(t % n.R);
+      if (folds) fold_v = atomicExch(ring_counter(n.ring_cnt, slot, q / kRingStripes, q % kRingStripes), 0ull);
+      if (threadIdx.x < 64 && q < kRingStripes) st_agent(spk_counter(n.spk_ring, t + 1, q), 0ull);
+      if (!grid_barrier(n.barrier, bar_sh, err, threadIdx.x == 0 ? spikes : 0ull, &sum)) return;
+      const bool deferred = n.pull_off && f.spk_prev + n.volley + n.forced_count >= n.defer_spikes;
+      f.due = ((f.due << 1) | (deferred && sum ? 1ull : 0ull)) & dmask;
+      f.spk_prev = sum;
     } else {
       for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
         update_block<MODEL, BLK>(n, t, base + threadIdx.x, usm);
+      if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
     }
-    if (!grid_barrier(n.barrier, bar_sh, err)) return;
   }
+  if (fold_v) atomicAdd(counter(n.counters, q < kRingStripes ? kCntEvents : kCntRows, q % kRingStripes), fold_v);
   if (carry) store_lane<MODEL>(n, id, ls);
   if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
 }
