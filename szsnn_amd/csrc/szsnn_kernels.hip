@@ -191,11 +191,13 @@ constexpr uint32_t kSparseIds = 8;
 constexpr uint32_t kPushBatch = 4;
 
 // Adds a thread's batch of loaded push entries into the accumulator ring and
-// counts them per delay in ev.  Every loaded entry is consumed before the
-// first atomic, so the atomics issue back to back.  Consecutive lanes hold
-// consecutive entries, sorted by delay within an id, so one LDS add per run
-// of equal delays counts the events.
-template <uint32_t NB>
+// counts them per delay: DIRECT, straight into the delivery-time counter ring
+// (few entries: these atomics drain together with the push's), else into the
+// block's LDS ev, flushed once per (block, delay).  Every loaded entry is
+// consumed before the first atomic, so the atomics issue back to back.
+// Consecutive lanes hold consecutive entries, sorted by delay within an id,
+// so one add per run of equal delays counts the events.
+template <uint32_t NB, bool DIRECT>
 __device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[NB], uint32_t first_slot, uint32_t* ev) {
   const uint32_t lane = __lane_id();
   long long* dst[NB];
@@ -206,12 +208,16 @@ __device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[NB], uint
     const uint32_t kp = __shfl_up(k, 1);
     const bool head = valid && (lane == 0 || kp != k);
     const uint64_t bound = __ballot(head) | ~__ballot(valid);
-    if (head) {
-      const uint64_t rest = lane == 63 ? 0ull : bound >> (lane + 1);
-      atomicAdd(&ev[k], rest ? (uint32_t)__ffsll((unsigned long long)rest) : 64u - lane);
-    }
     uint32_t ds = first_slot + k;
     if (ds >= n.R) ds -= n.R;
+    if (head) {
+      const uint64_t rest = lane == 63 ? 0ull : bound >> (lane + 1);
+      const uint32_t len = rest ? (uint32_t)__ffsll((unsigned long long)rest) : 64u - lane;
+      if (DIRECT)
+        atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)len);
+      else
+        atomicAdd(&ev[k], len);
+    }
     const int32_t w = (int32_t)sw[u].y;
     dst[u] = n.acc + (size_t)ds * 2 * n.npad + (w < 0 ? n.npad : 0u) + (sw[u].x & kSynTargetMask);
   }
@@ -343,7 +349,8 @@ __device__ inline void store_lane(const DevNet& n, uint32_t id, const LaneState<
 // loaded and stored here, and the drive table is staged in LDS here.
 template <int MODEL, int BLK, bool CARRY>
 __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm,
-                                                 LaneState<MODEL>& ls, const StepFacts& facts) {
+                                                 LaneState<MODEL>& ls, const StepFacts& facts,
+                                                 unsigned long long& out_drive) {
   const uint32_t lane = __lane_id();
   const uint32_t slot = (uint32_t)(t % n.R);
   // every load the lane needs, issued before the first barrier
@@ -457,12 +464,18 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     // rows per delay: the wave's spiking ids with a synapse of that delay
     if (__popcll(mask) <= 8) {
       // few spikes: lane k counts bit k of each spiking lane's delay bits
+      // and adds it straight into the delivery ring (issued now, it drains
+      // with the push)
       uint32_t c = 0;
       for (uint64_t m = mask; m; m &= m - 1) {
         const uint64_t b = readlane64(bits, (uint32_t)__ffsll((unsigned long long)m) - 1);
         c += (uint32_t)((b >> lane) & 1ull);
       }
-      if (c) atomicAdd(&sm.cnt[lane], c);
+      if (c) {
+        uint32_t ds = slot + 1 + lane;
+        if (ds >= n.R) ds -= n.R;
+        atomicAdd(ring_counter(n.ring_cnt, ds, 1, blockIdx.x), (unsigned long long)c);
+      }
     } else {
       uint64_t uni = bits;
 #pragma unroll
@@ -525,7 +538,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
             sw[u] = n.syn[src];
           }
         }
-        push_entries<kPushBatch>(n, sw, first_slot, sm.ev);
+        push_entries<kPushBatch, true>(n, sw, first_slot, sm.ev);
       }
     } else {
       // many ids: block-wide scan of the lanes' counts, binary search per entry
@@ -546,35 +559,44 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
             sw[u] = n.syn[sm.start[lo] + (e - sm.pref[lo])];
           }
         }
-        push_entries<kPushBatch>(n, sw, first_slot, sm.ev);
+        push_entries<kPushBatch, false>(n, sw, first_slot, sm.ev);
       }
     }
     __syncthreads();
   }
-  // counters: one atomic per (block, delay) into the delivery ring
+  // counters the dense paths kept in LDS: one atomic per (block, delay) into
+  // the delivery ring
   if (threadIdx.x < n.D) {
     const uint32_t k = threadIdx.x;
     uint32_t ds = slot + 1 + k;
     if (ds >= n.R) ds -= n.R;
-    if (sm.cnt[k]) atomicAdd(ring_counter(n.ring_cnt, ds, 1, blockIdx.x), (unsigned long long)sm.cnt[k]);
-    if (sm.ev[k]) atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)sm.ev[k]);
+    const uint32_t c = sm.cnt[k], e = sm.ev[k];
+    if (c) atomicAdd(ring_counter(n.ring_cnt, ds, 1, blockIdx.x), (unsigned long long)c);
+    if (e) atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)e);
   }
+  const unsigned long long spikes = sm.spikes, drive = sm.drive;
   if (threadIdx.x == 0) {
-    if (sm.spikes) {
-      atomicAdd(counter(n.counters, kCntSpikes, blockIdx.x), sm.spikes);
-      atomicAdd(spk_counter(n.spk_ring, t, blockIdx.x), sm.spikes);
-      if (defer) st_agent(&n.dstamp[slot], t + 1);
+    if (spikes && defer) st_agent(&n.dstamp[slot], t + 1);
+    // CARRY: the caller sums the block's spikes and drive events over the
+    // launch and adds them (and the last step's spike slot) at its end
+    if (!CARRY) {
+      if (spikes) {
+        atomicAdd(counter(n.counters, kCntSpikes, blockIdx.x), spikes);
+        atomicAdd(spk_counter(n.spk_ring, t, blockIdx.x), spikes);
+      }
+      if (drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), drive);
     }
-    if (sm.drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), sm.drive);
   }
   if (!CARRY && blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_val);
-  return sm.spikes;
+  out_drive = drive;
+  return spikes;
 }
 
 template <int MODEL, int BLK>
 __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
   LaneState<MODEL> ls;
-  update_step<MODEL, BLK, false>(n, t, id, sm, ls, StepFacts{0, 0});
+  unsigned long long drive = 0;
+  update_step<MODEL, BLK, false>(n, t, id, sm, ls, StepFacts{0, 0}, drive);
 }
 
 template <int MODEL, int BLK>
@@ -817,6 +839,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   const uint32_t q = blockIdx.x + __lane_id() * gridDim.x;
   const bool folds = carry && threadIdx.x < 64 && q < 2 * kRingStripes;
   unsigned long long fold_v = 0;
+  unsigned long long spikes_acc = 0, drive_acc = 0;
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
     if (fold_v) {
@@ -833,10 +856,12 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
     }
     if (carry) {
-      const unsigned long long spikes = update_step<MODEL, BLK, true>(n, t, id, usm, ls, f);
+      unsigned long long drive = 0;
+      const unsigned long long spikes = update_step<MODEL, BLK, true>(n, t, id, usm, ls, f, drive);
+      spikes_acc += spikes;
+      drive_acc += drive;
       const uint32_t slot = (uint32_t)(t % n.R);
       if (folds) fold_v = atomicExch(ring_counter(n.ring_cnt, slot, q / kRingStripes, q % kRingStripes), 0ull);
-      if (threadIdx.x < 64 && q < kRingStripes) st_agent(spk_counter(n.spk_ring, t + 1, q), 0ull);
       if (!grid_barrier(n.barrier, bar_sh, err, threadIdx.x == 0 ? spikes : 0ull, &sum)) return;
       const bool deferred = n.pull_off && f.spk_prev + n.volley + n.forced_count >= n.defer_spikes;
       f.due = ((f.due << 1) | (deferred && sum ? 1ull : 0ull)) & dmask;
@@ -848,7 +873,21 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
     }
   }
   if (fold_v) atomicAdd(counter(n.counters, q < kRingStripes ? kCntEvents : kCntRows, q % kRingStripes), fold_v);
-  if (carry) store_lane<MODEL>(n, id, ls);
+  if (carry) {
+    store_lane<MODEL>(n, id, ls);
+    if (threadIdx.x == 0) {
+      if (spikes_acc) atomicAdd(counter(n.counters, kCntSpikes, blockIdx.x), spikes_acc);
+      if (drive_acc) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), drive_acc);
+    }
+    // the spike-count slots a following single-step launch reads (step
+    // t0 + nsteps - 1: the last total) and fills (the next step: zero)
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kRingStripes && nsteps) {
+      const uint32_t st = threadIdx.x % kRingStripes;
+      const uint64_t tl = t0 + nsteps - 1;
+      st_agent(spk_counter(n.spk_ring, threadIdx.x < kRingStripes ? tl : tl + 1, st),
+               threadIdx.x == 0 ? f.spk_prev : 0ull);
+    }
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
 }
 
