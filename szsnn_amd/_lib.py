@@ -74,6 +74,7 @@ class NetInfo(ctypes.Structure):
         ("pull_groups", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
         ("raster_words", ctypes.c_uint64),
+        ("pull_entry_bytes", ctypes.c_uint64),
     ]
 
 

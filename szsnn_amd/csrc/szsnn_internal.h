@@ -80,12 +80,20 @@ struct DevNet {
   uint32_t volley;          // ids [0, volley) are forced every step
   const uint64_t* forced_bits;   // [W] forced ids this step, or null
   uint64_t* hist;           // [R][W] spike bitmaps of the last R steps (slot t % R)
-  // pull layout (dense gather): target blocks of kPullTB neurons; synapses of
-  // block b and delay k at pull_syn[pull_off[b*D+k] .. pull_off[b*D+k+1]),
-  // sorted by target, each (source << kPullShift | target % kPullTB, weight);
-  // segments padded to a multiple of kPullRun with inert (weight 0) entries.
-  const uint32_t* pull_off; // [NB*D + 1] or null (no pull layout)
-  const uint2* pull_syn;
+  // pull layout (dense gather), a sliced ELLPACK: target blocks of kPullTB
+  // neurons; segment (b, k) holds the delay-k synapses into block b in
+  // kPullSlices slices of 64 targets, slice s at entries pull_off[(b*D+k)*
+  // kPullSlices + s] .. (next), lane L serving target pull_perm[(b*D+k)*
+  // kPullTB + s*64 + L] of the block; 16-byte vectors [L_s/V][64][V].  Wide
+  // entries: (source - pull_lo[b]*64, weight); narrow: (source - pull_lo[b]*
+  // 64) << wbits | weight index.  Padding entries: weight 0.
+  const uint32_t* pull_off; // [NB*D*kPullSlices + 1] or null (no pull layout)
+  const uint16_t* pull_perm;  // [NB*D*kPullTB]
+  const uint2* pull_syn;    // wide entries, or null
+  const uint32_t* pull_syn32;  // narrow entries, or null
+  const int32_t* pull_wtab; // narrow: [pull_wn] weight of each index
+  uint32_t pull_wbits;      // narrow: weight-index bits (0: wide entries)
+  uint32_t pull_wn;         // narrow: weight-table entries
   const uint32_t* pull_lo;  // [NB] first raster word of the block's sources
   uint32_t pull_nb;         // NB
   uint32_t pull_words;      // LDS words staged per block (max over blocks)

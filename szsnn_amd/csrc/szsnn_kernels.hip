@@ -606,96 +606,137 @@ __global__ void __launch_bounds__(BLK) update_kernel(DevNet n, uint32_t j) {
   update_block<MODEL, BLK>(n, t, blockIdx.x * BLK + threadIdx.x, sm);
 }
 
-// Dense gather (pull) for target block `b` and delay group `g`: for each
-// due delay k of the group the spike bitmap of step t-k-1 (the block's source
-// range) is staged in LDS; each lane walks kPullRun consecutive synapses of
-// the (b, k) segment (16-byte loads), keeps a running sum while the target
-// stays the same, and adds it into the block's LDS accumulators when the
-// target changes.  The block stores its targets' sums for the group into
-// pull slot g: plain stores, no global atomics.
-// dyn: pull_words u64 bitmap + 2*kPullTB i64.
-template <int BLK>
+// Component c (compile-time after unrolling) of a 16-byte load.
+__device__ inline uint32_t comp(const uint4& v, uint32_t c) {
+  return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+}
+
+// Entry formats of the pull: wide (source offset, weight); narrow with a
+// weight table of at most kPullCountWeights entries (per lane, counts of
+// selected entries per weight index); narrow with a larger table (FP64
+// table lookups).
+enum : int { kPullWide = 0, kPullCount = 1, kPullTable = 2 };
+constexpr uint32_t kPullCountWeights = 4;  // 16-bit counters packed in a u64
+
+// Dense gather (pull) for target block `b` and delay group `g` over the
+// sliced layout: for each due delay k of the group the spike bitmap of step
+// t-k-1 (the block's source range) is staged in LDS; each wave takes slices
+// of 64 targets, one target per lane, and walks its lane's entries with
+// 16-byte loads (the wave's j-th load is contiguous; the next batch is in
+// flight while the current one is summed).  Per entry the source's spike bit
+// b selects the weight.  Sums are exact: FP64 sums of integers below 2^53
+// (wide: E += b * max(w, 0), I += b * min(w, 0); table: (E, I) of index
+// (b, weight index), zero for b = 0), or, for small tables, 16-bit counts of
+// selected entries per weight index, multiplied out once per slice.  The
+// lane adds its sums into the block's LDS sums of its target (one lane per
+// target and delay, delays one after the other: no atomics); the block
+// stores its targets' sums for the group into pull slot g: plain stores, no
+// global atomics.
+// dyn: pull_words u64 bitmap | 2*kPullTB f64 sums | narrow formats: 2 << wb
+// (E, I) f64 pairs indexed by (spike bit, weight index), zero for bit 0 and
+// for indices beyond the table.
+template <int BLK, int FMT>
 __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
+  constexpr bool kNarrow = FMT != kPullWide;
+  constexpr uint32_t kV = kNarrow ? 4 : 2;  // entries per 16-byte vector
+  constexpr uint32_t kB = 4;                // vectors per lane and batch
+  constexpr uint32_t kWaves = BLK / 64;
+  // count format: flush the 16-bit counters before any can reach 2^16
+  constexpr uint32_t kFlushVectors = 16380;
   uint64_t* bits = dyn;
-  long long* acc = reinterpret_cast<long long*>(dyn + n.pull_words);
-  for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) acc[i] = 0;
+  double* sums = reinterpret_cast<double*>(dyn + n.pull_words);  // [2][kPullTB]: E, I
+  double2* wt = reinterpret_cast<double2*>(sums + 2 * kPullTB);  // [2 << wb]: (bit, index)
+  const uint32_t wb = n.pull_wbits, wmask = (1u << wb) - 1u;
+  for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) sums[i] = 0.0;
+  if (kNarrow)
+    for (uint32_t i = threadIdx.x; i < (2u << wb); i += BLK) {
+      const uint32_t wi = i & wmask;
+      const double w = (i >> wb) && wi < n.pull_wn ? (double)n.pull_wtab[wi] : 0.0;
+      wt[i] = make_double2(w > 0.0 ? w : 0.0, w < 0.0 ? w : 0.0);
+    }
   const uint32_t lo = n.pull_lo[b];
   const uint32_t nw = min(n.pull_words, n.W - lo);
   const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
   const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
-  unsigned long long events = 0;
+  const uint4* vec = kNarrow ? reinterpret_cast<const uint4*>(n.pull_syn32) : reinterpret_cast<const uint4*>(n.pull_syn);
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  uint32_t events = 0;
   for (uint32_t k = k0; k < k1; ++k) {
     if (!((due >> k) & 1ull)) continue;
-    const uint32_t s0 = n.pull_off[b * n.D + k], s1 = n.pull_off[b * n.D + k + 1];
-    if (s0 == s1) continue;
+    const uint32_t seg = b * n.D + k;
+    const uint32_t* so = n.pull_off + (size_t)seg * kPullSlices;
+    if (so[0] == so[kPullSlices]) continue;  // no delay-k synapses into the block
     const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
     __syncthreads();  // the previous delay's readers are done with `bits`
     for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = ld_agent(&src[w]);
     __syncthreads();
-    const uint32_t base_id = lo * 64;
-    const uint32_t lane = __lane_id();
-    constexpr uint32_t kChunk = 64 * kPullRun;
-    constexpr uint32_t kWaves = BLK / 64;
-    // one lane-interleaved chunk per wave: the j-th 16-byte load of the wave
-    // is contiguous, and lane L receives its kPullRun consecutive entries.
-    // The next chunk's loads are issued before the current one is processed.
-    uint32_t cs = s0 + (threadIdx.x >> 6) * kChunk;
-    uint4 q[kPullRun / 2];
-    uint32_t nl = 0;
-    if (cs < s1) {
-      nl = min(kChunk, s1 - cs) / kPullRun;  // segments are whole lane runs
-      if (lane < nl) {
-        const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + cs);
+    for (uint32_t sl = wave; sl < kPullSlices; sl += kWaves) {
+      const uint32_t s0 = so[sl], s1 = so[sl + 1];
+      if (s0 == s1) continue;
+      const uint32_t nv = (s1 - s0) / (64 * kV);  // vectors per lane
+      const uint32_t tl = n.pull_perm[(size_t)seg * kPullTB + sl * 64 + lane];
+      const uint4* p = vec + s0 / kV + lane;     // the lane's vector j at p[64 * j]
+      double ae = 0.0, ai = 0.0;
+      uint64_t cpk = 0;  // count format: 16-bit counts per weight index
+      // every load is unconditional (indices clamped to the lane's last
+      // vector; the repeats hit the same line), so the compiler can wait for
+      // the current batch alone while the next one stays in flight
+      const uint32_t last = nv - 1;
+      uint4 q[kB];
 #pragma unroll
-        for (uint32_t j = 0; j < kPullRun / 2; ++j) q[j] = p4[j * nl + lane];
-      }
-    }
-    while (cs < s1) {
-      const uint32_t ncs = cs + kWaves * kChunk;
-      uint4 qn[kPullRun / 2];
-      uint32_t nnl = 0;
-      if (ncs < s1) {
-        nnl = min(kChunk, s1 - ncs) / kPullRun;
-        if (lane < nnl) {
-          const uint4* p4 = reinterpret_cast<const uint4*>(n.pull_syn + ncs);
+      for (uint32_t u = 0; u < kB; ++u) q[u] = p[64 * min(u, last)];
+      for (uint32_t j0 = 0; j0 < nv; j0 += kB) {
+        uint4 qn[kB];
 #pragma unroll
-          for (uint32_t j = 0; j < kPullRun / 2; ++j) qn[j] = p4[j * nnl + lane];
-        }
-      }
-      if (lane < nl) {
-        // spike bits of the run's sources, looked up before the sums
-        uint32_t sel = 0;
+        for (uint32_t u = 0; u < kB; ++u) qn[u] = p[64 * min(j0 + kB + u, last)];
 #pragma unroll
-        for (uint32_t j = 0; j < kPullRun; ++j) {
-          const uint32_t px = (j & 1) ? q[j >> 1].z : q[j >> 1].x;
-          const uint32_t rel = (px >> kPullShift) - base_id;
-          sel |= (uint32_t)((bits[rel >> 6] >> (rel & 63)) & 1ull) << j;
-        }
-        uint32_t cur = 0xffffffffu;
-        long long se = 0, si = 0;
+        for (uint32_t u = 0; u < kB; ++u) {
+          if (j0 + u >= nv) break;
 #pragma unroll
-        for (uint32_t j = 0; j < kPullRun; ++j) {
-          const uint32_t px = (j & 1) ? q[j >> 1].z : q[j >> 1].x;
-          const int32_t w = (int32_t)((j & 1) ? q[j >> 1].w : q[j >> 1].y);
-          const uint32_t tl = px & (kPullTB - 1);
-          if (tl != cur) {
-            if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
-            if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
-            se = si = 0;
-            cur = tl;
-          }
-          if (((sel >> j) & 1u) && w != 0) {
-            if (w >= 0) se += w; else si += w;
-            ++events;
+          for (uint32_t c = 0; c < kV; ++c) {
+            if (FMT == kPullWide) {
+              const uint32_t rel = comp(q[u], 2 * c);
+              const int32_t w = (int32_t)comp(q[u], 2 * c + 1);
+              const uint32_t bit = (uint32_t)(bits[rel >> 6] >> (rel & 63)) & 1u;
+              const double bw = (double)(bit ? w : 0);
+              ae += bw > 0.0 ? bw : 0.0;
+              ai += bw < 0.0 ? bw : 0.0;
+              events += bit & (w != 0 ? 1u : 0u);
+            } else {
+              const uint32_t e = comp(q[u], c);
+              const uint32_t rel = e >> wb, wi = e & wmask;
+              const uint32_t bit = (uint32_t)(bits[rel >> 6] >> (rel & 63)) & 1u;
+              if (FMT == kPullCount) {
+                cpk += (uint64_t)bit << (16 * wi);
+              } else {
+                const uint32_t idx = (bit << wb) | wi;  // (bit, index): zero weights for bit 0
+                const double2 wv = wt[idx];
+                ae += wv.x;
+                ai += wv.y;
+                events += idx > (1u << wb) ? 1u : 0u;  // index 0 is the inert 0
+              }
+            }
           }
         }
-        if (se) atomicAdd((unsigned long long*)&acc[cur], (unsigned long long)se);
-        if (si) atomicAdd((unsigned long long*)&acc[kPullTB + cur], (unsigned long long)si);
-      }
 #pragma unroll
-      for (uint32_t j = 0; j < kPullRun / 2; ++j) q[j] = qn[j];
-      nl = nnl;
-      cs = ncs;
+        for (uint32_t u = 0; u < kB; ++u) q[u] = qn[u];
+        if (FMT == kPullCount && ((j0 / kB + 1) % (kFlushVectors / kB) == 0 || j0 + kB >= nv)) {
+          // counts -> sums (index 0: padding and inert entries)
+#pragma unroll
+          for (uint32_t i = 1; i < kPullCountWeights; ++i) {
+            const uint32_t c = (uint32_t)(cpk >> (16 * i)) & 0xffffu;
+            if (i < n.pull_wn) {
+              const double2 wv = wt[(1u << wb) + i];
+              ae += (double)c * wv.x;
+              ai += (double)c * wv.y;
+              events += c;
+            }
+          }
+          cpk = 0;
+        }
+      }
+      sums[tl] += ae;
+      sums[kPullTB + tl] += ai;
     }
   }
   __syncthreads();
@@ -703,15 +744,26 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
     const uint32_t tgt = b * kPullTB + i;
     if (tgt < n.N) {
-      st_agent(&out[tgt], acc[i]);
-      st_agent(&out[n.npad + tgt], acc[kPullTB + i]);
+      st_agent(&out[tgt], (long long)sums[i]);
+      st_agent(&out[n.npad + tgt], (long long)sums[kPullTB + i]);
     }
   }
   // event count: one atomic per wave
   unsigned long long ev = events;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) ev += __shfl_xor(ev, off);
-  if (__lane_id() == 0 && ev) atomicAdd(counter(n.counters, kCntEvents, blockIdx.x * 16 + (threadIdx.x >> 6)), ev);
+  if (lane == 0 && ev) atomicAdd(counter(n.counters, kCntEvents, blockIdx.x * 16 + wave), ev);
+}
+
+// The pull of work item (b, g) in the network's entry format.
+template <int BLK>
+__device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
+  if (!n.pull_wbits)
+    pull_body<BLK, kPullWide>(n, t, b, g, due, dyn);
+  else if (n.pull_wn <= kPullCountWeights)
+    pull_body<BLK, kPullCount>(n, t, b, g, due, dyn);
+  else
+    pull_body<BLK, kPullTable>(n, t, b, g, due, dyn);
 }
 
 // gather(t): the pull of the deferred steps whose synapses are due at t, one
@@ -724,7 +776,7 @@ __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   const uint64_t due = due_mask(n, t);
   if (!due) return;
   for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-    pull_body<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
+    pull_item<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
     __syncthreads();  // LDS reuse by the next work item
   }
 }
@@ -850,7 +902,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
     const uint64_t due = carry ? f.due : due_mask(n, t);
     if (due) {
       for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-        pull_body<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
+        pull_item<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
         __syncthreads();
       }
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
@@ -922,7 +974,8 @@ hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_
 }
 
 size_t pull_dyn_bytes(const DevNet& n) {
-  return n.pull_off ? (size_t)n.pull_words * 8 + (size_t)2 * kPullTB * 8 : 0;
+  return n.pull_off ? (size_t)n.pull_words * 8 + (size_t)2 * kPullTB * 8 + (n.pull_wbits ? (size_t)16 << (n.pull_wbits + 1) : 0)
+                    : 0;
 }
 
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {

@@ -10,16 +10,18 @@
 //     the delay k packed above the target (target | k << kSynDelayShift);
 //   * the delay bitmap per id (bit k set iff row (id, k) is nonempty, the
 //     reference's bitmap, SynapseData.ipp:70-83);
-//   * the pull layout for dense gathers: targets in blocks of kPullTB, one
-//     segment per (block, delay) sorted by target, entries
-//     (id << kPullShift | target % kPullTB, weight), segments padded to a
-//     multiple of kPullRun entries with inert entries (the block's first
-//     source word, weight 0).  Lane-interleaved chunks: a segment is cut into
-//     chunks of 64 * kPullRun entries (the last one shorter); lane L of a wave
-//     owns the chunk's entries L*kPullRun .. L*kPullRun+kPullRun-1 in sorted
-//     order, and entry pair j of lane L is stored at pair j*nl + L of the
-//     chunk (nl = lanes in the chunk), so the wave's j-th 16-byte load is
-//     contiguous.
+//   * the pull layout for dense gathers, a sliced ELLPACK: targets in blocks
+//     of kPullTB; per (block, delay k) segment the targets are sorted by
+//     their number of delay-k synapses and cut into kPullSlices slices of 64
+//     (one target per lane); a slice holds L entries per lane (its longest
+//     lane, rounded up to whole 16-byte vectors), stored as vectors
+//     [L / V][64 lanes][V entries], so the wave's j-th 16-byte load is
+//     contiguous.  Entries name the source relative to the block's first
+//     staged id.  Wide entries (8 bytes, V = 2): (source offset, weight).
+//     Narrow entries (4 bytes, V = 4), when the network has at most
+//     kPullMaxWeights distinct weights and the offsets fit: source offset <<
+//     wb | weight index, with a weight table whose index 0 is the inert 0.
+//     Padding entries are (offset 0, weight 0).
 // Shared by the runtime and by the CPU layout check (tests/cpp/layout_check.cpp).
 #pragma once
 
@@ -46,9 +48,13 @@ struct HostLayout {
   uint64_t active = 0;                // ids with at least one synapse
   std::vector<uint64_t> delay_bits;   // [T]
   uint64_t nonempty = 0;              // nonempty rows
-  std::vector<uint32_t> pull_off;     // [NB*D + 1] or empty
-  std::vector<uint32_t> pull_lo;      // [NB]
-  std::vector<LayoutPair> pull_syn;
+  std::vector<uint32_t> pull_off;     // [NB*D*kPullSlices + 1] slice starts (entries), or empty
+  std::vector<uint32_t> pull_lo;      // [NB] first staged raster word of each target block
+  std::vector<uint16_t> pull_perm;    // [NB*D][kPullTB] target (within the block) of each slice lane
+  std::vector<LayoutPair> pull_syn;   // wide entries (8 bytes), or empty
+  std::vector<uint32_t> pull_syn32;   // narrow entries (4 bytes), or empty
+  std::vector<int32_t> pull_wtab;     // narrow: weight of each index (index 0: the inert 0)
+  uint32_t pull_wbits = 0;            // narrow: weight-index bits (0: wide entries)
   uint32_t pull_words = 0;
   uint64_t NB = 0;
   std::string error;
@@ -113,76 +119,122 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
         ++L.nonempty;
       }
 
-  // pull layout
+  // pull layout: per (target block, delay) segment, kPullSlices slices of 64
+  // targets (targets sorted by their synapse count in the segment, so the
+  // slices' lengths are close); slice s holds L_s entries per lane, stored
+  // as 16-byte vectors [L_s / V][64 lanes][V entries] (V = 4 narrow, 2 wide)
   L.NB = (N + kPullTB - 1) / kPullTB;
   L.pull_off.clear();
   L.pull_lo.clear();
+  L.pull_perm.clear();
   L.pull_syn.clear();
+  L.pull_syn32.clear();
+  L.pull_wtab.clear();
+  L.pull_wbits = 0;
   L.pull_words = 0;
-  if (d.pull_layout < 0 || T >= (1ull << (32 - kPullShift)) || S == 0) return true;
+  if (d.pull_layout < 0 || S == 0) return true;
   const uint64_t NB = L.NB;
-  const uint64_t nkeys = NB * D * kPullTB;
-  std::vector<uint32_t> kcnt(nkeys + 1, 0);
+  const uint64_t nseg = NB * D;
+  std::vector<uint32_t> cnt(nseg * kPullTB, 0);  // synapses per (segment, target)
+  std::vector<uint32_t> indeg(N, 0);
   std::vector<uint32_t> smin(NB, 0xffffffffu), smax(NB, 0);
+  // distinct nonzero weights (up to kPullMaxWeights; more: wide entries)
+  std::vector<int32_t> wset;
+  bool narrow = d.pull_layout == 0;
+  int64_t wmax = 0;
   for (uint64_t id = 0; id < T; ++id)
     for (uint64_t k = 0; k < D; ++k)
       for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
         const uint32_t tg = L.syn[q].x;
         const uint64_t b = tg >> kPullShift;
-        kcnt[(b * D + k) * kPullTB + (tg & (kPullTB - 1)) + 1]++;
+        cnt[(b * D + k) * kPullTB + (tg & (kPullTB - 1))]++;
+        indeg[tg]++;
         smin[b] = std::min(smin[b], (uint32_t)id);
         smax[b] = std::max(smax[b], (uint32_t)id);
+        const int32_t w = (int32_t)L.syn[q].y;
+        wmax = std::max<int64_t>(wmax, w < 0 ? -(int64_t)w : (int64_t)w);
+        if (narrow && std::find(wset.begin(), wset.end(), w) == wset.end()) {
+          wset.push_back(w);
+          if (wset.size() > kPullMaxWeights) narrow = false;
+        }
       }
+  // the gather sums a target's weights exactly in FP64: |sum| < 2^53
+  uint32_t max_indeg = 0;
+  for (uint32_t c : indeg) max_indeg = std::max(max_indeg, c);
+  if ((double)max_indeg * (double)wmax >= 9007199254740992.0) return true;
   std::vector<uint32_t> lo(NB, 0);
-  uint32_t words = 0;
+  uint32_t words = 0, max_rel = 0;
   for (uint64_t b = 0; b < NB; ++b) {
     if (smin[b] > smax[b]) continue;  // no synapses into this block
     lo[b] = smin[b] >> 6;
     words = std::max(words, (smax[b] >> 6) - (smin[b] >> 6) + 1);
+    max_rel = std::max(max_rel, smax[b] - lo[b] * 64);
   }
   if (words > kPullMaxWords) return true;  // source spans too wide to stage: push only
-  std::vector<uint32_t> off(NB * D + 1, 0), kpos(nkeys);
+  // narrow entries: (source - block base) << wb | weight index
+  uint32_t wb = 0;
+  if (narrow) {
+    while ((1ull << wb) < wset.size() + 1) ++wb;  // index 0 is the inert weight 0
+    if (wb == 0) wb = 1;
+    if ((uint64_t)max_rel >= (1ull << (32 - wb))) narrow = false;
+  }
+  const uint32_t V = narrow ? 4 : 2;  // entries per 16-byte vector
+  // target order per segment and slice offsets
+  std::vector<uint16_t> perm(nseg * kPullTB);
+  std::vector<uint32_t> off(nseg * kPullSlices + 1, 0);
+  std::vector<uint32_t> pos_of(nseg * kPullTB);  // (segment, target) -> its lane's first entry
   uint64_t pos = 0;
-  for (uint64_t sg = 0; sg < NB * D; ++sg) {
-    off[sg] = (uint32_t)pos;
-    for (uint64_t j = 0; j < kPullTB; ++j) {
-      kpos[sg * kPullTB + j] = (uint32_t)pos;
-      pos += kcnt[sg * kPullTB + j + 1];
+  for (uint64_t sg = 0; sg < nseg; ++sg) {
+    uint16_t* pm = &perm[sg * kPullTB];
+    const uint32_t* c = &cnt[sg * kPullTB];
+    for (uint32_t i = 0; i < kPullTB; ++i) pm[i] = (uint16_t)i;
+    std::stable_sort(pm, pm + kPullTB, [&](uint16_t a, uint16_t b2) { return c[a] > c[b2]; });
+    for (uint32_t sl = 0; sl < kPullSlices; ++sl) {
+      off[sg * kPullSlices + sl] = (uint32_t)pos;
+      const uint32_t len = (c[pm[sl * 64]] + V - 1) / V * V;  // longest lane of the slice
+      for (uint32_t lane = 0; lane < 64; ++lane) pos_of[sg * kPullTB + pm[sl * 64 + lane]] = (uint32_t)(pos + lane * V);
+      pos += (uint64_t)len * 64;
+      if (pos > 0xffffffffull) return true;
     }
-    pos = (pos + kPullRun - 1) / kPullRun * kPullRun;  // pad to whole lane runs
-    if (pos > 0xffffffffull) return true;
   }
-  off[NB * D] = (uint32_t)pos;
-  std::vector<LayoutPair> ps(pos);
-  for (uint64_t sg = 0; sg < NB * D; ++sg) {
-    const uint64_t b = sg / D;
-    for (uint32_t e = off[sg]; e < off[sg + 1]; ++e) ps[e] = LayoutPair{(lo[b] * 64u) << kPullShift, 0u};
+  off[nseg * kPullSlices] = (uint32_t)pos;
+  // entry j of a lane sits at lane_first + (j / V) * 64 * V + j % V
+  std::vector<uint32_t> fill(nseg * kPullTB, 0);
+  auto place = [&](uint64_t key) {
+    const uint32_t j = fill[key]++;
+    return (uint64_t)pos_of[key] + (uint64_t)(j / V) * 64 * V + j % V;
+  };
+  if (narrow) {
+    std::sort(wset.begin(), wset.end());
+    std::vector<int32_t> wtab(1, 0);
+    wtab.insert(wtab.end(), wset.begin(), wset.end());
+    std::vector<uint32_t> ps(pos, 0u);  // padding: first staged source, weight index 0
+    for (uint64_t id = 0; id < T; ++id)
+      for (uint64_t k = 0; k < D; ++k)
+        for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
+          const uint32_t tg = L.syn[q].x;
+          const uint64_t b = tg >> kPullShift;
+          const uint32_t wi = (uint32_t)(std::lower_bound(wtab.begin() + 1, wtab.end(), (int32_t)L.syn[q].y) -
+                                         wtab.begin());
+          ps[place((b * D + k) * kPullTB + (tg & (kPullTB - 1)))] = (((uint32_t)id - lo[b] * 64) << wb) | wi;
+        }
+    L.pull_syn32.swap(ps);
+    L.pull_wtab.swap(wtab);
+    L.pull_wbits = wb;
+  } else {
+    std::vector<LayoutPair> ps(pos, LayoutPair{0u, 0u});  // padding: first staged source, weight 0
+    for (uint64_t id = 0; id < T; ++id)
+      for (uint64_t k = 0; k < D; ++k)
+        for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
+          const uint32_t tg = L.syn[q].x;
+          const uint64_t b = tg >> kPullShift;
+          ps[place((b * D + k) * kPullTB + (tg & (kPullTB - 1)))] = LayoutPair{(uint32_t)id - lo[b] * 64, L.syn[q].y};
+        }
+    L.pull_syn.swap(ps);
   }
-  for (uint64_t id = 0; id < T; ++id)
-    for (uint64_t k = 0; k < D; ++k)
-      for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
-        const uint32_t tg = L.syn[q].x;
-        const uint64_t key = ((uint64_t)(tg >> kPullShift) * D + k) * kPullTB + (tg & (kPullTB - 1));
-        ps[kpos[key]++] = LayoutPair{((uint32_t)id << kPullShift) | (tg & (kPullTB - 1)), L.syn[q].y};
-      }
-  // interleave every chunk: logical pair q = lane*(kPullRun/2) + j goes to j*nl + lane
-  const uint32_t ppl = kPullRun / 2;  // pairs per lane
-  std::vector<LayoutPair> tmp(64 * kPullRun);
-  for (uint64_t sg = 0; sg < NB * D; ++sg)
-    for (uint64_t cs = off[sg]; cs < off[sg + 1]; cs += 64 * kPullRun) {
-      const uint64_t m = std::min<uint64_t>(64 * kPullRun, off[sg + 1] - cs);
-      const uint64_t nl = m / kPullRun;
-      std::copy(ps.begin() + cs, ps.begin() + cs + m, tmp.begin());
-      for (uint64_t q = 0; q < m / 2; ++q) {
-        const uint64_t lane = q / ppl, j = q % ppl;
-        const uint64_t dst = cs + 2 * (j * nl + lane);
-        ps[dst] = tmp[2 * q];
-        ps[dst + 1] = tmp[2 * q + 1];
-      }
-    }
   L.pull_off.swap(off);
   L.pull_lo.swap(lo);
-  L.pull_syn.swap(ps);
+  L.pull_perm.swap(perm);
   L.pull_words = words;
   return true;
 }

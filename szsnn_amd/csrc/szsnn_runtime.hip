@@ -439,10 +439,16 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint64_t* hist = nullptr;
   uint32_t *d_pull_off = nullptr, *d_pull_lo = nullptr;
   uint2* d_pull_syn = nullptr;
+  uint32_t* d_pull_syn32 = nullptr;
+  int32_t* d_pull_wtab = nullptr;
+  uint16_t* d_pull_perm = nullptr;
   if (rc == SZSNN_OK) guard(dev_alloc(net, &hist, (size_t)n.R * n.W));
   if (rc == SZSNN_OK && !pull_off.empty())
-    guard(dev_alloc(net, &d_pull_off, pull_off.size())) && guard(dev_alloc(net, &d_pull_syn, pull_syn_size)) &&
-        guard(dev_alloc(net, &d_pull_lo, pull_lo.size())) &&
+    guard(dev_alloc(net, &d_pull_off, pull_off.size())) &&
+        (L.pull_wbits ? guard(dev_alloc(net, &d_pull_syn32, L.pull_syn32.size())) &&
+                            guard(dev_alloc(net, &d_pull_wtab, L.pull_wtab.size()))
+                      : guard(dev_alloc(net, &d_pull_syn, pull_syn_size))) &&
+        guard(dev_alloc(net, &d_pull_lo, pull_lo.size())) && guard(dev_alloc(net, &d_pull_perm, L.pull_perm.size())) &&
         guard(dev_alloc(net, &acc_pull, (size_t)groups * 2 * n.npad));
   if (rc != SZSNN_OK) {
     szsnn_net_destroy(net);
@@ -481,8 +487,11 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   if (rc == SZSNN_OK && drive_cdf) guard(upload(net, drive_cdf, d->drive_cdf, d->drive_k));
   if (rc == SZSNN_OK && d_pull_off)
     guard(upload(net, d_pull_off, pull_off.data(), pull_off.size())) &&
-        guard(upload(net, d_pull_syn, pull_syn_data, pull_syn_size)) &&
-        guard(upload(net, d_pull_lo, pull_lo.data(), pull_lo.size()));
+        (L.pull_wbits ? guard(upload(net, d_pull_syn32, L.pull_syn32.data(), L.pull_syn32.size())) &&
+                            guard(upload(net, d_pull_wtab, L.pull_wtab.data(), L.pull_wtab.size()))
+                      : guard(upload(net, d_pull_syn, pull_syn_data, pull_syn_size))) &&
+        guard(upload(net, d_pull_lo, pull_lo.data(), pull_lo.size())) &&
+        guard(upload(net, d_pull_perm, L.pull_perm.data(), L.pull_perm.size()));
   if (rc == SZSNN_OK) {
     hipStream_t s = ctx->stream;
     he = hipMemsetAsync(acc, 0, (size_t)n.R * 2 * n.npad * 8, s);
@@ -530,7 +539,12 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.forced_bits = nullptr;
   n.hist = hist;
   n.pull_off = d_pull_off;
+  n.pull_perm = d_pull_perm;
   n.pull_syn = d_pull_syn;
+  n.pull_syn32 = d_pull_syn32;
+  n.pull_wtab = d_pull_wtab;
+  n.pull_wbits = L.pull_wbits;
+  n.pull_wn = (uint32_t)L.pull_wtab.size();
   n.pull_lo = d_pull_lo;
   n.pull_nb = d_pull_off ? (uint32_t)NB : 0;
   n.pull_words = pull_words;
@@ -546,6 +560,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   net->info.n_syn_stored = S;
   net->info.n_rows = nrows;
   net->info.pull_groups = d_pull_off ? groups : 0;
+  net->info.pull_entry_bytes = d_pull_off ? (L.pull_wbits ? 4 : 8) : 0;
   net->info.raster_words = n.W;
   net->tiling.gather_grid = default_gather_grid(net, net->tiling.gather_block);
   *out = net;

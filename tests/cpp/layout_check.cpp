@@ -8,6 +8,7 @@
 // deferred to the pull must deliver, at step t, the naive sum over the
 // synapse list.  Exit code 0 and "layout_check OK" on success; any failure
 // aborts with a message.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -34,16 +35,22 @@ struct Net {
   std::vector<uint16_t> dl;
 };
 
-static Net random_net(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, uint64_t S, bool block_gaps) {
+// nweights > 0: weights drawn from that many values (a narrow pull layout
+// is possible); 0: continuous weights (wide entries)
+static Net random_net(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, uint64_t S, bool block_gaps,
+                      uint32_t nweights) {
   Net n{N, M, D, {}, {}, {}, {}};
   std::uniform_int_distribution<uint32_t> us(0, N + M - 1), ut(0, N - 1), ud(0, D - 1);
   std::uniform_real_distribution<double> uw(-1.0, 1.0);
+  std::vector<double> table;
+  for (uint32_t i = 0; i < nweights; ++i) table.push_back(uw(rng));
+  std::uniform_int_distribution<uint32_t> ui(0, nweights ? nweights - 1 : 0);
   for (uint64_t s = 0; s < S; ++s) {
     uint32_t t = ut(rng);
     if (block_gaps && ((t >> kPullShift) % 3 == 1)) continue;  // leave some target blocks empty
     n.src.push_back(us(rng));
     n.tgt.push_back(t);
-    double x = uw(rng);
+    double x = nweights ? table[ui(rng)] : uw(rng);
     if (s % 17 == 0) x = 0.0;  // inert
     n.w.push_back((float)std::ldexp(std::nearbyint(std::ldexp(x, 20)), -20));
     n.dl.push_back((uint16_t)ud(rng));
@@ -51,7 +58,7 @@ static Net random_net(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, 
   return n;
 }
 
-static szsnn_net_desc desc_of(const Net& n) {
+static szsnn_net_desc desc_of(const Net& n, int32_t pull_layout = 0) {
   szsnn_net_desc d{};
   d.n_neurons = n.N;
   d.n_sources = n.M;
@@ -62,7 +69,7 @@ static szsnn_net_desc desc_of(const Net& n) {
   d.syn_delay = n.dl.data();
   d.max_delay = n.D;
   d.frac_bits = 20;
-  d.pull_layout = 0;
+  d.pull_layout = pull_layout;
   return d;
 }
 
@@ -123,16 +130,25 @@ static void push_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
 
 // pull replay for step t with history bitmaps hist[R][W] and due mask `due`
 // (bit k: step t-k-1 was deferred); G delay groups, each writing its slot of
-// `out` ([G][2][N]); the caller sums the slots
+// `out` ([G][2][N]); the caller sums the slots.  Follows pull_body's indexing:
+// per (block, delay) segment kPullSlices slices, lane L of slice s serving
+// target pull_perm[seg*kPullTB + s*64 + L], its vector j at 16-byte index
+// s0/V + 64*j + L; FP64 sums as the kernel forms them.
 static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t t, uint64_t due,
                         const std::vector<uint64_t>& hist, uint32_t R, uint32_t W, uint32_t G,
                         std::vector<long long>& out, uint64_t& events) {
   const uint32_t words = L.pull_words;
   std::vector<uint64_t> bits(words);
-  std::vector<long long> acc(2 * kPullTB);
+  std::vector<double> sums(2 * kPullTB);
+  const bool narrow = L.pull_wbits != 0;
+  const uint32_t V = narrow ? 4 : 2, wb = L.pull_wbits;
+  const size_t nent = narrow ? L.pull_syn32.size() : L.pull_syn.size();
+  CHECK(L.pull_off.size() == (size_t)L.NB * D * kPullSlices + 1, "pull_off size");
+  CHECK(L.pull_perm.size() == (size_t)L.NB * D * kPullTB, "pull_perm size");
+  CHECK(L.pull_off.back() == nent, "entries %zu vs offsets %u", nent, L.pull_off.back());
   for (uint32_t bg = 0; bg < L.NB * G; ++bg) {
     const uint32_t b = bg / G, g = bg % G;
-    std::fill(acc.begin(), acc.end(), 0);
+    std::fill(sums.begin(), sums.end(), 0.0);
     CHECK(b < L.pull_lo.size(), "pull_lo b=%u", b);
     const uint32_t lo = L.pull_lo[b];
     CHECK(lo <= W, "lo=%u W=%u", lo, W);
@@ -142,40 +158,57 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
     for (uint32_t k = k0; k < k1; ++k) {
       if (!((due >> k) & 1ull)) continue;
       CHECK(t >= (uint64_t)k + 1, "due delay before the first step");
-      CHECK((uint64_t)b * D + k + 1 < L.pull_off.size(), "pull_off");
-      const uint32_t s0 = L.pull_off[b * D + k], s1 = L.pull_off[b * D + k + 1];
-      CHECK(s0 <= s1 && s1 <= L.pull_syn.size(), "segment");
-      CHECK(((s1 - s0) % kPullRun) == 0 && (s0 % kPullRun) == 0, "segment not made of whole lane runs");
-      if (s0 == s1) continue;
+      const size_t seg = (size_t)b * D + k;
+      const uint32_t* so = &L.pull_off[seg * kPullSlices];
+      // the slices' lanes serve every target of the block once
+      std::vector<uint8_t> seen(kPullTB, 0);
+      for (uint32_t i = 0; i < kPullTB; ++i) {
+        const uint16_t tl = L.pull_perm[seg * kPullTB + i];
+        CHECK(tl < kPullTB && !seen[tl], "perm of segment %zu is not a permutation", seg);
+        seen[tl] = 1;
+      }
+      if (so[0] == so[kPullSlices]) continue;
       const uint64_t hbase = ((t - k - 1) % R) * W + lo;
       for (uint32_t w = 0; w < nw; ++w) {
         CHECK(hbase + w < hist.size(), "hist read");
         bits[w] = hist[hbase + w];
       }
-      const uint32_t base_id = lo * 64;
-      const uint32_t kChunk = 64 * kPullRun;
-      for (uint32_t cs = s0; cs < s1; cs += kChunk) {  // every wave's chunk, every lane's run
-        const uint32_t nl = std::min(kChunk, s1 - cs) / kPullRun;
-        CHECK(nl >= 1 && nl <= 64, "lanes per chunk");
-        for (uint32_t lane = 0; lane < nl; ++lane)
-        for (uint32_t jj = 0; jj < kPullRun; ++jj) {
-          const uint32_t pair = (jj / 2) * nl + lane;
-          CHECK(pair < std::min(kChunk, s1 - cs) / 2, "pair index beyond the chunk");
-          const uint32_t e = cs + 2 * pair + (jj & 1);
-          CHECK(e < s1, "entry beyond the segment");
-          const LayoutPair p = L.pull_syn[e];
-          const uint32_t tl = p.x & (kPullTB - 1);
-          const uint32_t id = p.x >> kPullShift;
-          CHECK(id < T, "source id %u", id);
-          CHECK(id >= base_id, "source below staged range");
-          const uint32_t rel = id - base_id;
-          CHECK((rel >> 6) < nw, "LDS bit read rel=%u nw=%u", rel, nw);
-          const int32_t w = (int32_t)p.y;
-          if (((bits[rel >> 6] >> (rel & 63)) & 1ull) && w != 0) {
-            CHECK(b * kPullTB + tl < N, "pull target beyond N");
-            acc[(w >= 0 ? 0 : kPullTB) + tl] += w;
-            ++events;
-          }
+      for (uint32_t sl = 0; sl < kPullSlices; ++sl) {
+        const uint32_t s0 = so[sl], s1 = so[sl + 1];
+        CHECK(s0 <= s1 && s1 <= nent, "slice bounds");
+        CHECK(s0 % V == 0 && (s1 - s0) % (64 * V) == 0, "slice not made of whole vectors per lane");
+        if (s0 == s1) continue;
+        const uint32_t nv = (s1 - s0) / (64 * V);
+        for (uint32_t lane = 0; lane < 64; ++lane) {
+          const uint32_t tl = L.pull_perm[seg * kPullTB + sl * 64 + lane];
+          double ae = 0.0, ai = 0.0;
+          for (uint32_t j = 0; j < nv; ++j)
+            for (uint32_t c = 0; c < V; ++c) {
+              const uint64_t e = (uint64_t)(s0 / V + 64 * j + lane) * V + c;
+              CHECK(e < s1, "entry beyond the slice");
+              uint32_t rel;
+              int32_t w;
+              if (narrow) {
+                const uint32_t x = L.pull_syn32[e];
+                rel = x >> wb;
+                const uint32_t wi = x & ((1u << wb) - 1u);
+                CHECK(wi < L.pull_wtab.size(), "weight index %u of %zu", wi, L.pull_wtab.size());
+                w = L.pull_wtab[wi];
+              } else {
+                rel = L.pull_syn[e].x;
+                w = (int32_t)L.pull_syn[e].y;
+              }
+              CHECK((rel >> 6) < nw, "LDS bit read rel=%u nw=%u", rel, nw);
+              CHECK((uint64_t)lo * 64 + rel < T, "source id %llu", (unsigned long long)lo * 64 + rel);
+              if ((bits[rel >> 6] >> (rel & 63)) & 1ull) {
+                if (w > 0) ae += (double)w;
+                if (w < 0) ai += (double)w;
+                if (w != 0) ++events;
+              }
+            }
+          if (b * kPullTB + tl >= N) CHECK(ae == 0.0 && ai == 0.0, "sum for a target beyond N");
+          sums[tl] += ae;
+          sums[kPullTB + tl] += ai;
         }
       }
     }
@@ -183,18 +216,17 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
       const uint32_t tgt = b * kPullTB + i;
       if (tgt < N) {
         CHECK((size_t)g * 2 * N + N + tgt < out.size(), "slot write");
-        out[(size_t)g * 2 * N + tgt] = acc[i];
-        out[(size_t)g * 2 * N + N + tgt] = acc[kPullTB + i];
-      } else {
-        CHECK(acc[i] == 0 && acc[kPullTB + i] == 0, "sum for a target beyond N");
+        out[(size_t)g * 2 * N + tgt] = (long long)sums[i];
+        out[(size_t)g * 2 * N + N + tgt] = (long long)sums[kPullTB + i];
       }
     }
   }
 }
 
-static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, uint64_t S, bool gaps) {
-  Net net = random_net(rng, N, M, D, S, gaps);
-  const szsnn_net_desc d = desc_of(net);
+static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, uint64_t S, bool gaps,
+                       uint32_t nweights, int32_t pull_layout) {
+  Net net = random_net(rng, N, M, D, S, gaps, nweights);
+  const szsnn_net_desc d = desc_of(net, pull_layout);
   HostLayout L;
   CHECK(build_layout(d, L), "build_layout: %s", L.error.c_str());
   const uint32_t T = N + M, W = (T + 63) / 64, R = D + 1;
@@ -215,7 +247,18 @@ static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D,
   CHECK(active == L.active, "active ids");
   const bool pull = !L.pull_off.empty();
   CHECK(pull, "pull layout expected for N=%u", N);
-  CHECK(L.pull_off.size() == L.NB * D + 1, "pull_off size");
+  // narrow entries exactly when allowed and asked for (these nets' sources fit)
+  std::vector<long long> distinct;
+  for (float w : net.w) {
+    const long long q = (long long)std::nearbyint(std::ldexp((double)w, d.frac_bits));
+    if (q != 0) distinct.push_back(q);
+  }
+  std::sort(distinct.begin(), distinct.end());
+  distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+  const bool narrow_expected = pull_layout == 0 && distinct.size() <= kPullMaxWeights;
+  CHECK((L.pull_wbits != 0) == narrow_expected, "narrow=%u expected %d", L.pull_wbits, (int)narrow_expected);
+  CHECK(L.pull_wbits == 0 || (L.pull_wtab.size() >= 2 && L.pull_wtab[0] == 0 &&
+                              L.pull_wtab.size() <= (1u << L.pull_wbits)), "weight table");
   // random spike history up to step t - 1
   const uint64_t t = 3 * D + 2;
   std::vector<uint64_t> hist((size_t)R * W);
@@ -277,7 +320,12 @@ int main() {
   const C cases[] = {{1, 0, 1, 5, false},     {63, 2, 5, 400, false},   {64, 0, 3, 500, false},
                      {255, 7, 64, 3000, false}, {256, 0, 2, 2000, true}, {257, 31, 9, 4000, false},
                      {1000, 100, 32, 30000, true}, {5000, 10, 16, 60000, false}, {3000, 500, 1, 20000, true}};
-  for (const C& c : cases) check_case(rng, c.N, c.M, c.D, c.S, c.gaps);
+  for (const C& c : cases) {
+    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 0, 0);   // continuous weights: wide unless few synapses
+    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 0);   // three weights: narrow entries
+    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 200, 0); // 200 weights: narrow, 8 index bits
+    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 1);   // wide entries asked for
+  }
   // a network whose sources span more than the LDS budget gets no pull layout
   {
     Net n{200, 800000, 1, {0, 799999}, {0, 1}, {0.5f, -0.5f}, {0, 0}};

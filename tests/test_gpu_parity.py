@@ -370,3 +370,37 @@ def test_persistent_mode_pull_volley_equals_graph(ctx):
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
     assert runs[0][2]["events"] == runs[1][2]["events"] > 0
+
+
+@pytest.mark.parametrize("which", ["count", "table"])
+def test_narrow_and_wide_pull_entries_agree(ctx, which):
+    """The same network pulled through 4-byte entries (weight table: few
+    weights counted per index, or up to 255 looked up) and through 8-byte
+    entries: rasters, state and event counts identical to pushing every step;
+    the pulled table network also matches the FP32 oracle."""
+    import dataclasses
+    from szsnn_amd import configs
+    from szsnn_amd.simulation import Simulation
+    if which == "count":
+        net, steps, volley = configs.if_cond_exp_100k(n_exc=4000, n_inh=1000, k_exc=200, k_inh=50), 70, 1500
+    else:
+        net, steps, volley = configs.izhikevich2003(), 300, 0
+        w = net.syn_w.copy()
+        net = dataclasses.replace(net, syn_w=(np.round(w * 25.0) / 25.0).astype(np.float32))
+        assert 20 < np.unique(net.syn_w[net.syn_w != 0]).size <= 255
+    print(f"case: narrow vs wide pull entries ({which})", flush=True)
+    for layout, width in ((0, 4), (1, 8)):
+        s = Simulation(ctx, dataclasses.replace(net, pull_layout=layout))
+        assert s.info.pull_entry_bytes == width
+        del s
+    rp, vp, cp = _run_policy(ctx, net, steps, 2.0, volley)                                    # push only
+    r4, v4, c4 = _run_policy(ctx, net, steps, 0.0, volley)                                    # pull, 4 B
+    r8, v8, c8 = _run_policy(ctx, dataclasses.replace(net, pull_layout=1), steps, 0.0, volley)  # pull, 8 B
+    for r, v, c in ((r4, v4, c4), (r8, v8, c8)):
+        np.testing.assert_array_equal(r, rp)
+        np.testing.assert_array_equal(v, vp)
+        assert c["events"] == cp["events"] > 0
+    if volley == 0:
+        o, ro, _ = _oracle_run(net, steps, 32)
+        np.testing.assert_array_equal(r4, ro)
+        assert c4["events"] == o.events
