@@ -10,7 +10,10 @@ Prints one JSON line (the driver's contract) with:
                  timed; algorithmic bytes = 8 B per delivered synapse (f32
                  weight + u32 target) + 8 B per delivered (source, delay)
                  row; `traffic` = HBM bytes per launch from the rocprofv3
-                 FETCH_SIZE / WRITE_SIZE passes recorded in profiles/
+                 FETCH_SIZE / WRITE_SIZE passes recorded in profiles/ (the
+                 pull layout stores 4-byte entries when the network has few
+                 distinct weights, so traffic is below the algorithmic
+                 bytes; hbm_gbs is traffic / launch time)
   cpu_baseline   the FP32 CPU restatement (one thread) on a bounded sample
 """
 import argparse
@@ -43,6 +46,9 @@ def parse():
     ap.add_argument("--profile-pass", action="store_true",
                     help="profiler pass: a 1 GiB device copy (calibration), the volley's delay slots filled by "
                          "one persistent launch, then exactly --volley-steps gather_kernel launches")
+    ap.add_argument("--natural-pass", action="store_true",
+                    help="profiler pass: natural dynamics, --warmup steps then --steps steps as ONE persistent "
+                         "launch (block 512, grid 256), then --volley-steps single-step gather+update launches")
     ap.add_argument("--gather-block", type=int, default=256, help="profile pass: pull gather block size")
     ap.add_argument("--gather-grid", type=int, default=4096, help="profile pass: pull gather grid")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_pull_volley_traffic.json"),
@@ -250,6 +256,23 @@ def main():
                           "events_per_launch": ev, "rows_per_launch": rw, "alg_bytes_per_launch": per,
                           "gbs": per / (g_ms * 1e-3 / n) / 1e9}), flush=True)
         return
+    if args.natural_pass:
+        sim.set_mode(1, 512, 256)
+        sim.run(args.warmup)
+        ctx.synchronize()
+        c0 = sim.counters()
+        t1 = time.perf_counter()
+        sim.run(args.steps)
+        ctx.synchronize()
+        wall = time.perf_counter() - t1
+        c1 = sim.counters()
+        g_ms, u_ms = sim.time_kernels(args.volley_steps)
+        print(json.dumps({"natural_pass": True, "steps": args.steps, "us_per_step": wall * 1e6 / args.steps,
+                          "events_per_step": (c1["events"] - c0["events"]) / args.steps,
+                          "spikes_per_step": (c1["spikes"] - c0["spikes"]) / args.steps,
+                          "single_step_gather_us": g_ms * 1e3 / args.volley_steps,
+                          "single_step_update_us": u_ms * 1e3 / args.volley_steps}), flush=True)
+        return
     if not args.no_autotune:
         log(f"[bench] autotuned for the natural dynamics: {sim.autotune(200)}")
     # warmup (also builds the step graph)
@@ -295,6 +318,7 @@ def main():
 
     default_policy = sim.gather_policy()["dense_fraction"]
     groups = int(sim.info.pull_groups)
+    pull_entry_bytes = int(sim.info.pull_entry_bytes)
 
     def volley(fraction):
         sim.set_gather_policy(fraction)
@@ -320,9 +344,7 @@ def main():
     v_auto = volley(default_policy)
     per = gather_alg_bytes(v_auto["events_per_launch"], v_auto["rows_per_launch"])
     v_auto["alg_bytes_per_launch"] = per
-    # what the pull layout moves besides the entries: one 16-byte E/I sum per
-    # target and delay group, written once per launch
-    v_auto["pull_bytes_per_launch"] = 8 * v_auto["events_per_launch"] + 16 * net.n_neurons * groups
+    v_auto["pull_groups"] = groups
     v_auto["gbs"] = per / (v_auto["gather_us"] * 1e-6) / 1e9
     log(f"[bench] volley pull (policy {default_policy}): {v_auto}")
     sim.set_gather_policy(default_policy)
@@ -384,6 +406,9 @@ def main():
                      "traffic_detail": traffic,
                      "kernel": "gather_kernel pull (volley: all 100k neurons forced each step)",
                      "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6,
+                     "pull_entry_bytes": int(pull_entry_bytes),
+                     "hbm_gbs": traffic["hbm_bytes_per_launch"] / g_avg_s / 1e9 if traffic else None,
+                     "hbm_frac": traffic["hbm_bytes_per_launch"] / g_avg_s / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "batched": batched,
