@@ -84,6 +84,17 @@ double oracle_time_steps(void* p, uint64_t steps) {
   return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// Wall time (seconds) of `steps` steps of the FP32 engine on `threads`
+// OpenMP threads (step_parallel); -1 for the FP64 engine.
+double oracle_time_steps_parallel(void* p, uint64_t steps, int threads) {
+  auto h = static_cast<Handle*>(p);
+  if (!h->f || threads < 1) return -1.0;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint64_t s = 0; s < steps; ++s) h->f->step_parallel(threads);
+  const auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
 uint64_t oracle_words(void* p) {
   auto h = static_cast<Handle*>(p);
   return h->f ? h->f->words() : h->d->words();

@@ -55,6 +55,8 @@ def load():
                                c_f64p, ctypes.c_uint32]
     lib.oracle_time_steps.restype = ctypes.c_double
     lib.oracle_time_steps.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    lib.oracle_time_steps_parallel.restype = ctypes.c_double
+    lib.oracle_time_steps_parallel.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
     for f in ("oracle_words", "oracle_events", "oracle_drive_events"):
         getattr(lib, f).restype = ctypes.c_uint64
         getattr(lib, f).argtypes = [ctypes.c_void_p]
@@ -149,8 +151,15 @@ class Oracle:
             raise RuntimeError("oracle: " + self.lib.oracle_last_error().decode())
         return raster, v
 
-    def time_steps(self, steps):
-        return self.lib.oracle_time_steps(self.h, steps)
+    def time_steps(self, steps, threads=1):
+        """Wall seconds of `steps` steps (no forced input): the serial step,
+        or, with threads > 1, the FP32 engine's parallel step."""
+        if threads <= 1:
+            return self.lib.oracle_time_steps(self.h, steps)
+        secs = self.lib.oracle_time_steps_parallel(self.h, steps, int(threads))
+        if secs < 0:
+            raise RuntimeError("oracle: the parallel step needs the FP32 engine")
+        return secs
 
     def vars(self):
         out = np.zeros((self.nv, self.n), np.float64)

@@ -51,6 +51,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <omp.h>
+
 namespace szsnn_oracle {
 
 // ---------------------------------------------------------------- random drive
@@ -314,6 +316,83 @@ class Engine {
       for (uint32_t id : spikes_) raster_row[id / 64] |= (uint64_t)1 << (id % 64);
     }
     ++t_;
+  }
+
+  // The same step on `threads` OpenMP threads, for timing the CPU baseline
+  // (no forced input, no outputs).  FP32 engine only: its accumulators are
+  // integers, so delivering the spikes with relaxed atomic adds in any order
+  // gives the serial step's sums exactly; neurons update independently and
+  // the spike list keeps ascending ids (each thread a contiguous range).
+  void step_parallel(int threads) {
+    if constexpr (!std::is_same<Acc, int64_t>::value) {
+      (void)threads;
+      throw std::runtime_error("step_parallel: FP32 engine only");
+    } else {
+    const size_t slot = (size_t)(t_ % RS_);
+    Acc* accE = &ring_[(slot * 2 + 0) * N_];
+    Acc* accI = &ring_[(slot * 2 + 1) * N_];
+    events_ += ring_events_[slot];
+    ring_events_[slot] = 0;
+    std::vector<std::vector<uint32_t>> tsp(threads);
+    std::vector<uint64_t> tdrive(threads, 0);
+#pragma omp parallel num_threads(threads)
+    {
+      const int th = omp_get_thread_num();
+      const uint32_t i0 = (uint32_t)((uint64_t)N_ * th / threads), i1 = (uint32_t)((uint64_t)N_ * (th + 1) / threads);
+      std::vector<R> p(np_), x(nv_);
+      uint64_t drive = 0;  // the thread's own count (one write at the end: no shared line per neuron)
+      for (uint32_t i = i0; i < i1; ++i) {
+        for (int k = 0; k < np_; ++k) p[k] = params_[(size_t)k * N_ + i];
+        for (int k = 0; k < nv_; ++k) x[k] = vars_[(size_t)k * N_ + i];
+        Acc ae = accE[i];
+        if (d_.drive_k) {
+          const uint64_t u = draw(keys_[i], t_, 2);
+          uint32_t kk = 0;
+          for (uint32_t j = 0; j < d_.drive_k; ++j) kk += u >= d_.drive_cdf[j] ? 1u : 0u;
+          ae += (Acc)kk * drive_acc_;
+          drive += kk;
+        }
+        const R e = from_acc(ae), in = from_acc(accI[i]);
+        const R iinj = d_.noise_amp ? (R)d_.noise_amp[i] * gauss<R>(keys_[i], t_) : R(0);
+        const bool sp = update_neuron<R>(d_.model, p.data(), x.data(), e, in, iinj, false);
+        for (int k = 0; k < nv_; ++k) vars_[(size_t)k * N_ + i] = x[k];
+        if (sp) tsp[th].push_back(i);
+        accE[i] = Acc(0);
+        accI[i] = Acc(0);
+      }
+      tdrive[th] = drive;
+    }
+    spikes_.clear();
+    for (int th = 0; th < threads; ++th) {
+      spikes_.insert(spikes_.end(), tsp[th].begin(), tsp[th].end());
+      drive_events_ += tdrive[th];
+    }
+    for (uint32_t j = 0; j < M_; ++j) {
+      const uint32_t id = N_ + j;
+      bool sp = false;
+      if (d_.source_model == SRC_CONST_RATE) {
+        R last = src_last_[j] + R(1);
+        sp = last == (R)d_.src_rate[j];
+        if (sp) last = R(0);
+        src_last_[j] = last;
+      } else if (d_.source_model == SRC_POISSON) {
+        sp = draw(keys_[id], t_, 3) < d_.src_thresh[j];
+      }
+      if (sp) spikes_.push_back(id);
+    }
+    const int64_t ns = (int64_t)spikes_.size();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4)
+    for (int64_t si = 0; si < ns; ++si) {
+      const uint32_t id = spikes_[si];
+      for (uint64_t q = off_[id]; q < off_[id + 1]; ++q) {
+        const size_t sl = (size_t)((t_ + dl_[q] + 1) % RS_);
+        const int ch = w_[q] < 0.0f ? 1 : 0;
+        __atomic_fetch_add(&ring_[(sl * 2 + ch) * N_ + tgt_[q]], wq_[q], __ATOMIC_RELAXED);
+        __atomic_fetch_add(&ring_events_[sl], (uint64_t)1, __ATOMIC_RELAXED);
+      }
+    }
+    ++t_;
+    }
   }
 
  private:

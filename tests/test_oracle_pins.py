@@ -273,3 +273,24 @@ def test_binomial_thresholds_and_rate(oracle_mod):
     for j, t in enumerate(cdf):
         assert abs(int(t) / 2.0 ** 64 - binom.cdf(j, 100, 5e-4)) < 1e-15
     assert np.all(np.diff(cdf.astype(np.float64)) >= 0)
+
+
+@pytest.mark.parametrize("which", ["cond", "curr", "izh"])
+def test_parallel_step_equals_serial(oracle_mod, which):
+    """The FP32 engine's parallel step (the CPU baseline's multi-core timing)
+    ends in the serial step's exact state: integer accumulation is
+    order-free, so atomic delivery from many threads changes no bit."""
+    from szsnn_amd import configs
+    if which == "cond":
+        net = configs.if_cond_exp_100k(n_exc=4000, n_inh=1000, k_exc=200, k_inh=50)
+    elif which == "curr":
+        net = configs.if_curr_exp_10k(n_exc=2000, n_inh=500, n_poisson=300, poisson_hz=100.0, w_poisson=1.5)
+    else:
+        net = configs.izhikevich2003()
+    serial = oracle_mod.Oracle(net, 32)
+    parallel = oracle_mod.Oracle(net, 32)
+    serial.time_steps(300)
+    parallel.time_steps(300, threads=4)
+    np.testing.assert_array_equal(serial.vars(), parallel.vars())
+    assert serial.events == parallel.events > 0
+    assert serial.drive_events == parallel.drive_events
