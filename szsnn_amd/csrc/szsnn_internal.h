@@ -136,7 +136,7 @@ constexpr unsigned long long kBarTimeout = 1ull << 32;
 // blocks adding to one counter do not serialise on a single address; the host
 // sums the stripes.  counters = [kCntSlots][kCntStripes][16] u64.  Events and
 // rows are counted when delivered (at the step that consumes them).
-constexpr uint32_t kCntStripes = 64;
+constexpr uint32_t kCntStripes = 256;
 constexpr uint32_t kCntStride = 16;
 constexpr size_t kCntWords = (size_t)kCntSlots * kCntStripes * kCntStride;
 __host__ __device__ inline unsigned long long* counter(unsigned long long* base, int slot, uint32_t stripe) {
@@ -145,7 +145,7 @@ __host__ __device__ inline unsigned long long* counter(unsigned long long* base,
 
 // Delivery-time counter rings: which = 0 events, 1 rows; each entry on its own
 // 128-byte line.
-constexpr uint32_t kRingStripes = 16;
+constexpr uint32_t kRingStripes = 64;
 constexpr size_t kRingCntWords(uint32_t R) { return (size_t)R * 2 * kRingStripes * kCntStride; }
 constexpr size_t kSpkRingWords = (size_t)4 * kRingStripes * kCntStride;
 __host__ __device__ inline unsigned long long* ring_counter(unsigned long long* base, uint32_t slot, uint32_t which,

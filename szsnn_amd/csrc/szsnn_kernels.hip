@@ -134,6 +134,17 @@ __device__ inline bool izh_update(NeuronRegs<kIzhikevich>& r, float accE, float 
 // (expcnt 7 and lgkmcnt 15 mean no wait on those counters).
 constexpr int kWaitVmcnt0 = 0x0F70;
 
+// Load of state that other blocks write: inside the persistent loop (AGENT)
+// written by the same launch, so agent-coherent; in a single-step launch
+// written by earlier launches (a kernel boundary makes them visible), so a
+// plain, cacheable load (many blocks reading the same words then hit in L2
+// instead of all going to memory).
+template <bool AGENT, class T>
+__device__ inline T ld_step(const T* p) {
+  if constexpr (AGENT) return ld_agent(p);
+  else return *p;
+}
+
 // x of lane j (j uniform across the wave).
 __device__ inline uint64_t readlane64(uint64_t x, uint32_t j) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)j);
@@ -172,13 +183,14 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, ui
 
 // Bit k set iff step t - k - 1 was deferred, i.e. its synapses of delay k
 // are pulled at t.  Every lane of the calling wave gets the mask.
+template <bool AGENT>
 __device__ inline uint64_t due_mask(const DevNet& n, uint64_t t) {
   if (!n.pull_off) return 0;
   const uint32_t k = __lane_id();
   bool due = false;
   if (k < n.D && t >= (uint64_t)k + 1) {
     const uint64_t s = t - k - 1;
-    due = ld_agent(&n.dstamp[s % n.R]) == s + 1;
+    due = ld_step<AGENT>(&n.dstamp[s % n.R]) == s + 1;
   }
   return __ballot(due);
 }
@@ -191,13 +203,12 @@ constexpr uint32_t kSparseIds = 8;
 constexpr uint32_t kPushBatch = 4;
 
 // Adds a thread's batch of loaded push entries into the accumulator ring and
-// counts them per delay: DIRECT, straight into the delivery-time counter ring
-// (few entries: these atomics drain together with the push's), else into the
-// block's LDS ev, flushed once per (block, delay).  Every loaded entry is
-// consumed before the first atomic, so the atomics issue back to back.
-// Consecutive lanes hold consecutive entries, sorted by delay within an id,
-// so one add per run of equal delays counts the events.
-template <uint32_t NB, bool DIRECT>
+// counts them per delay in the block's LDS ev (flushed once per (block,
+// delay)).  Every loaded entry is consumed before the first atomic, so the
+// atomics issue back to back.  Consecutive lanes hold consecutive entries,
+// sorted by delay within an id, so one LDS add per run of equal delays counts
+// the events.
+template <uint32_t NB>
 __device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[NB], uint32_t first_slot, uint32_t* ev) {
   const uint32_t lane = __lane_id();
   long long* dst[NB];
@@ -212,11 +223,7 @@ __device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[NB], uint
     if (ds >= n.R) ds -= n.R;
     if (head) {
       const uint64_t rest = lane == 63 ? 0ull : bound >> (lane + 1);
-      const uint32_t len = rest ? (uint32_t)__ffsll((unsigned long long)rest) : 64u - lane;
-      if (DIRECT)
-        atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)len);
-      else
-        atomicAdd(&ev[k], len);
+      atomicAdd(&ev[k], rest ? (uint32_t)__ffsll((unsigned long long)rest) : 64u - lane);
     }
     const int32_t w = (int32_t)sw[u].y;
     dst[u] = n.acc + (size_t)ds * 2 * n.npad + (w < 0 ? n.npad : 0u) + (sw[u].x & kSynTargetMask);
@@ -254,41 +261,44 @@ struct StepFacts {
 // Block-uniform facts of step t, gathered by one wave (the block's first)
 // with all its loads issued together: the pull's due mask (as due_mask)
 // into sm.due, and the spikes of step t - 1 into sm.spk_prev (LDS atomics of
-// the stripes, no wave reduction).  With `fold` (block 0), each of the
-// wave's first 2 * kRingStripes lanes also reads one stripe of the events /
-// rows due at t into `fold_val`; fold_finish moves them into the counters at
-// the end of the step.
-template <int BLK>
+// the stripes, no wave reduction).  With `fold` (block 0), lane s also reads
+// stripe s of the events and of the rows due at t into fold_e / fold_r;
+// fold_finish moves them into the counters at the end of the step.
+template <int BLK, bool AGENT>
 __device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bool fold, UpdateSmem<BLK>& sm,
-                                  unsigned long long& fold_val) {
+                                  unsigned long long& fold_e, unsigned long long& fold_r) {
+  static_assert(kRingStripes == 64, "one stripe per lane");
   const uint32_t lane = __lane_id();
   if (lane == 0) sm.spk_prev = 0;
   // unconditional loads at always-valid addresses (no divergent branch between
   // them, so they are in flight together); validity is applied afterwards
   const uint32_t k = lane < n.D ? lane : 0u;
-  const uint64_t stamp = n.pull_off ? ld_agent(&n.dstamp[(t - k - 1) % n.R]) : 0ull;
-  const unsigned long long spk = ld_agent(spk_counter(n.spk_ring, t - 1, lane));
-  const unsigned long long fc =
-      fold ? ld_agent(ring_counter(n.ring_cnt, slot, (lane / kRingStripes) & 1u, lane % kRingStripes)) : 0ull;
+  const uint64_t stamp = n.pull_off ? ld_step<AGENT>(&n.dstamp[(t - k - 1) % n.R]) : 0ull;
+  const unsigned long long spk = ld_step<AGENT>(spk_counter(n.spk_ring, t - 1, lane));
+  fold_e = fold ? ld_step<AGENT>(ring_counter(n.ring_cnt, slot, 0, lane)) : 0ull;
+  fold_r = fold ? ld_step<AGENT>(ring_counter(n.ring_cnt, slot, 1, lane)) : 0ull;
   const bool stamp_ok = n.pull_off && lane < n.D && t >= (uint64_t)lane + 1 && stamp == t - lane;
-  fold_val = lane < 2 * kRingStripes ? fc : 0ull;
   const uint64_t due = __ballot(stamp_ok);
   if (lane == 0) sm.due = due;
-  if (t > 0 && lane < kRingStripes && spk) atomicAdd(&sm.spk_prev, spk);
+  if (t > 0 && spk) atomicAdd(&sm.spk_prev, spk);
 }
 
 // Block 0, one wave, end of step t: add the events and rows due at t (read
 // by step_facts) to the counters, stripe by stripe, clear their ring
 // stripes, and clear the spike-count slot that step t + 1 will fill.  Done
 // last so that these writes do not hold up the step's own loads.
-__device__ inline void fold_finish(const DevNet& n, uint64_t t, uint32_t slot, unsigned long long fold_val) {
+__device__ inline void fold_finish(const DevNet& n, uint64_t t, uint32_t slot, unsigned long long fold_e,
+                                   unsigned long long fold_r) {
   const uint32_t lane = __lane_id();
-  if (fold_val) {
-    st_agent(ring_counter(n.ring_cnt, slot, lane / kRingStripes, lane % kRingStripes), 0ull);
-    atomicAdd(counter(n.counters, lane < kRingStripes ? kCntEvents : kCntRows, lane % kRingStripes), fold_val);
+  if (fold_e) {
+    st_agent(ring_counter(n.ring_cnt, slot, 0, lane), 0ull);
+    atomicAdd(counter(n.counters, kCntEvents, lane), fold_e);
   }
-  if (lane >= 2 * kRingStripes && lane < 3 * kRingStripes)
-    st_agent(spk_counter(n.spk_ring, t + 1, lane - 2 * kRingStripes), 0ull);
+  if (fold_r) {
+    st_agent(ring_counter(n.ring_cnt, slot, 1, lane), 0ull);
+    atomicAdd(counter(n.counters, kCntRows, lane), fold_r);
+  }
+  st_agent(spk_counter(n.spk_ring, t + 1, lane), 0ull);
 }
 
 // What a lane keeps about its id across steps: the neuron's parameters and
@@ -347,7 +357,7 @@ __device__ inline void store_lane(const DevNet& n, uint32_t id, const LaneState<
 // with one atomic per (block, delay).  CARRY: `ls` already holds the lane's
 // state and keeps it (the caller stores it at the end); otherwise it is
 // loaded and stored here, and the drive table is staged in LDS here.
-template <int MODEL, int BLK, bool CARRY>
+template <int MODEL, int BLK, bool CARRY, bool AGENT>
 __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm,
                                                  LaneState<MODEL>& ls, const StepFacts& facts,
                                                  unsigned long long& out_drive) {
@@ -358,15 +368,15 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
   uint64_t fbits = 0;
   long long* ring = n.acc + (size_t)slot * 2 * n.npad;
   if (id < n.N) {
-    ring_e = ld_agent(&ring[id]);
-    ring_i = ld_agent(&ring[n.npad + id]);
+    ring_e = ld_step<AGENT>(&ring[id]);
+    ring_i = ld_step<AGENT>(&ring[n.npad + id]);
   }
   if (!CARRY) {
     load_lane<MODEL>(n, id, ls);
     if (threadIdx.x < n.drive_k) sm.cdf[threadIdx.x] = n.drive_cdf[threadIdx.x];
   }
   if (id < n.T && n.forced_bits) fbits = n.forced_bits[id >> 6];
-  unsigned long long fold_val = 0;
+  unsigned long long fold_e = 0, fold_r = 0;
   if (threadIdx.x < 64) {
     sm.cnt[threadIdx.x] = 0;
     sm.ev[threadIdx.x] = 0;
@@ -374,7 +384,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       sm.spikes = sm.drive = 0;
       sm.nlist = 0;
     }
-    if (!CARRY) step_facts<BLK>(n, t, slot, blockIdx.x == 0, sm, fold_val);
+    if (!CARRY) step_facts<BLK, AGENT>(n, t, slot, blockIdx.x == 0, sm, fold_e, fold_r);
   }
   __syncthreads();
   const uint64_t due = CARRY ? facts.due : sm.due;
@@ -398,8 +408,8 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
         for (uint32_t g = 0; g < kMaxPullGroups; ++g) {
           pe[g] = pi[g] = 0;
           if (g < n.pull_groups) {
-            pe[g] = ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
-            pi[g] = ld_agent(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
+            pe[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
+            pi[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
           }
         }
 #pragma unroll
@@ -441,7 +451,6 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
         spike = spike || (rng_draw(ls.key, t, 3) < ls.src_thresh);
       }
     }
-    if (!CARRY) store_lane<MODEL>(n, id, ls);
   }
   // Every load of the step so far has been consumed.  Saying so explicitly
   // (vmcnt 0) keeps the compiler from waiting on the stores and atomics issued
@@ -464,18 +473,12 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     // rows per delay: the wave's spiking ids with a synapse of that delay
     if (__popcll(mask) <= 8) {
       // few spikes: lane k counts bit k of each spiking lane's delay bits
-      // and adds it straight into the delivery ring (issued now, it drains
-      // with the push)
       uint32_t c = 0;
       for (uint64_t m = mask; m; m &= m - 1) {
         const uint64_t b = readlane64(bits, (uint32_t)__ffsll((unsigned long long)m) - 1);
         c += (uint32_t)((b >> lane) & 1ull);
       }
-      if (c) {
-        uint32_t ds = slot + 1 + lane;
-        if (ds >= n.R) ds -= n.R;
-        atomicAdd(ring_counter(n.ring_cnt, ds, 1, blockIdx.x), (unsigned long long)c);
-      }
+      if (c) atomicAdd(&sm.cnt[lane], c);
     } else {
       uint64_t uni = bits;
 #pragma unroll
@@ -538,7 +541,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
             sw[u] = n.syn[src];
           }
         }
-        push_entries<kPushBatch, true>(n, sw, first_slot, sm.ev);
+        push_entries<kPushBatch>(n, sw, first_slot, sm.ev);
       }
     } else {
       // many ids: block-wide scan of the lanes' counts, binary search per entry
@@ -559,13 +562,13 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
             sw[u] = n.syn[sm.start[lo] + (e - sm.pref[lo])];
           }
         }
-        push_entries<kPushBatch, false>(n, sw, first_slot, sm.ev);
+        push_entries<kPushBatch>(n, sw, first_slot, sm.ev);
       }
     }
     __syncthreads();
   }
-  // counters the dense paths kept in LDS: one atomic per (block, delay) into
-  // the delivery ring
+  // rows and events per delay: one atomic per (block, delay) into the
+  // delivery ring
   if (threadIdx.x < n.D) {
     const uint32_t k = threadIdx.x;
     uint32_t ds = slot + 1 + k;
@@ -576,7 +579,8 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
   }
   const unsigned long long spikes = sm.spikes, drive = sm.drive;
   if (threadIdx.x == 0) {
-    if (spikes && defer) st_agent(&n.dstamp[slot], t + 1);
+    // the deferral stamp, by one block (every block takes the same decision)
+    if (defer && blockIdx.x == 0) st_agent(&n.dstamp[slot], t + 1);
     // CARRY: the caller sums the block's spikes and drive events over the
     // launch and adds them (and the last step's spike slot) at its end
     if (!CARRY) {
@@ -587,23 +591,25 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       if (drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), drive);
     }
   }
-  if (!CARRY && blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_val);
+  if (!CARRY && blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_e, fold_r);
+  // the new state, stored last (nothing in the step waits for these stores)
+  if (!CARRY && id < n.T) store_lane<MODEL>(n, id, ls);
   out_drive = drive;
   return spikes;
 }
 
-template <int MODEL, int BLK>
+template <int MODEL, int BLK, bool AGENT>
 __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
   LaneState<MODEL> ls;
   unsigned long long drive = 0;
-  update_step<MODEL, BLK, false>(n, t, id, sm, ls, StepFacts{0, 0}, drive);
+  update_step<MODEL, BLK, false, AGENT>(n, t, id, sm, ls, StepFacts{0, 0}, drive);
 }
 
 template <int MODEL, int BLK>
 __global__ void __launch_bounds__(BLK) update_kernel(DevNet n, uint32_t j) {
   __shared__ UpdateSmem<BLK> sm;
   const uint64_t t = *n.t_base + j;
-  update_block<MODEL, BLK>(n, t, blockIdx.x * BLK + threadIdx.x, sm);
+  update_block<MODEL, BLK, false>(n, t, blockIdx.x * BLK + threadIdx.x, sm);
 }
 
 // Component c (compile-time after unrolling) of a 16-byte load.
@@ -635,7 +641,7 @@ constexpr uint32_t kPullCountWeights = 4;  // 16-bit counters packed in a u64
 // dyn: pull_words u64 bitmap | 2*kPullTB f64 sums | narrow formats: 2 << wb
 // (E, I) f64 pairs indexed by (spike bit, weight index), zero for bit 0 and
 // for indices beyond the table.
-template <int BLK, int FMT>
+template <int BLK, int FMT, bool AGENT>
 __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
   constexpr bool kNarrow = FMT != kPullWide;
   constexpr uint32_t kV = kNarrow ? 4 : 2;  // entries per 16-byte vector
@@ -668,7 +674,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     if (so[0] == so[kPullSlices]) continue;  // no delay-k synapses into the block
     const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
     __syncthreads();  // the previous delay's readers are done with `bits`
-    for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = ld_agent(&src[w]);
+    for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = ld_step<AGENT>(&src[w]);
     __syncthreads();
     for (uint32_t sl = wave; sl < kPullSlices; sl += kWaves) {
       const uint32_t s0 = so[sl], s1 = so[sl + 1];
@@ -756,14 +762,14 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
 }
 
 // The pull of work item (b, g) in the network's entry format.
-template <int BLK>
+template <int BLK, bool AGENT>
 __device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
   if (!n.pull_wbits)
-    pull_body<BLK, kPullWide>(n, t, b, g, due, dyn);
+    pull_body<BLK, kPullWide, AGENT>(n, t, b, g, due, dyn);
   else if (n.pull_wn <= kPullCountWeights)
-    pull_body<BLK, kPullCount>(n, t, b, g, due, dyn);
+    pull_body<BLK, kPullCount, AGENT>(n, t, b, g, due, dyn);
   else
-    pull_body<BLK, kPullTable>(n, t, b, g, due, dyn);
+    pull_body<BLK, kPullTable, AGENT>(n, t, b, g, due, dyn);
 }
 
 // gather(t): the pull of the deferred steps whose synapses are due at t, one
@@ -773,10 +779,10 @@ template <int BLK>
 __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
   extern __shared__ uint64_t dyn[];
   const uint64_t t = *n.t_base + j;
-  const uint64_t due = due_mask(n, t);
+  const uint64_t due = due_mask<false>(n, t);
   if (!due) return;
   for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-    pull_item<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
+    pull_item<BLK, false>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
     __syncthreads();  // LDS reuse by the next work item
   }
 }
@@ -846,7 +852,7 @@ __device__ inline bool grid_barrier(uint32_t* bar, uint32_t* sh, unsigned long l
 // grid covers every id with one lane, each lane keeps its id's parameters and
 // state in registers for the whole launch and stores the state once at the
 // end, and the step facts are carried in registers: the due mask as a shift
-// register of deferral bits (step s deferred with spikes: the same decision
+// register of deferral bits (step s deferred: the same decision
 // in every block, from the barrier's spike total), so a step starts with no
 // loads but its accumulators.  Then the delivery-time counters are folded by
 // the first 2 * kRingStripes blocks (one stripe each: exchanged with 0 at the
@@ -874,7 +880,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       const unsigned long long spk = ld_agent(spk_counter(n.spk_ring, t0 - 1, lane));
       const uint64_t due = __ballot(n.pull_off && lane < n.D && t0 >= (uint64_t)lane + 1 && stamp == t0 - lane);
       if (lane == 0) f0.due = due;
-      if (t0 > 0 && lane < kRingStripes && spk) atomicAdd(&f0.spk_prev, spk);
+      if (t0 > 0 && spk) atomicAdd(&f0.spk_prev, spk);
     }
   }
   extern __shared__ uint64_t dyn[];
@@ -899,28 +905,28 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       fold_v = 0;
     }
     unsigned long long sum = 0;
-    const uint64_t due = carry ? f.due : due_mask(n, t);
+    const uint64_t due = carry ? f.due : due_mask<true>(n, t);
     if (due) {
       for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-        pull_item<BLK>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
+        pull_item<BLK, true>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
         __syncthreads();
       }
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
     }
     if (carry) {
       unsigned long long drive = 0;
-      const unsigned long long spikes = update_step<MODEL, BLK, true>(n, t, id, usm, ls, f, drive);
+      const unsigned long long spikes = update_step<MODEL, BLK, true, true>(n, t, id, usm, ls, f, drive);
       spikes_acc += spikes;
       drive_acc += drive;
       const uint32_t slot = (uint32_t)(t % n.R);
       if (folds) fold_v = atomicExch(ring_counter(n.ring_cnt, slot, q / kRingStripes, q % kRingStripes), 0ull);
       if (!grid_barrier(n.barrier, bar_sh, err, threadIdx.x == 0 ? spikes : 0ull, &sum)) return;
       const bool deferred = n.pull_off && f.spk_prev + n.volley + n.forced_count >= n.defer_spikes;
-      f.due = ((f.due << 1) | (deferred && sum ? 1ull : 0ull)) & dmask;
+      f.due = ((f.due << 1) | (deferred ? 1ull : 0ull)) & dmask;
       f.spk_prev = sum;
     } else {
       for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
-        update_block<MODEL, BLK>(n, t, base + threadIdx.x, usm);
+        update_block<MODEL, BLK, true>(n, t, base + threadIdx.x, usm);
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
     }
   }
