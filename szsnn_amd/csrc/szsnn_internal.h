@@ -83,12 +83,14 @@ struct DevNet {
   // pull layout (dense gather), a sliced ELLPACK: target blocks of kPullTB
   // neurons; segment (b, k) holds the delay-k synapses into block b in
   // kPullSlices slices of 64 targets, slice s at entries pull_off[(b*D+k)*
-  // kPullSlices + s] .. (next), lane L serving target pull_perm[(b*D+k)*
-  // kPullTB + s*64 + L] of the block; 16-byte vectors [L_s/V][64][V].  Wide
+  // kPullSlices + s] .. (next), lane L serving target pull_lane[(b*D+k)*
+  // kPullTB + s*64 + L] % kPullTB of the block, whose entries fill the first
+  // pull_lane[...] >> kPullShift vectors of its column; 16-byte vectors
+  // [L_s/V][64][V].  Wide
   // entries: (source - pull_lo[b]*64, weight); narrow: (source - pull_lo[b]*
   // 64) << wbits | weight index.  Padding entries: weight 0.
   const uint32_t* pull_off; // [NB*D*kPullSlices + 1] or null (no pull layout)
-  const uint16_t* pull_perm;  // [NB*D*kPullTB]
+  const uint32_t* pull_lane;  // [NB*D*kPullTB] target | vectors << kPullShift
   const uint2* pull_syn;    // wide entries, or null
   const uint32_t* pull_syn32;  // narrow entries, or null
   const int32_t* pull_wtab; // narrow: [pull_wn] weight of each index

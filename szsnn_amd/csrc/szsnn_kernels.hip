@@ -679,15 +679,17 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     for (uint32_t sl = wave; sl < kPullSlices; sl += kWaves) {
       const uint32_t s0 = so[sl], s1 = so[sl + 1];
       if (s0 == s1) continue;
-      const uint32_t nv = (s1 - s0) / (64 * kV);  // vectors per lane
-      const uint32_t tl = n.pull_perm[(size_t)seg * kPullTB + sl * 64 + lane];
+      const uint32_t nv = (s1 - s0) / (64 * kV);  // vectors per lane (the slice's longest)
+      const uint32_t ln = n.pull_lane[(size_t)seg * kPullTB + sl * 64 + lane];
+      const uint32_t tl = ln & (kPullTB - 1), mine = ln >> kPullShift;  // target, the lane's own vectors
       const uint4* p = vec + s0 / kV + lane;     // the lane's vector j at p[64 * j]
       double ae = 0.0, ai = 0.0;
       uint64_t cpk = 0;  // count format: 16-bit counts per weight index
-      // every load is unconditional (indices clamped to the lane's last
-      // vector; the repeats hit the same line), so the compiler can wait for
-      // the current batch alone while the next one stays in flight
-      const uint32_t last = nv - 1;
+      // every load is unconditional (indices clamped to the lane's last own
+      // vector, so its padding vectors are never read: the repeats hit the
+      // same line), so the compiler can wait for the current batch alone
+      // while the next one stays in flight
+      const uint32_t last = mine ? mine - 1 : 0u;
       uint4 q[kB];
 #pragma unroll
       for (uint32_t u = 0; u < kB; ++u) q[u] = p[64 * min(u, last)];
@@ -697,7 +699,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
         for (uint32_t u = 0; u < kB; ++u) qn[u] = p[64 * min(j0 + kB + u, last)];
 #pragma unroll
         for (uint32_t u = 0; u < kB; ++u) {
-          if (j0 + u >= nv) break;
+          if (j0 + u >= mine) break;
 #pragma unroll
           for (uint32_t c = 0; c < kV; ++c) {
             if (FMT == kPullWide) {

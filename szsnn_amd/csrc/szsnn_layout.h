@@ -16,7 +16,8 @@
 //     (one target per lane); a slice holds L entries per lane (its longest
 //     lane, rounded up to whole 16-byte vectors), stored as vectors
 //     [L / V][64 lanes][V entries], so the wave's j-th 16-byte load is
-//     contiguous.  Entries name the source relative to the block's first
+//     contiguous; each lane also records how many vectors hold its own
+//     entries (the rest of its column is padding and is not read).  Entries name the source relative to the block's first
 //     staged id.  Wide entries (8 bytes, V = 2): (source offset, weight).
 //     Narrow entries (4 bytes, V = 4), when the network has at most
 //     kPullMaxWeights distinct weights and the offsets fit: source offset <<
@@ -50,7 +51,7 @@ struct HostLayout {
   uint64_t nonempty = 0;              // nonempty rows
   std::vector<uint32_t> pull_off;     // [NB*D*kPullSlices + 1] slice starts (entries), or empty
   std::vector<uint32_t> pull_lo;      // [NB] first staged raster word of each target block
-  std::vector<uint16_t> pull_perm;    // [NB*D][kPullTB] target (within the block) of each slice lane
+  std::vector<uint32_t> pull_lane;    // [NB*D][kPullTB] slice lane: target (within the block) | vectors << kPullShift
   std::vector<LayoutPair> pull_syn;   // wide entries (8 bytes), or empty
   std::vector<uint32_t> pull_syn32;   // narrow entries (4 bytes), or empty
   std::vector<int32_t> pull_wtab;     // narrow: weight of each index (index 0: the inert 0)
@@ -126,7 +127,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
   L.NB = (N + kPullTB - 1) / kPullTB;
   L.pull_off.clear();
   L.pull_lo.clear();
-  L.pull_perm.clear();
+  L.pull_lane.clear();
   L.pull_syn.clear();
   L.pull_syn32.clear();
   L.pull_wtab.clear();
@@ -179,20 +180,27 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     if ((uint64_t)max_rel >= (1ull << (32 - wb))) narrow = false;
   }
   const uint32_t V = narrow ? 4 : 2;  // entries per 16-byte vector
-  // target order per segment and slice offsets
-  std::vector<uint16_t> perm(nseg * kPullTB);
+  // target order per segment and slice offsets; each slice lane records its
+  // target and how many of the slice's vectors hold its entries
+  std::vector<uint16_t> perm(kPullTB);
+  std::vector<uint32_t> lanes(nseg * kPullTB);
   std::vector<uint32_t> off(nseg * kPullSlices + 1, 0);
   std::vector<uint32_t> pos_of(nseg * kPullTB);  // (segment, target) -> its lane's first entry
   uint64_t pos = 0;
   for (uint64_t sg = 0; sg < nseg; ++sg) {
-    uint16_t* pm = &perm[sg * kPullTB];
+    uint16_t* pm = perm.data();
     const uint32_t* c = &cnt[sg * kPullTB];
     for (uint32_t i = 0; i < kPullTB; ++i) pm[i] = (uint16_t)i;
     std::stable_sort(pm, pm + kPullTB, [&](uint16_t a, uint16_t b2) { return c[a] > c[b2]; });
     for (uint32_t sl = 0; sl < kPullSlices; ++sl) {
       off[sg * kPullSlices + sl] = (uint32_t)pos;
       const uint32_t len = (c[pm[sl * 64]] + V - 1) / V * V;  // longest lane of the slice
-      for (uint32_t lane = 0; lane < 64; ++lane) pos_of[sg * kPullTB + pm[sl * 64 + lane]] = (uint32_t)(pos + lane * V);
+      if (len / V >= (1u << (32 - kPullShift))) return true;
+      for (uint32_t lane = 0; lane < 64; ++lane) {
+        const uint32_t tl = pm[sl * 64 + lane];
+        pos_of[sg * kPullTB + tl] = (uint32_t)(pos + lane * V);
+        lanes[sg * kPullTB + sl * 64 + lane] = tl | ((c[tl] + V - 1) / V) << kPullShift;
+      }
       pos += (uint64_t)len * 64;
       if (pos > 0xffffffffull) return true;
     }
@@ -234,7 +242,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
   }
   L.pull_off.swap(off);
   L.pull_lo.swap(lo);
-  L.pull_perm.swap(perm);
+  L.pull_lane.swap(lanes);
   L.pull_words = words;
   return true;
 }

@@ -441,14 +441,14 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint2* d_pull_syn = nullptr;
   uint32_t* d_pull_syn32 = nullptr;
   int32_t* d_pull_wtab = nullptr;
-  uint16_t* d_pull_perm = nullptr;
+  uint32_t* d_pull_lane = nullptr;
   if (rc == SZSNN_OK) guard(dev_alloc(net, &hist, (size_t)n.R * n.W));
   if (rc == SZSNN_OK && !pull_off.empty())
     guard(dev_alloc(net, &d_pull_off, pull_off.size())) &&
         (L.pull_wbits ? guard(dev_alloc(net, &d_pull_syn32, L.pull_syn32.size())) &&
                             guard(dev_alloc(net, &d_pull_wtab, L.pull_wtab.size()))
                       : guard(dev_alloc(net, &d_pull_syn, pull_syn_size))) &&
-        guard(dev_alloc(net, &d_pull_lo, pull_lo.size())) && guard(dev_alloc(net, &d_pull_perm, L.pull_perm.size())) &&
+        guard(dev_alloc(net, &d_pull_lo, pull_lo.size())) && guard(dev_alloc(net, &d_pull_lane, L.pull_lane.size())) &&
         guard(dev_alloc(net, &acc_pull, (size_t)groups * 2 * n.npad));
   if (rc != SZSNN_OK) {
     szsnn_net_destroy(net);
@@ -491,7 +491,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
                             guard(upload(net, d_pull_wtab, L.pull_wtab.data(), L.pull_wtab.size()))
                       : guard(upload(net, d_pull_syn, pull_syn_data, pull_syn_size))) &&
         guard(upload(net, d_pull_lo, pull_lo.data(), pull_lo.size())) &&
-        guard(upload(net, d_pull_perm, L.pull_perm.data(), L.pull_perm.size()));
+        guard(upload(net, d_pull_lane, L.pull_lane.data(), L.pull_lane.size()));
   if (rc == SZSNN_OK) {
     hipStream_t s = ctx->stream;
     he = hipMemsetAsync(acc, 0, (size_t)n.R * 2 * n.npad * 8, s);
@@ -539,7 +539,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.forced_bits = nullptr;
   n.hist = hist;
   n.pull_off = d_pull_off;
-  n.pull_perm = d_pull_perm;
+  n.pull_lane = d_pull_lane;
   n.pull_syn = d_pull_syn;
   n.pull_syn32 = d_pull_syn32;
   n.pull_wtab = d_pull_wtab;

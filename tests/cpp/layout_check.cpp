@@ -132,8 +132,9 @@ static void push_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
 // (bit k: step t-k-1 was deferred); G delay groups, each writing its slot of
 // `out` ([G][2][N]); the caller sums the slots.  Follows pull_body's indexing:
 // per (block, delay) segment kPullSlices slices, lane L of slice s serving
-// target pull_perm[seg*kPullTB + s*64 + L], its vector j at 16-byte index
-// s0/V + 64*j + L; FP64 sums as the kernel forms them.
+// target pull_lane[seg*kPullTB + s*64 + L] % kPullTB, its vector j at 16-byte
+// index s0/V + 64*j + L, read for j < pull_lane[...] >> kPullShift (the rest
+// of the column must be padding); FP64 sums as the kernel forms them.
 static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t t, uint64_t due,
                         const std::vector<uint64_t>& hist, uint32_t R, uint32_t W, uint32_t G,
                         std::vector<long long>& out, uint64_t& events) {
@@ -144,7 +145,7 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
   const uint32_t V = narrow ? 4 : 2, wb = L.pull_wbits;
   const size_t nent = narrow ? L.pull_syn32.size() : L.pull_syn.size();
   CHECK(L.pull_off.size() == (size_t)L.NB * D * kPullSlices + 1, "pull_off size");
-  CHECK(L.pull_perm.size() == (size_t)L.NB * D * kPullTB, "pull_perm size");
+  CHECK(L.pull_lane.size() == (size_t)L.NB * D * kPullTB, "pull_lane size");
   CHECK(L.pull_off.back() == nent, "entries %zu vs offsets %u", nent, L.pull_off.back());
   for (uint32_t bg = 0; bg < L.NB * G; ++bg) {
     const uint32_t b = bg / G, g = bg % G;
@@ -163,7 +164,7 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
       // the slices' lanes serve every target of the block once
       std::vector<uint8_t> seen(kPullTB, 0);
       for (uint32_t i = 0; i < kPullTB; ++i) {
-        const uint16_t tl = L.pull_perm[seg * kPullTB + i];
+        const uint32_t tl = L.pull_lane[seg * kPullTB + i] & (kPullTB - 1);
         CHECK(tl < kPullTB && !seen[tl], "perm of segment %zu is not a permutation", seg);
         seen[tl] = 1;
       }
@@ -180,9 +181,18 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
         if (s0 == s1) continue;
         const uint32_t nv = (s1 - s0) / (64 * V);
         for (uint32_t lane = 0; lane < 64; ++lane) {
-          const uint32_t tl = L.pull_perm[seg * kPullTB + sl * 64 + lane];
+          const uint32_t ln = L.pull_lane[seg * kPullTB + sl * 64 + lane];
+          const uint32_t tl = ln & (kPullTB - 1), mine = ln >> kPullShift;
+          CHECK(mine <= nv, "lane vectors %u beyond the slice's %u", mine, nv);
+          // the column beyond the lane's own vectors is padding
+          for (uint32_t j = mine; j < nv; ++j)
+            for (uint32_t c = 0; c < V; ++c) {
+              const uint64_t e = (uint64_t)(s0 / V + 64 * j + lane) * V + c;
+              const bool pad = narrow ? (L.pull_syn32[e] & ((1u << wb) - 1u)) == 0 : L.pull_syn[e].y == 0;
+              CHECK(pad, "real entry beyond the lane's vectors");
+            }
           double ae = 0.0, ai = 0.0;
-          for (uint32_t j = 0; j < nv; ++j)
+          for (uint32_t j = 0; j < mine; ++j)
             for (uint32_t c = 0; c < V; ++c) {
               const uint64_t e = (uint64_t)(s0 / V + 64 * j + lane) * V + c;
               CHECK(e < s1, "entry beyond the slice");
