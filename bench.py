@@ -392,267 +392,6 @@ def main():
                "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count()}
         log(f"[bench] CPU: {cpu}")
 
-    out = {"batch": 64, "steps": 200, "n_in": 1000, "n_hidden": 2000, "n_out": 100, "checkpoint_every": 20,
-           "gpu_ms_per_batch": gpu_ms}
-    if not args.no_cpu:
-        from oracle import adjoint_ref
-        T, B, H = 200, 64, 2000
-        b = bits.cpu().numpy().view(np.uint64)
-        u = np.unpackbits(b.view(np.uint8).reshape(T, -1), axis=1, bitorder="little")[:, :B * H]
-        spikes = torch.tensor(u.reshape(T, B, H).astype(bool))
-        d = {k: torch.tensor(P[k], dtype=torch.float64) for k in ("W1", "W2", "X", "target", "params")}
-        W1 = d["W1"].clone().requires_grad_(True)
-        W2 = d["W2"].clone().requires_grad_(True)
-        t0 = time.perf_counter()
-        loss, _ = adjoint_ref.forward(W1, W2, d["X"], d["target"], d["params"], P["alpha"], spikes=spikes)
-        loss.backward()
-        cpu_ms = (time.perf_counter() - t0) * 1e3
-        rel1 = float((g1.cpu().double() - W1.grad).norm() / W1.grad.norm())
-        rel2 = float((g2.cpu().double() - W2.grad).norm() / W2.grad.norm())
-        out.update({"cpu_fp64_autograd_ms_per_batch": cpu_ms, "cpu_threads": torch.get_num_threads(),
-                    "rel_err_dW1": rel1, "rel_err_dW2": rel2})
-    log(f"[bench] adjoint: {out}")
-    return out
-
-
-def run_batched(args, ctx, log):
-    """Batched Izhikevich instances (log-uniform sizes 500..20000, own seeds and
-    noise amplitudes) in one launch vs the same instances run one after
-    another as plain networks on the same GPU.  Rasters must match per instance."""
-    import numpy as np
-    from szsnn_amd.batch import BatchSimulation, batched_izhikevich
-    from szsnn_amd.simulation import Simulation, raster_to_bool
-
-    t0 = time.time()
-    nets = batched_izhikevich(n_instances=args.batch_instances)
-    n_tot = sum(n.n_neurons for n in nets)
-    s_tot = sum(n.n_syn for n in nets)
-    log(f"[bench] batched: {len(nets)} instances, {n_tot} neurons, {s_tot} synapses built in {time.time() - t0:.1f}s")
-    T = args.batch_steps
-    bs = BatchSimulation(ctx, nets)
-    tiling = bs.autotune(100)
-    log(f"[bench] batched tiling: {tiling}")
-    bs.set_recording(0, T)
-    ctx.synchronize()
-    c0 = bs.counters()
-    t1 = time.perf_counter()
-    bs.run(T)
-    ctx.synchronize()
-    wall_b = time.perf_counter() - t1
-    ev_b = bs.counters()["events"] - c0["events"]
-    log(f"[bench] batched run: {wall_b * 1e3:.1f} ms for {T} steps, {ev_b / wall_b:.3e} events/s")
-    per_inst = bs.instance_rasters()
-    del bs
-    # each instance run alone, once in graph mode and once as a persistent
-    # cooperative launch; the comparison uses the faster of the two per instance
-    wall = {"graph": 0.0, "persistent": 0.0, "best": 0.0}
-    ev_p = 0
-    same = True
-    for b, net in enumerate(nets):
-        best = None
-        for mode in ("graph", "persistent"):
-            s = Simulation(ctx, net)
-            if mode == "persistent":
-                s.set_mode(1, 256, 0)
-            s.set_recording(0, T)
-            ctx.synchronize()
-            t1 = time.perf_counter()
-            s.run(T)
-            ctx.synchronize()
-            w = time.perf_counter() - t1
-            wall[mode] += w
-            best = w if best is None else min(best, w)
-            if mode == "graph":
-                ev_p += s.counters()["events"]
-            same = same and np.array_equal(raster_to_bool(s.raster(), net.n_neurons), per_inst[b])
-            del s
-        wall["best"] += best
-        if b % 32 == 31:
-            log(f"[bench] plain runs: {b + 1}/{len(nets)}, {wall} s so far")
-    wall_p = wall["best"]
-    out = {"instances": len(nets), "neurons": n_tot, "synapses": s_tot, "steps": T, "dt_ms": 1.0,
-           "batched_events_per_s": ev_b / wall_b, "plain_sequential_events_per_s": ev_p / wall_p,
-           "speedup": (ev_b / wall_b) / (ev_p / wall_p), "batched_wall_s": wall_b,
-           "plain_sequential_wall_s": wall_p, "plain_graph_wall_s": wall["graph"],
-           "plain_persistent_wall_s": wall["persistent"], "events": ev_b,
-           "rasters_identical": bool(same and ev_b == ev_p), "tiling": tiling}
-    log(f"[bench] batched: {out}")
-    return out
-
-
-def main():
-    args = parse()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-    import numpy as np
-    import torch
-    from szsnn_amd.configs import if_cond_exp_100k
-    from szsnn_amd.simulation import Context, Simulation
-
-    t0 = time.time()
-    if args.small:
-        net = if_cond_exp_100k(n_exc=8000, n_inh=2000, k_exc=400, k_inh=100)
-    else:
-        net = if_cond_exp_100k()
-    log(f"[bench] network built in {time.time() - t0:.1f}s: {net.n_neurons} neurons, {net.n_syn} synapses")
-    dt_ms = net.meta["dt_ms"]
-    ctx = Context(0)
-    t0 = time.time()
-    sim = Simulation(ctx, net)
-    log(f"[bench] uploaded in {time.time() - t0:.1f}s to {ctx.name}; device bytes {sim.info.device_bytes}")
-
-    if args.profile_pass:
-        # calibration: a plain copy of known size (FETCH_SIZE / WRITE_SIZE of 1 GiB each)
-        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
-        # fill every delay slot with one persistent launch (no gather_kernel
-        # dispatch), then launch gather_kernel exactly --volley-steps times
-        sim.set_tiling(gather_block=args.gather_block, gather_grid=args.gather_grid)
-        sim.set_mode(1)
-        sim.set_volley(net.n_neurons)
-        sim.run(net.max_delay + 2)
-        ctx.synchronize()
-        c0 = sim.counters()
-        g_ms, u_ms = sim.time_kernels(args.volley_steps)
-        c1 = sim.counters()
-        n = args.volley_steps
-        ev = (c1["events"] - c0["events"]) / n
-        rw = (c1["rows"] - c0["rows"]) / n
-        per = gather_alg_bytes(ev, rw)
-        print(json.dumps({"profile_pass": True, "gather_block": args.gather_block, "gather_grid": args.gather_grid,
-                          "launches": n, "gather_us": g_ms * 1e3 / n, "update_us": u_ms * 1e3 / n,
-                          "events_per_launch": ev, "rows_per_launch": rw, "alg_bytes_per_launch": per,
-                          "gbs": per / (g_ms * 1e-3 / n) / 1e9}), flush=True)
-        return
-    if args.natural_pass:
-        sim.set_mode(1, 512, 256)
-        sim.run(args.warmup)
-        ctx.synchronize()
-        c0 = sim.counters()
-        t1 = time.perf_counter()
-        sim.run(args.steps)
-        ctx.synchronize()
-        wall = time.perf_counter() - t1
-        c1 = sim.counters()
-        g_ms, u_ms = sim.time_kernels(args.volley_steps)
-        print(json.dumps({"natural_pass": True, "steps": args.steps, "us_per_step": wall * 1e6 / args.steps,
-                          "events_per_step": (c1["events"] - c0["events"]) / args.steps,
-                          "spikes_per_step": (c1["spikes"] - c0["spikes"]) / args.steps,
-                          "single_step_gather_us": g_ms * 1e3 / args.volley_steps,
-                          "single_step_update_us": u_ms * 1e3 / args.volley_steps}), flush=True)
-        return
-    if not args.no_autotune:
-        log(f"[bench] autotuned for the natural dynamics: {sim.autotune(200)}")
-    # warmup (also builds the step graph)
-    sim.run(args.warmup)
-    torch.cuda.synchronize()
-    ctx.synchronize()
-    c0 = sim.counters()
-    t1 = time.perf_counter()
-    sim.run(args.steps)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
-    c1 = sim.counters()
-    events = c1["events"] - c0["events"]
-    spikes = c1["spikes"] - c0["spikes"]
-    rows = c1["rows"] - c0["rows"]
-    ms_per_step = wall * 1e3 / args.steps
-    ev_per_s = events / wall
-    rtf = args.steps * dt_ms / (wall * 1e3)
-    rate_hz = spikes / net.n_neurons / (args.steps * dt_ms * 1e-3)
-    log(f"[bench] natural: {ms_per_step * 1e3:.2f} us/step, {ev_per_s:.3e} events/s, RTF {rtf:.2f}, "
-        f"{rate_hz:.2f} Hz, {events / args.steps:.0f} events/step")
-
-    # per-kernel time on the natural dynamics (HIP events, same stream); the
-    # update kernel also pushes the step's spikes
-    kn = 500
-    cg0 = sim.counters()
-    g_ms, u_ms = sim.time_kernels(kn)
-    cg1 = sim.counters()
-    info = sim.info
-    # per neuron: params, vars read+written, its accumulator slot, key,
-    # synapse range and delay bits; per pushed synapse: its 8-byte entry
-    nat_u_bytes = ((4 * info.n_params + 8 * info.n_vars + 16 + 24) * net.n_neurons * kn
-                   + 8 * (cg1["events"] - cg0["events"]))
-    natural = {
-        "gather_us": g_ms * 1e3 / kn, "update_us": u_ms * 1e3 / kn,
-        "update_gbs": nat_u_bytes / (u_ms * 1e-3) / 1e9 if u_ms > 0 else None,
-    }
-    log(f"[bench] natural kernels: {natural}")
-
-    # volley: every neuron forced every step -> every (source, delay) row is due
-    natural_tiling = (sim.tiling(), sim.mode())
-
-    default_policy = sim.gather_policy()["dense_fraction"]
-    groups = int(sim.info.pull_groups)
-    pull_entry_bytes = int(sim.info.pull_entry_bytes)
-
-    def volley(fraction):
-        sim.set_gather_policy(fraction)
-        sim.set_volley(net.n_neurons)
-        sim.run(net.max_delay + 2)  # fill all delay slots
-        if not args.no_autotune:
-            log(f"[bench] autotuned for the volley (policy {fraction}): {sim.autotune(10)}")
-        sim.set_mode(0)  # time the kernels one launch at a time
-        cv0 = sim.counters()
-        gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
-        cv1 = sim.counters()
-        sim.set_volley(0)
-        ev = (cv1["events"] - cv0["events"]) / args.volley_steps
-        rw = (cv1["rows"] - cv0["rows"]) / args.volley_steps
-        g_s = gv_ms * 1e-3 / args.volley_steps
-        u_s = uv_ms * 1e-3 / args.volley_steps
-        return {"gather_us": g_s * 1e6, "update_us": u_s * 1e6, "events_per_launch": ev, "rows_per_launch": rw}
-    # push only: the update kernel pushes 1e8 synapses per step with atomics
-    v_push = volley(2.0)
-    v_push["push_events_per_s"] = v_push["events_per_launch"] / (v_push["update_us"] * 1e-6)
-    log(f"[bench] volley push-only: {v_push}")
-    # default policy: every step is deferred and pulled
-    v_auto = volley(default_policy)
-    per = gather_alg_bytes(v_auto["events_per_launch"], v_auto["rows_per_launch"])
-    v_auto["alg_bytes_per_launch"] = per
-    v_auto["pull_groups"] = groups
-    v_auto["gbs"] = per / (v_auto["gather_us"] * 1e-6) / 1e9
-    log(f"[bench] volley pull (policy {default_policy}): {v_auto}")
-    sim.set_gather_policy(default_policy)
-    v_auto["tiling"] = sim.tiling()
-    per_launch = v_auto["alg_bytes_per_launch"]
-    g_avg_s = v_auto["gather_us"] * 1e-6
-    achieved = v_auto["gbs"]
-
-    del sim
-    copy_gbs = copy_bandwidth_gbs(torch)
-    log(f"[bench] device copy: {copy_gbs:.0f} GB/s (read + write)")
-    traffic = None
-    if os.path.exists(args.traffic_file):
-        with open(args.traffic_file) as f:
-            traffic = json.load(f)
-    batched = None if args.no_batched else run_batched(args, ctx, log)
-    adjoint = None if args.no_adjoint else run_adjoint(args, log)
-
-    cpu = None
-    if not args.no_cpu:
-        from oracle.pyoracle import Oracle
-        t0 = time.time()
-        o = Oracle(net, 32)
-        log(f"[bench] CPU oracle built in {time.time() - t0:.1f}s")
-        secs = o.time_steps(args.cpu_steps)
-        cpu = {"value": o.events / secs, "unit": "events/s", "cores": 1, "kind": "port",
-               "sample": f"same network, FP32 CPU restatement, first {args.cpu_steps} steps "
-                         f"({args.cpu_steps * dt_ms:.0f} ms simulated) from the initial state; "
-                         f"{secs:.1f} s wall, RTF {args.cpu_steps * dt_ms / (secs * 1e3):.3f}",
-               "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count()}
-        log(f"[bench] CPU: {cpu}")
-
     out = {
         "metric": "synaptic events/s (plain IF_cond_exp 100k net); realtime factor; gather % HBM peak",
         "value": ev_per_s,
