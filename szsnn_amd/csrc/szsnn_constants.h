@@ -15,6 +15,26 @@ constexpr uint32_t kPullSlices = kPullTB / 64;   // slices of 64 targets per (bl
 constexpr uint32_t kPullMaxWeights = 255;        // distinct nonzero weights of narrow entries
 constexpr uint32_t kMaxPullGroups = 8;           // delay groups of the pull (work items per target block)
 
+// Pull entry formats.  Every entry locates its source in the block's staged
+// spike bitmap by rel = source - 64 * (first staged word), in the low
+// kPullRelBits bits: bit rel % 32 of 32-bit word rel / 32.  Padding entries
+// point at a zero word staged after the bitmap (rel = 64 * staged words).
+//   wide:  8 bytes (rel, fixed-point weight);
+//   count: 4 bytes rel | shift << kPullRelBits, for networks with at most
+//          kPullCountFields distinct weights: the lane counts selected
+//          entries in kPullCountBits-bit fields of one register, the field
+//          of weight i (1-based) at shift kPullCountBits * (i - 1);
+//   table: 4 bytes rel | i << kPullRelBits, weight i (1-based) of a table of
+//          at most kPullMaxWeights.
+enum : uint32_t { kPullWide = 0, kPullCount = 1, kPullTable = 2 };
+constexpr uint32_t kPullRelBits = 21;
+constexpr uint32_t kPullRelMask = (1u << kPullRelBits) - 1;
+constexpr uint32_t kPullCountFields = 3;
+constexpr uint32_t kPullCountBits = 10;
+static_assert((uint64_t)(kPullMaxWords + 1) * 64 <= kPullRelMask, "rel of the zero word must fit");
+static_assert(kPullMaxWeights < (1u << (32 - kPullRelBits)), "weight index must fit");
+static_assert(kPullCountBits * kPullCountFields <= 32, "count fields must fit one register");
+
 // Push entries carry the API delay in the target word's top bits.
 constexpr uint32_t kSynDelayShift = 26;
 constexpr uint32_t kSynTargetMask = (1u << kSynDelayShift) - 1;

@@ -86,23 +86,35 @@ struct DevNet {
   // kPullSlices + s] .. (next), lane L serving target pull_lane[(b*D+k)*
   // kPullTB + s*64 + L] % kPullTB of the block, whose entries fill the first
   // pull_lane[...] >> kPullShift vectors of its column; 16-byte vectors
-  // [L_s/V][64][V].  Wide
-  // entries: (source - pull_lo[b]*64, weight); narrow: (source - pull_lo[b]*
-  // 64) << wbits | weight index.  Padding entries: weight 0.
+  // [L_s/V][64][V].  Entry formats (pull_fmt): see kPullWide, kPullCount,
+  // kPullTable; rel counts from source pull_lo[b]*64.
   const uint32_t* pull_off; // [NB*D*kPullSlices + 1] or null (no pull layout)
   const uint32_t* pull_lane;  // [NB*D*kPullTB] target | vectors << kPullShift
   const uint2* pull_syn;    // wide entries, or null
-  const uint32_t* pull_syn32;  // narrow entries, or null
-  const int32_t* pull_wtab; // narrow: [pull_wn] weight of each index
-  uint32_t pull_wbits;      // narrow: weight-index bits (0: wide entries)
+  const uint32_t* pull_syn32;  // narrow (count, table) entries, or null
+  const int32_t* pull_wtab; // narrow: [pull_wn] weight of each index (index 0: 0)
+  uint32_t pull_fmt;        // kPullWide, kPullCount or kPullTable
   uint32_t pull_wn;         // narrow: weight-table entries
   const uint32_t* pull_lo;  // [NB] first raster word of the block's sources
   uint32_t pull_nb;         // NB
   uint32_t pull_words;      // LDS words staged per block (max over blocks)
 };
 
+// 64-bit words of a pull block's LDS bitmap: the staged words, the zero word
+// the padding entries read, rounded up to an even count so the FP64 pairs
+// after it stay 16-byte aligned.
+__host__ __device__ inline uint32_t pull_bitmap_u64(uint32_t words) { return (words + 2) & ~1u; }
+
 // Counter slots.
 enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntRows = 4, kCntSlots = 5 };
+
+// Load of data that no kernel writes (the network's layout tables) at a
+// wave-uniform address, through the scalar cache (constant address space), so
+// the wave's waits for it do not also wait for its vector loads in flight.
+template <class T>
+__device__ inline T ld_const(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
 
 // Agent-coherent loads and stores (relaxed agent-scope atomics, `sc1` on
 // gfx950) for the state that one workgroup writes and another reads within a

@@ -617,135 +617,189 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
   return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
 }
 
-// Entry formats of the pull: wide (source offset, weight); narrow with a
-// weight table of at most kPullCountWeights entries (per lane, counts of
-// selected entries per weight index); narrow with a larger table (FP64
-// table lookups).
-enum : int { kPullWide = 0, kPullCount = 1, kPullTable = 2 };
-constexpr uint32_t kPullCountWeights = 4;  // 16-bit counters packed in a u64
-
 // Dense gather (pull) for target block `b` and delay group `g` over the
 // sliced layout: for each due delay k of the group the spike bitmap of step
 // t-k-1 (the block's source range) is staged in LDS; each wave takes slices
 // of 64 targets, one target per lane, and walks its lane's entries with
 // 16-byte loads (the wave's j-th load is contiguous; the next batch is in
-// flight while the current one is summed).  Per entry the source's spike bit
-// b selects the weight.  Sums are exact: FP64 sums of integers below 2^53
-// (wide: E += b * max(w, 0), I += b * min(w, 0); table: (E, I) of index
-// (b, weight index), zero for b = 0), or, for small tables, 16-bit counts of
-// selected entries per weight index, multiplied out once per slice.  The
-// lane adds its sums into the block's LDS sums of its target (one lane per
-// target and delay, delays one after the other: no atomics); the block
-// stores its targets' sums for the group into pull slot g: plain stores, no
-// global atomics.
-// dyn: pull_words u64 bitmap | 2*kPullTB f64 sums | narrow formats: 2 << wb
-// (E, I) f64 pairs indexed by (spike bit, weight index), zero for bit 0 and
-// for indices beyond the table.
+// flight while the current one is summed).  Per entry one 32-bit LDS read
+// and one bit-field extract give the source's spike bit b.  Sums are exact:
+// FP64 sums of integers below 2^53 (wide: E += b * max(w, 0), I += b *
+// min(w, 0); table: the (E, I) pair at 2 * index + b, zero for b = 0), or,
+// in the count format, per-weight counts of selected entries in bit fields
+// of one register, multiplied out once per slice.  The lane adds its sums
+// into the block's LDS sums of its target (one lane per target and delay,
+// delays one after the other: no atomics); the block stores its targets'
+// sums for the group into pull slot g: plain stores, no global atomics.
+// dyn: bitmap (pull_bitmap_u64 words; the two 32-bit words after the staged
+// ones stay zero for the padding entries) | 2*kPullTB f64 sums | narrow
+// formats: 2 * pull_wn (E, I) f64 pairs.
 template <int BLK, int FMT, bool AGENT>
 __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
   constexpr bool kNarrow = FMT != kPullWide;
   constexpr uint32_t kV = kNarrow ? 4 : 2;  // entries per 16-byte vector
   constexpr uint32_t kB = 4;                // vectors per lane and batch
   constexpr uint32_t kWaves = BLK / 64;
-  // count format: flush the 16-bit counters before any can reach 2^16
-  constexpr uint32_t kFlushVectors = 16380;
-  uint64_t* bits = dyn;
-  double* sums = reinterpret_cast<double*>(dyn + n.pull_words);  // [2][kPullTB]: E, I
-  double2* wt = reinterpret_cast<double2*>(sums + 2 * kPullTB);  // [2 << wb]: (bit, index)
-  const uint32_t wb = n.pull_wbits, wmask = (1u << wb) - 1u;
+  // count format: flush the fields before any can pass 2^kPullCountBits - 1
+  constexpr uint32_t kFlushVectors = ((1u << kPullCountBits) - 1) / kV / kB * kB;
+  static_assert(kFlushVectors >= kB, "count fields too narrow for one batch");
+  const uint32_t* bits = reinterpret_cast<const uint32_t*>(dyn);
+  double* sums = reinterpret_cast<double*>(dyn + pull_bitmap_u64(n.pull_words));  // [2][kPullTB]: E, I
+  double2* wt = reinterpret_cast<double2*>(sums + 2 * kPullTB);                   // [2 * pull_wn]
   for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) sums[i] = 0.0;
+  if (threadIdx.x == 0) dyn[n.pull_words] = 0ull;  // the padding entries' zero word
   if (kNarrow)
-    for (uint32_t i = threadIdx.x; i < (2u << wb); i += BLK) {
-      const uint32_t wi = i & wmask;
-      const double w = (i >> wb) && wi < n.pull_wn ? (double)n.pull_wtab[wi] : 0.0;
+    for (uint32_t i = threadIdx.x; i < 2 * n.pull_wn; i += BLK) {
+      const double w = (i & 1u) ? (double)n.pull_wtab[i >> 1] : 0.0;
       wt[i] = make_double2(w > 0.0 ? w : 0.0, w < 0.0 ? w : 0.0);
     }
-  const uint32_t lo = n.pull_lo[b];
+  const uint32_t lo = ld_const(&n.pull_lo[b]);
   const uint32_t nw = min(n.pull_words, n.W - lo);
   const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
   const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
   const uint4* vec = kNarrow ? reinterpret_cast<const uint4*>(n.pull_syn32) : reinterpret_cast<const uint4*>(n.pull_syn);
-  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
   uint32_t events = 0;
-  for (uint32_t k = k0; k < k1; ++k) {
-    if (!((due >> k) & 1ull)) continue;
-    const uint32_t seg = b * n.D + k;
-    const uint32_t* so = n.pull_off + (size_t)seg * kPullSlices;
-    if (so[0] == so[kPullSlices]) continue;  // no delay-k synapses into the block
+  // The first due delay at or after k with synapses into the block (k1: none).
+  auto due_from = [&](uint32_t k) {
+    for (; k < k1; ++k) {
+      if (!((due >> k) & 1ull)) continue;
+      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices;
+      if (ld_const(&so[0]) != ld_const(&so[kPullSlices])) break;
+    }
+    return k;
+  };
+  // The wave walks its columns (slices wave, wave + kWaves, ... of each due
+  // delay, one lane per target) as one stream.  col_from moves (k, sl) to
+  // the next nonempty column at or after it (k = k1: none) and gives its
+  // first entry s0 and its vectors per lane nv (the slice's longest lane).
+  auto col_from = [&](uint32_t& k, uint32_t& sl, uint32_t& s0, uint32_t& nv) {
+    while (k < k1) {
+      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices;
+      for (; sl < kPullSlices; sl += kWaves) {
+        s0 = ld_const(&so[sl]);
+        const uint32_t s1 = ld_const(&so[sl + 1]);
+        if (s1 != s0) {
+          nv = (s1 - s0) / (64 * kV);
+          return;
+        }
+      }
+      k = due_from(k + 1);
+      sl = wave;
+    }
+    nv = 0;
+  };
+  // The lane's pull_lane word of column (k, sl), loaded unconditionally:
+  // past the end of the stream it reads a valid word and ignores it.
+  auto lane_word = [&](uint32_t k, uint32_t sl) {
+    return n.pull_lane[(size_t)(b * n.D + min(k, k1 - 1)) * kPullTB + min(sl, kPullSlices - 1) * 64 + lane];
+  };
+  // Entry loads run two batches ahead of the sums in two buffers, A and B,
+  // that take turns (no register copies, so the compiler waits for the batch
+  // being summed alone).  Every load is unconditional: a lane's loads are
+  // clamped to its last own vector (its padding vectors are never read: the
+  // repeats hit the same line, and the sums skip them), a column is walked
+  // in an even number of batches, and the batch after a column's last is
+  // the next column's first (clamped to that slice), loaded into A before
+  // the bitmap of its delay is staged.  Past the end of the stream the loads
+  // re-read vector `lane`, which exists (a nonempty slice has 64 lanes).
+  uint32_t k = due_from(k0);
+  uint32_t ck = k, csl = wave, cs0 = 0, cnv = 0;
+  col_from(ck, csl, cs0, cnv);
+  uint32_t cln = lane_word(ck, csl);
+  uint4 qa[kB], qb[kB];
+  {
+    // clamped to the lane's own vectors: these loads follow the lane word's
+    const uint4* p0 = vec + (cnv ? cs0 / kV : 0u) + lane;
+    const uint32_t m0 = cnv ? cln >> kPullShift : 0u;
+    const uint32_t l0 = m0 ? m0 - 1 : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kB; ++u) qa[u] = p0[64 * min(u, l0)];
+  }
+  while (k < k1) {  // block-uniform: every wave stages every due delay
     const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
-    __syncthreads();  // the previous delay's readers are done with `bits`
-    for (uint32_t w = threadIdx.x; w < nw; w += BLK) bits[w] = ld_step<AGENT>(&src[w]);
+    __syncthreads();  // the previous delay's readers are done with the bitmap
+    for (uint32_t w = threadIdx.x; w < nw; w += BLK) dyn[w] = ld_step<AGENT>(&src[w]);
     __syncthreads();
-    for (uint32_t sl = wave; sl < kPullSlices; sl += kWaves) {
-      const uint32_t s0 = so[sl], s1 = so[sl + 1];
-      if (s0 == s1) continue;
-      const uint32_t nv = (s1 - s0) / (64 * kV);  // vectors per lane (the slice's longest)
-      const uint32_t ln = n.pull_lane[(size_t)seg * kPullTB + sl * 64 + lane];
-      const uint32_t tl = ln & (kPullTB - 1), mine = ln >> kPullShift;  // target, the lane's own vectors
-      const uint4* p = vec + s0 / kV + lane;     // the lane's vector j at p[64 * j]
-      double ae = 0.0, ai = 0.0;
-      uint64_t cpk = 0;  // count format: 16-bit counts per weight index
-      // every load is unconditional (indices clamped to the lane's last own
-      // vector, so its padding vectors are never read: the repeats hit the
-      // same line), so the compiler can wait for the current batch alone
-      // while the next one stays in flight
+    while (ck == k) {  // the wave's columns of delay k
+      uint32_t nk = ck, nsl = csl + kWaves, ns0 = 0, nnv = 0;
+      col_from(nk, nsl, ns0, nnv);
+      const uint32_t nln = lane_word(nk, nsl);
+      const uint32_t tl = cln & (kPullTB - 1), mine = cln >> kPullShift;  // target, the lane's own vectors
       const uint32_t last = mine ? mine - 1 : 0u;
-      uint4 q[kB];
-#pragma unroll
-      for (uint32_t u = 0; u < kB; ++u) q[u] = p[64 * min(u, last)];
-      for (uint32_t j0 = 0; j0 < nv; j0 += kB) {
-        uint4 qn[kB];
-#pragma unroll
-        for (uint32_t u = 0; u < kB; ++u) qn[u] = p[64 * min(j0 + kB + u, last)];
+      const uint4* p = vec + cs0 / kV + lane;                 // the lane's vector j at p[64 * j]
+      const uint4* pn = vec + (nnv ? ns0 / kV : 0u) + lane;   // the next column's
+      const uint32_t nlast = nnv ? nnv - 1 : 0u;
+      const uint32_t nv = cnv;
+      double ae = 0.0, ai = 0.0;
+      uint32_t cnt = 0;  // count format: kPullCountBits-bit fields per weight
+      // sums of the batch of vectors j0 .. j0 + kB - 1 held in qq
+      auto sum_batch = [&](const uint4 (&qq)[kB], uint32_t j0) {
 #pragma unroll
         for (uint32_t u = 0; u < kB; ++u) {
           if (j0 + u >= mine) break;
 #pragma unroll
           for (uint32_t c = 0; c < kV; ++c) {
+            const uint32_t e = comp(qq[u], kNarrow ? c : 2 * c);
+            const uint32_t word = bits[(e >> 5) & (kPullRelMask >> 5)];
+            const uint32_t bit = __builtin_amdgcn_ubfe(word, e, 1u);  // bit e % 32
             if (FMT == kPullWide) {
-              const uint32_t rel = comp(q[u], 2 * c);
-              const int32_t w = (int32_t)comp(q[u], 2 * c + 1);
-              const uint32_t bit = (uint32_t)(bits[rel >> 6] >> (rel & 63)) & 1u;
+              const int32_t w = (int32_t)comp(qq[u], 2 * c + 1);
               const double bw = (double)(bit ? w : 0);
               ae += bw > 0.0 ? bw : 0.0;
               ai += bw < 0.0 ? bw : 0.0;
-              events += bit & (w != 0 ? 1u : 0u);
+              events += bit;
+            } else if (FMT == kPullCount) {
+              cnt += bit << (e >> kPullRelBits);
             } else {
-              const uint32_t e = comp(q[u], c);
-              const uint32_t rel = e >> wb, wi = e & wmask;
-              const uint32_t bit = (uint32_t)(bits[rel >> 6] >> (rel & 63)) & 1u;
-              if (FMT == kPullCount) {
-                cpk += (uint64_t)bit << (16 * wi);
-              } else {
-                const uint32_t idx = (bit << wb) | wi;  // (bit, index): zero weights for bit 0
-                const double2 wv = wt[idx];
-                ae += wv.x;
-                ai += wv.y;
-                events += idx > (1u << wb) ? 1u : 0u;  // index 0 is the inert 0
-              }
+              const double2 wv = wt[((e >> kPullRelBits) << 1) | bit];
+              ae += wv.x;
+              ai += wv.y;
+              events += bit;
             }
           }
         }
-#pragma unroll
-        for (uint32_t u = 0; u < kB; ++u) q[u] = qn[u];
         if (FMT == kPullCount && ((j0 / kB + 1) % (kFlushVectors / kB) == 0 || j0 + kB >= nv)) {
-          // counts -> sums (index 0: padding and inert entries)
+          // fields -> sums: field i counts the selected entries of weight i + 1
 #pragma unroll
-          for (uint32_t i = 1; i < kPullCountWeights; ++i) {
-            const uint32_t c = (uint32_t)(cpk >> (16 * i)) & 0xffffu;
-            if (i < n.pull_wn) {
-              const double2 wv = wt[(1u << wb) + i];
+          for (uint32_t i = 0; i < kPullCountFields; ++i) {
+            const uint32_t c = (cnt >> (kPullCountBits * i)) & ((1u << kPullCountBits) - 1u);
+            if (i + 1 < n.pull_wn) {
+              const double2 wv = wt[2 * (i + 1) + 1];
               ae += (double)c * wv.x;
               ai += (double)c * wv.y;
               events += c;
             }
           }
-          cpk = 0;
+          cnt = 0;
         }
-      }
+      };
+      // a column has at least one vector: the loop body runs, so every path
+      // issues its loads after the lane words (the compiler's waits rely on it)
+      uint32_t j0 = 0;
+      do {
+#pragma unroll
+        for (uint32_t u = 0; u < kB; ++u) qb[u] = p[64 * min(j0 + kB + u, last)];
+        sum_batch(qa, j0);
+        const bool here = j0 + 2 * kB < nv;
+#pragma unroll
+        for (uint32_t u = 0; u < kB; ++u) {
+          const uint4* a = here ? p + 64 * min(j0 + 2 * kB + u, last) : pn + 64 * min(u, nlast);
+          qa[u] = *a;
+        }
+        sum_batch(qb, j0 + kB);
+        j0 += 2 * kB;
+      } while (j0 < nv);
       sums[tl] += ae;
       sums[kPullTB + tl] += ai;
+      ck = nk;
+      csl = nsl;
+      cs0 = ns0;
+      cnv = nnv;
+      cln = nln;
     }
+    k = due_from(k + 1);
   }
   __syncthreads();
   long long* out = n.acc_pull + (size_t)g * 2 * n.npad;
@@ -766,9 +820,9 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
 // The pull of work item (b, g) in the network's entry format.
 template <int BLK, bool AGENT>
 __device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
-  if (!n.pull_wbits)
+  if (n.pull_fmt == kPullWide)
     pull_body<BLK, kPullWide, AGENT>(n, t, b, g, due, dyn);
-  else if (n.pull_wn <= kPullCountWeights)
+  else if (n.pull_fmt == kPullCount)
     pull_body<BLK, kPullCount, AGENT>(n, t, b, g, due, dyn);
   else
     pull_body<BLK, kPullTable, AGENT>(n, t, b, g, due, dyn);
@@ -779,7 +833,7 @@ __device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32
 // when nothing is due (every block reads the same stamps).
 template <int BLK>
 __global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
-  extern __shared__ uint64_t dyn[];
+  extern __shared__ __align__(16) uint64_t dyn[];
   const uint64_t t = *n.t_base + j;
   const uint64_t due = due_mask<false>(n, t);
   if (!due) return;
@@ -885,7 +939,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       if (t0 > 0 && spk) atomicAdd(&f0.spk_prev, spk);
     }
   }
-  extern __shared__ uint64_t dyn[];
+  extern __shared__ __align__(16) uint64_t dyn[];
   const uint32_t id = blockIdx.x * BLK + threadIdx.x;
   LaneState<MODEL> ls;
   if (carry) {
@@ -982,7 +1036,8 @@ hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_
 }
 
 size_t pull_dyn_bytes(const DevNet& n) {
-  return n.pull_off ? (size_t)n.pull_words * 8 + (size_t)2 * kPullTB * 8 + (n.pull_wbits ? (size_t)16 << (n.pull_wbits + 1) : 0)
+  return n.pull_off ? (size_t)pull_bitmap_u64(n.pull_words) * 8 + (size_t)2 * kPullTB * 8 +
+                          (n.pull_fmt != kPullWide ? (size_t)32 * n.pull_wn : 0)
                     : 0;
 }
 

@@ -55,7 +55,7 @@ struct HostLayout {
   std::vector<LayoutPair> pull_syn;   // wide entries (8 bytes), or empty
   std::vector<uint32_t> pull_syn32;   // narrow entries (4 bytes), or empty
   std::vector<int32_t> pull_wtab;     // narrow: weight of each index (index 0: the inert 0)
-  uint32_t pull_wbits = 0;            // narrow: weight-index bits (0: wide entries)
+  uint32_t pull_fmt = kPullWide;      // entry format (kPullWide, kPullCount, kPullTable)
   uint32_t pull_words = 0;
   uint64_t NB = 0;
   std::string error;
@@ -131,7 +131,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
   L.pull_syn.clear();
   L.pull_syn32.clear();
   L.pull_wtab.clear();
-  L.pull_wbits = 0;
+  L.pull_fmt = kPullWide;
   L.pull_words = 0;
   if (d.pull_layout < 0 || S == 0) return true;
   const uint64_t NB = L.NB;
@@ -172,13 +172,12 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     max_rel = std::max(max_rel, smax[b] - lo[b] * 64);
   }
   if (words > kPullMaxWords) return true;  // source spans too wide to stage: push only
-  // narrow entries: (source - block base) << wb | weight index
-  uint32_t wb = 0;
-  if (narrow) {
-    while ((1ull << wb) < wset.size() + 1) ++wb;  // index 0 is the inert weight 0
-    if (wb == 0) wb = 1;
-    if ((uint64_t)max_rel >= (1ull << (32 - wb))) narrow = false;
+  if (max_rel >= (uint64_t)words * 64) {
+    L.error = "pull layout: source offset beyond the staged words";
+    return false;
   }
+  // padding entries read the zero word staged after the bitmap
+  const uint32_t pad_rel = words * 64;
   const uint32_t V = narrow ? 4 : 2;  // entries per 16-byte vector
   // target order per segment and slice offsets; each slice lane records its
   // target and how many of the slice's vectors hold its entries
@@ -216,7 +215,8 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     std::sort(wset.begin(), wset.end());
     std::vector<int32_t> wtab(1, 0);
     wtab.insert(wtab.end(), wset.begin(), wset.end());
-    std::vector<uint32_t> ps(pos, 0u);  // padding: first staged source, weight index 0
+    const bool count = wset.size() <= kPullCountFields;
+    std::vector<uint32_t> ps(pos, pad_rel);  // padding: the zero word, tag 0
     for (uint64_t id = 0; id < T; ++id)
       for (uint64_t k = 0; k < D; ++k)
         for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
@@ -224,13 +224,14 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
           const uint64_t b = tg >> kPullShift;
           const uint32_t wi = (uint32_t)(std::lower_bound(wtab.begin() + 1, wtab.end(), (int32_t)L.syn[q].y) -
                                          wtab.begin());
-          ps[place((b * D + k) * kPullTB + (tg & (kPullTB - 1)))] = (((uint32_t)id - lo[b] * 64) << wb) | wi;
+          const uint32_t tag = count ? kPullCountBits * (wi - 1) : wi;
+          ps[place((b * D + k) * kPullTB + (tg & (kPullTB - 1)))] = ((uint32_t)id - lo[b] * 64) | tag << kPullRelBits;
         }
     L.pull_syn32.swap(ps);
     L.pull_wtab.swap(wtab);
-    L.pull_wbits = wb;
+    L.pull_fmt = count ? kPullCount : kPullTable;
   } else {
-    std::vector<LayoutPair> ps(pos, LayoutPair{0u, 0u});  // padding: first staged source, weight 0
+    std::vector<LayoutPair> ps(pos, LayoutPair{pad_rel, 0u});  // padding: the zero word, weight 0
     for (uint64_t id = 0; id < T; ++id)
       for (uint64_t k = 0; k < D; ++k)
         for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {

@@ -134,15 +134,18 @@ static void push_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
 // per (block, delay) segment kPullSlices slices, lane L of slice s serving
 // target pull_lane[seg*kPullTB + s*64 + L] % kPullTB, its vector j at 16-byte
 // index s0/V + 64*j + L, read for j < pull_lane[...] >> kPullShift (the rest
-// of the column must be padding); FP64 sums as the kernel forms them.
+// of the column must be padding); the spike bit of rel from 32-bit word
+// rel / 32 of the staged bitmap, padding from the zero word after it; FP64
+// sums as the kernel forms them.
 static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t t, uint64_t due,
                         const std::vector<uint64_t>& hist, uint32_t R, uint32_t W, uint32_t G,
                         std::vector<long long>& out, uint64_t& events) {
   const uint32_t words = L.pull_words;
   std::vector<uint64_t> bits(words);
   std::vector<double> sums(2 * kPullTB);
-  const bool narrow = L.pull_wbits != 0;
-  const uint32_t V = narrow ? 4 : 2, wb = L.pull_wbits;
+  const bool narrow = L.pull_fmt != kPullWide;
+  const uint32_t V = narrow ? 4 : 2;
+  const uint32_t pad_rel = words * 64;
   const size_t nent = narrow ? L.pull_syn32.size() : L.pull_syn.size();
   CHECK(L.pull_off.size() == (size_t)L.NB * D * kPullSlices + 1, "pull_off size");
   CHECK(L.pull_lane.size() == (size_t)L.NB * D * kPullTB, "pull_lane size");
@@ -188,7 +191,7 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
           for (uint32_t j = mine; j < nv; ++j)
             for (uint32_t c = 0; c < V; ++c) {
               const uint64_t e = (uint64_t)(s0 / V + 64 * j + lane) * V + c;
-              const bool pad = narrow ? (L.pull_syn32[e] & ((1u << wb) - 1u)) == 0 : L.pull_syn[e].y == 0;
+              const bool pad = narrow ? L.pull_syn32[e] == pad_rel : (L.pull_syn[e].x == pad_rel && L.pull_syn[e].y == 0);
               CHECK(pad, "real entry beyond the lane's vectors");
             }
           double ae = 0.0, ai = 0.0;
@@ -200,17 +203,29 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
               int32_t w;
               if (narrow) {
                 const uint32_t x = L.pull_syn32[e];
-                rel = x >> wb;
-                const uint32_t wi = x & ((1u << wb) - 1u);
+                rel = x & kPullRelMask;
+                const uint32_t tag = x >> kPullRelBits;
+                uint32_t wi = tag;
+                if (L.pull_fmt == kPullCount) {
+                  CHECK(tag % kPullCountBits == 0 && tag / kPullCountBits < kPullCountFields, "count shift %u", tag);
+                  wi = tag / kPullCountBits + 1;
+                }
+                if (x == pad_rel) wi = 0;
                 CHECK(wi < L.pull_wtab.size(), "weight index %u of %zu", wi, L.pull_wtab.size());
                 w = L.pull_wtab[wi];
               } else {
                 rel = L.pull_syn[e].x;
                 w = (int32_t)L.pull_syn[e].y;
               }
+              if (rel == pad_rel) {
+                CHECK(w == 0, "padding entry with weight %d", w);
+                continue;  // reads the zero word
+              }
+              CHECK(w != 0, "real entry with weight 0");
               CHECK((rel >> 6) < nw, "LDS bit read rel=%u nw=%u", rel, nw);
               CHECK((uint64_t)lo * 64 + rel < T, "source id %llu", (unsigned long long)lo * 64 + rel);
-              if ((bits[rel >> 6] >> (rel & 63)) & 1ull) {
+              const uint32_t word32 = (uint32_t)(bits[rel >> 6] >> (32 * ((rel >> 5) & 1)));
+              if ((word32 >> (rel & 31)) & 1u) {
                 if (w > 0) ae += (double)w;
                 if (w < 0) ai += (double)w;
                 if (w != 0) ++events;
@@ -266,9 +281,11 @@ static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D,
   std::sort(distinct.begin(), distinct.end());
   distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
   const bool narrow_expected = pull_layout == 0 && distinct.size() <= kPullMaxWeights;
-  CHECK((L.pull_wbits != 0) == narrow_expected, "narrow=%u expected %d", L.pull_wbits, (int)narrow_expected);
-  CHECK(L.pull_wbits == 0 || (L.pull_wtab.size() >= 2 && L.pull_wtab[0] == 0 &&
-                              L.pull_wtab.size() <= (1u << L.pull_wbits)), "weight table");
+  const uint32_t fmt_expected = !narrow_expected ? kPullWide
+                                : distinct.size() <= kPullCountFields ? kPullCount : kPullTable;
+  CHECK(L.pull_fmt == fmt_expected, "format %u expected %u", L.pull_fmt, fmt_expected);
+  CHECK(L.pull_fmt == kPullWide || (L.pull_wtab.size() == distinct.size() + 1 && L.pull_wtab[0] == 0),
+        "weight table");
   // random spike history up to step t - 1
   const uint64_t t = 3 * D + 2;
   std::vector<uint64_t> hist((size_t)R * W);
@@ -332,8 +349,9 @@ int main() {
                      {1000, 100, 32, 30000, true}, {5000, 10, 16, 60000, false}, {3000, 500, 1, 20000, true}};
   for (const C& c : cases) {
     check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 0, 0);   // continuous weights: wide unless few synapses
-    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 0);   // three weights: narrow entries
-    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 200, 0); // 200 weights: narrow, 8 index bits
+    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 0);   // three weights: count format
+    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 4, 0);   // four weights: table format
+    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 200, 0); // 200 weights: table format
     check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 1);   // wide entries asked for
   }
   // a network whose sources span more than the LDS budget gets no pull layout
