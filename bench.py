@@ -11,9 +11,9 @@ Prints one JSON line (the driver's contract) with:
                  weight + u32 target) + 8 B per delivered (source, delay)
                  row; `traffic` = HBM bytes per launch from the rocprofv3
                  FETCH_SIZE / WRITE_SIZE passes recorded in profiles/ (the
-                 pull layout stores 4-byte entries when the network has few
-                 distinct weights, so traffic is below the algorithmic
-                 bytes; hbm_gbs is traffic / launch time)
+                 pull layout stores 2-byte entries when the network has at
+                 most 3 distinct weights, so traffic is below the
+                 algorithmic bytes; hbm_gbs is traffic / launch time)
   cpu_baseline   the FP32 CPU restatement on the GPU run's timed steps, on
                  every host core this process may use (and a one-thread sample)
 """

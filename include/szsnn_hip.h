@@ -82,9 +82,9 @@ typedef struct szsnn_net_desc {
   uint32_t queue_capacity;    /* ignored: delivery goes through a ring of per-step accumulator
                                  slots and has no queue that could overflow (reference D4) */
   int32_t pull_layout;        /* 0: also build the target-blocked layout for dense gathers when the
-                                 network allows it (ids < 2^23, bounded source spans), with 4-byte
-                                 entries when it has at most 255 distinct weights; 1: the same with
-                                 8-byte entries always; -1: never */
+                                 network allows it (ids < 2^23, bounded source spans), with 2-byte
+                                 entries when it has at most 3 distinct weights, 4-byte entries up to
+                                 255; 1: the same with 8-byte entries always; -1: never */
 } szsnn_net_desc;
 
 typedef struct szsnn_net_info {
@@ -95,7 +95,7 @@ typedef struct szsnn_net_info {
   uint64_t pull_groups;       /* delay groups of the pull gather (0: no pull layout) */
   uint64_t device_bytes;      /* device memory held by the network */
   uint64_t raster_words;      /* u64 words per raster row = ceil((N+M)/64) */
-  uint64_t pull_entry_bytes;  /* bytes per synapse entry of the pull layout (4, 8; 0: none) */
+  uint64_t pull_entry_bytes;  /* bytes per synapse entry of the pull layout (2, 4, 8; 0: none) */
 } szsnn_net_info;
 
 /* Counters since upload (or the last szsnn_reset_counters). */

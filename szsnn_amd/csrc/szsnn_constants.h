@@ -16,24 +16,34 @@ constexpr uint32_t kPullMaxWeights = 255;        // distinct nonzero weights of 
 constexpr uint32_t kMaxPullGroups = 8;           // delay groups of the pull (work items per target block)
 
 // Pull entry formats.  Every entry locates its source in the block's staged
-// spike bitmap by rel = source - 64 * (first staged word), in the low
-// kPullRelBits bits: bit rel % 32 of 32-bit word rel / 32.  Padding entries
-// point at a zero word staged after the bitmap (rel = 64 * staged words).
+// spike bitmap by rel = source - 64 * (first staged word): bit rel % 32 of
+// 32-bit word rel / 32.
 //   wide:  8 bytes (rel, fixed-point weight);
-//   count: 4 bytes rel | shift << kPullRelBits, for networks with at most
-//          kPullCountFields distinct weights: the lane counts selected
-//          entries in kPullCountBits-bit fields of one register, the field
-//          of weight i (1-based) at shift kPullCountBits * (i - 1);
 //   table: 4 bytes rel | i << kPullRelBits, weight i (1-based) of a table of
-//          at most kPullMaxWeights.
+//          at most kPullMaxWeights;
+//   count: 2 bytes delta | tag << kPullDeltaBits, for networks with at most
+//          kPullCountFields distinct weights.  A lane's entries come in
+//          ascending source order and rel is the running sum of the deltas
+//          (from 0 at the start of each column).  Tag i < 3 is weight i + 1:
+//          the lane counts selected entries in kPullCountBits-bit fields of
+//          one register, tag i's field at shift kPullCountBits * i.  Tag 3
+//          is a skip (a gap wider than a delta, or padding with delta 0): its
+//          shift lands above the fields.
+// Wide and table padding entries point at a zero word staged after the
+// bitmap (rel = 64 * staged words), weight 0 or index 0.
 enum : uint32_t { kPullWide = 0, kPullCount = 1, kPullTable = 2 };
 constexpr uint32_t kPullRelBits = 21;
 constexpr uint32_t kPullRelMask = (1u << kPullRelBits) - 1;
 constexpr uint32_t kPullCountFields = 3;
 constexpr uint32_t kPullCountBits = 10;
+constexpr uint32_t kPullDeltaBits = 14;
+constexpr uint32_t kPullMaxDelta = (1u << kPullDeltaBits) - 1;
+constexpr uint16_t kPullSkipTag = 3;
 static_assert((uint64_t)(kPullMaxWords + 1) * 64 <= kPullRelMask, "rel of the zero word must fit");
 static_assert(kPullMaxWeights < (1u << (32 - kPullRelBits)), "weight index must fit");
 static_assert(kPullCountBits * kPullCountFields <= 32, "count fields must fit one register");
+static_assert(kPullCountBits * kPullSkipTag < 32 && kPullCountBits * kPullSkipTag >= kPullCountBits * kPullCountFields,
+              "the skip tag's shift lands above the count fields");
 
 // Push entries carry the API delay in the target word's top bits.
 constexpr uint32_t kSynDelayShift = 26;

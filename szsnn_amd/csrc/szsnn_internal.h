@@ -91,10 +91,11 @@ struct DevNet {
   const uint32_t* pull_off; // [NB*D*kPullSlices + 1] or null (no pull layout)
   const uint32_t* pull_lane;  // [NB*D*kPullTB] target | vectors << kPullShift
   const uint2* pull_syn;    // wide entries, or null
-  const uint32_t* pull_syn32;  // narrow (count, table) entries, or null
-  const int32_t* pull_wtab; // narrow: [pull_wn] weight of each index (index 0: 0)
+  const uint32_t* pull_syn32;  // table-format entries, or null
+  const uint16_t* pull_syn16;  // count-format entries, or null
+  const int32_t* pull_wtab; // count, table: [pull_wn] weight of each index (index 0: 0)
   uint32_t pull_fmt;        // kPullWide, kPullCount or kPullTable
-  uint32_t pull_wn;         // narrow: weight-table entries
+  uint32_t pull_wn;         // count, table: weight-table entries
   const uint32_t* pull_lo;  // [NB] first raster word of the block's sources
   uint32_t pull_nb;         // NB
   uint32_t pull_words;      // LDS words staged per block (max over blocks)

@@ -626,8 +626,9 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
 // and one bit-field extract give the source's spike bit b.  Sums are exact:
 // FP64 sums of integers below 2^53 (wide: E += b * max(w, 0), I += b *
 // min(w, 0); table: the (E, I) pair at 2 * index + b, zero for b = 0), or,
-// in the count format, per-weight counts of selected entries in bit fields
-// of one register, multiplied out once per slice.  The lane adds its sums
+// in the count format (2-byte entries: the source as a running sum of
+// deltas), per-weight counts of selected entries in bit fields of one
+// register, multiplied out once per column.  The lane adds its sums
 // into the block's LDS sums of its target (one lane per target and delay,
 // delays one after the other: no atomics); the block stores its targets'
 // sums for the group into pull slot g: plain stores, no global atomics.
@@ -637,7 +638,7 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
 template <int BLK, int FMT, bool AGENT>
 __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
   constexpr bool kNarrow = FMT != kPullWide;
-  constexpr uint32_t kV = kNarrow ? 4 : 2;  // entries per 16-byte vector
+  constexpr uint32_t kV = FMT == kPullWide ? 2 : FMT == kPullTable ? 4 : 8;  // entries per 16-byte vector
   constexpr uint32_t kB = 4;                // vectors per lane and batch
   constexpr uint32_t kWaves = BLK / 64;
   // count format: flush the fields before any can pass 2^kPullCountBits - 1
@@ -657,7 +658,9 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   const uint32_t nw = min(n.pull_words, n.W - lo);
   const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
   const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
-  const uint4* vec = kNarrow ? reinterpret_cast<const uint4*>(n.pull_syn32) : reinterpret_cast<const uint4*>(n.pull_syn);
+  const uint4* vec = FMT == kPullWide    ? reinterpret_cast<const uint4*>(n.pull_syn)
+                     : FMT == kPullTable ? reinterpret_cast<const uint4*>(n.pull_syn32)
+                                         : reinterpret_cast<const uint4*>(n.pull_syn16);
   const uint32_t lane = __lane_id();
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
   uint32_t events = 0;
@@ -734,6 +737,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
       const uint32_t nv = cnv;
       double ae = 0.0, ai = 0.0;
       uint32_t cnt = 0;  // count format: kPullCountBits-bit fields per weight
+      uint32_t rel = 0;  // count format: the running source offset
       // sums of the batch of vectors j0 .. j0 + kB - 1 held in qq
       auto sum_batch = [&](const uint4 (&qq)[kB], uint32_t j0) {
 #pragma unroll
@@ -741,22 +745,29 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
           if (j0 + u >= mine) break;
 #pragma unroll
           for (uint32_t c = 0; c < kV; ++c) {
-            const uint32_t e = comp(qq[u], kNarrow ? c : 2 * c);
-            const uint32_t word = bits[(e >> 5) & (kPullRelMask >> 5)];
-            const uint32_t bit = __builtin_amdgcn_ubfe(word, e, 1u);  // bit e % 32
-            if (FMT == kPullWide) {
-              const int32_t w = (int32_t)comp(qq[u], 2 * c + 1);
-              const double bw = (double)(bit ? w : 0);
-              ae += bw > 0.0 ? bw : 0.0;
-              ai += bw < 0.0 ? bw : 0.0;
-              events += bit;
-            } else if (FMT == kPullCount) {
-              cnt += bit << (e >> kPullRelBits);
+            if (FMT == kPullCount) {
+              // 16-bit entry c: delta, then the tag
+              const uint32_t x = comp(qq[u], c >> 1), h = 16 * (c & 1);
+              rel += __builtin_amdgcn_ubfe(x, h, kPullDeltaBits);
+              const uint32_t tag = __builtin_amdgcn_ubfe(x, h + kPullDeltaBits, 16 - kPullDeltaBits);
+              const uint32_t word = bits[(rel >> 5) & (kPullRelMask >> 5)];
+              cnt += __builtin_amdgcn_ubfe(word, rel, 1u) << (tag * kPullCountBits);  // bit rel % 32
             } else {
-              const double2 wv = wt[((e >> kPullRelBits) << 1) | bit];
-              ae += wv.x;
-              ai += wv.y;
-              events += bit;
+              const uint32_t e = comp(qq[u], FMT == kPullWide ? 2 * c : c);
+              const uint32_t word = bits[(e >> 5) & (kPullRelMask >> 5)];
+              const uint32_t bit = __builtin_amdgcn_ubfe(word, e, 1u);  // bit e % 32
+              if (FMT == kPullWide) {
+                const int32_t w = (int32_t)comp(qq[u], 2 * c + 1);
+                const double bw = (double)(bit ? w : 0);
+                ae += bw > 0.0 ? bw : 0.0;
+                ai += bw < 0.0 ? bw : 0.0;
+                events += bit;
+              } else {
+                const double2 wv = wt[((e >> kPullRelBits) << 1) | bit];
+                ae += wv.x;
+                ai += wv.y;
+                events += bit;
+              }
             }
           }
         }

@@ -53,8 +53,9 @@ struct HostLayout {
   std::vector<uint32_t> pull_lo;      // [NB] first staged raster word of each target block
   std::vector<uint32_t> pull_lane;    // [NB*D][kPullTB] slice lane: target (within the block) | vectors << kPullShift
   std::vector<LayoutPair> pull_syn;   // wide entries (8 bytes), or empty
-  std::vector<uint32_t> pull_syn32;   // narrow entries (4 bytes), or empty
-  std::vector<int32_t> pull_wtab;     // narrow: weight of each index (index 0: the inert 0)
+  std::vector<uint32_t> pull_syn32;   // table-format entries (4 bytes), or empty
+  std::vector<uint16_t> pull_syn16;   // count-format entries (2 bytes), or empty
+  std::vector<int32_t> pull_wtab;     // count, table: weight of each index (index 0: the inert 0)
   uint32_t pull_fmt = kPullWide;      // entry format (kPullWide, kPullCount, kPullTable)
   uint32_t pull_words = 0;
   uint64_t NB = 0;
@@ -123,13 +124,15 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
   // pull layout: per (target block, delay) segment, kPullSlices slices of 64
   // targets (targets sorted by their synapse count in the segment, so the
   // slices' lengths are close); slice s holds L_s entries per lane, stored
-  // as 16-byte vectors [L_s / V][64 lanes][V entries] (V = 4 narrow, 2 wide)
+  // as 16-byte vectors [L_s / V][64 lanes][V entries] (V = 8 count, 4 table,
+  // 2 wide)
   L.NB = (N + kPullTB - 1) / kPullTB;
   L.pull_off.clear();
   L.pull_lo.clear();
   L.pull_lane.clear();
   L.pull_syn.clear();
   L.pull_syn32.clear();
+  L.pull_syn16.clear();
   L.pull_wtab.clear();
   L.pull_fmt = kPullWide;
   L.pull_words = 0;
@@ -176,9 +179,38 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     L.error = "pull layout: source offset beyond the staged words";
     return false;
   }
-  // padding entries read the zero word staged after the bitmap
+  // padding entries of the wide and table formats read the zero word staged
+  // after the bitmap
   const uint32_t pad_rel = words * 64;
-  const uint32_t V = narrow ? 4 : 2;  // entries per 16-byte vector
+  const uint32_t fmt = !narrow ? kPullWide : wset.size() <= kPullCountFields ? kPullCount : kPullTable;
+  const uint32_t V = fmt == kPullWide ? 2 : fmt == kPullTable ? 4 : 8;  // entries per 16-byte vector
+  std::sort(wset.begin(), wset.end());
+  std::vector<int32_t> wtab(1, 0);  // index 0: the inert weight 0
+  wtab.insert(wtab.end(), wset.begin(), wset.end());
+  auto widx = [&](int32_t w) {
+    return (uint32_t)(std::lower_bound(wtab.begin() + 1, wtab.end(), w) - wtab.begin());
+  };
+  // count format: a gap wider than kPullMaxDelta takes skip entries first
+  auto skips = [](uint32_t gap) { return gap > kPullMaxDelta ? (gap - 1) / kPullMaxDelta : 0u; };
+  std::vector<uint32_t> prev;  // count format: last rel of each (segment, target)
+  if (fmt == kPullCount) {
+    // entries per (segment, target) with the skips; sources arrive in
+    // ascending order per target (id-major rows)
+    prev.assign(nseg * kPullTB, 0u);
+    std::fill(cnt.begin(), cnt.end(), 0u);
+    for (uint64_t id = 0; id < T; ++id)
+      for (uint64_t k = 0; k < D; ++k)
+        for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
+          const uint32_t tg = L.syn[q].x;
+          const uint64_t b = tg >> kPullShift;
+          const uint64_t key = (b * D + k) * kPullTB + (tg & (kPullTB - 1));
+          const uint32_t rel = (uint32_t)id - lo[b] * 64;
+          const uint64_t c = (uint64_t)cnt[key] + skips(rel - prev[key]) + 1;
+          if (c > 0xffffffffull) return true;
+          cnt[key] = (uint32_t)c;
+          prev[key] = rel;
+        }
+  }
   // target order per segment and slice offsets; each slice lane records its
   // target and how many of the slice's vectors hold its entries
   std::vector<uint16_t> perm(kPullTB);
@@ -193,14 +225,14 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     std::stable_sort(pm, pm + kPullTB, [&](uint16_t a, uint16_t b2) { return c[a] > c[b2]; });
     for (uint32_t sl = 0; sl < kPullSlices; ++sl) {
       off[sg * kPullSlices + sl] = (uint32_t)pos;
-      const uint32_t len = (c[pm[sl * 64]] + V - 1) / V * V;  // longest lane of the slice
+      const uint64_t len = ((uint64_t)c[pm[sl * 64]] + V - 1) / V * V;  // longest lane of the slice
       if (len / V >= (1u << (32 - kPullShift))) return true;
       for (uint32_t lane = 0; lane < 64; ++lane) {
         const uint32_t tl = pm[sl * 64 + lane];
         pos_of[sg * kPullTB + tl] = (uint32_t)(pos + lane * V);
-        lanes[sg * kPullTB + sl * 64 + lane] = tl | ((c[tl] + V - 1) / V) << kPullShift;
+        lanes[sg * kPullTB + sl * 64 + lane] = tl | (uint32_t)(((uint64_t)c[tl] + V - 1) / V) << kPullShift;
       }
-      pos += (uint64_t)len * 64;
+      pos += len * 64;
       if (pos > 0xffffffffull) return true;
     }
   }
@@ -211,25 +243,40 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     const uint32_t j = fill[key]++;
     return (uint64_t)pos_of[key] + (uint64_t)(j / V) * 64 * V + j % V;
   };
-  if (narrow) {
-    std::sort(wset.begin(), wset.end());
-    std::vector<int32_t> wtab(1, 0);
-    wtab.insert(wtab.end(), wset.begin(), wset.end());
-    const bool count = wset.size() <= kPullCountFields;
-    std::vector<uint32_t> ps(pos, pad_rel);  // padding: the zero word, tag 0
+  if (fmt == kPullCount) {
+    const uint16_t pad = (uint16_t)(kPullSkipTag << kPullDeltaBits);  // skip by 0
+    std::vector<uint16_t> ps(pos, pad);
+    std::fill(prev.begin(), prev.end(), 0u);
     for (uint64_t id = 0; id < T; ++id)
       for (uint64_t k = 0; k < D; ++k)
         for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
           const uint32_t tg = L.syn[q].x;
           const uint64_t b = tg >> kPullShift;
-          const uint32_t wi = (uint32_t)(std::lower_bound(wtab.begin() + 1, wtab.end(), (int32_t)L.syn[q].y) -
-                                         wtab.begin());
-          const uint32_t tag = count ? kPullCountBits * (wi - 1) : wi;
-          ps[place((b * D + k) * kPullTB + (tg & (kPullTB - 1)))] = ((uint32_t)id - lo[b] * 64) | tag << kPullRelBits;
+          const uint64_t key = (b * D + k) * kPullTB + (tg & (kPullTB - 1));
+          const uint32_t rel = (uint32_t)id - lo[b] * 64;
+          uint32_t gap = rel - prev[key];
+          for (uint32_t i = skips(gap); i > 0; --i) {
+            ps[place(key)] = (uint16_t)(kPullSkipTag << kPullDeltaBits | kPullMaxDelta);
+            gap -= kPullMaxDelta;
+          }
+          const uint32_t tag = widx((int32_t)L.syn[q].y) - 1;
+          ps[place(key)] = (uint16_t)(tag << kPullDeltaBits | gap);
+          prev[key] = rel;
+        }
+    L.pull_syn16.swap(ps);
+    L.pull_wtab.swap(wtab);
+  } else if (fmt == kPullTable) {
+    std::vector<uint32_t> ps(pos, pad_rel);  // padding: the zero word, index 0
+    for (uint64_t id = 0; id < T; ++id)
+      for (uint64_t k = 0; k < D; ++k)
+        for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
+          const uint32_t tg = L.syn[q].x;
+          const uint64_t b = tg >> kPullShift;
+          ps[place((b * D + k) * kPullTB + (tg & (kPullTB - 1)))] =
+              ((uint32_t)id - lo[b] * 64) | widx((int32_t)L.syn[q].y) << kPullRelBits;
         }
     L.pull_syn32.swap(ps);
     L.pull_wtab.swap(wtab);
-    L.pull_fmt = count ? kPullCount : kPullTable;
   } else {
     std::vector<LayoutPair> ps(pos, LayoutPair{pad_rel, 0u});  // padding: the zero word, weight 0
     for (uint64_t id = 0; id < T; ++id)
@@ -241,6 +288,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
         }
     L.pull_syn.swap(ps);
   }
+  L.pull_fmt = fmt;
   L.pull_off.swap(off);
   L.pull_lo.swap(lo);
   L.pull_lane.swap(lanes);

@@ -440,14 +440,16 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint32_t *d_pull_off = nullptr, *d_pull_lo = nullptr;
   uint2* d_pull_syn = nullptr;
   uint32_t* d_pull_syn32 = nullptr;
+  uint16_t* d_pull_syn16 = nullptr;
   int32_t* d_pull_wtab = nullptr;
   uint32_t* d_pull_lane = nullptr;
   if (rc == SZSNN_OK) guard(dev_alloc(net, &hist, (size_t)n.R * n.W));
   if (rc == SZSNN_OK && !pull_off.empty())
     guard(dev_alloc(net, &d_pull_off, pull_off.size())) &&
-        (L.pull_fmt != kPullWide ? guard(dev_alloc(net, &d_pull_syn32, L.pull_syn32.size())) &&
-                            guard(dev_alloc(net, &d_pull_wtab, L.pull_wtab.size()))
-                      : guard(dev_alloc(net, &d_pull_syn, pull_syn_size))) &&
+        (L.pull_fmt == kPullWide    ? guard(dev_alloc(net, &d_pull_syn, pull_syn_size))
+         : L.pull_fmt == kPullTable ? guard(dev_alloc(net, &d_pull_syn32, L.pull_syn32.size()))
+                                    : guard(dev_alloc(net, &d_pull_syn16, L.pull_syn16.size()))) &&
+        (L.pull_fmt == kPullWide || guard(dev_alloc(net, &d_pull_wtab, L.pull_wtab.size()))) &&
         guard(dev_alloc(net, &d_pull_lo, pull_lo.size())) && guard(dev_alloc(net, &d_pull_lane, L.pull_lane.size())) &&
         guard(dev_alloc(net, &acc_pull, (size_t)groups * 2 * n.npad));
   if (rc != SZSNN_OK) {
@@ -487,9 +489,10 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   if (rc == SZSNN_OK && drive_cdf) guard(upload(net, drive_cdf, d->drive_cdf, d->drive_k));
   if (rc == SZSNN_OK && d_pull_off)
     guard(upload(net, d_pull_off, pull_off.data(), pull_off.size())) &&
-        (L.pull_fmt != kPullWide ? guard(upload(net, d_pull_syn32, L.pull_syn32.data(), L.pull_syn32.size())) &&
-                            guard(upload(net, d_pull_wtab, L.pull_wtab.data(), L.pull_wtab.size()))
-                      : guard(upload(net, d_pull_syn, pull_syn_data, pull_syn_size))) &&
+        (L.pull_fmt == kPullWide    ? guard(upload(net, d_pull_syn, pull_syn_data, pull_syn_size))
+         : L.pull_fmt == kPullTable ? guard(upload(net, d_pull_syn32, L.pull_syn32.data(), L.pull_syn32.size()))
+                                    : guard(upload(net, d_pull_syn16, L.pull_syn16.data(), L.pull_syn16.size()))) &&
+        (L.pull_fmt == kPullWide || guard(upload(net, d_pull_wtab, L.pull_wtab.data(), L.pull_wtab.size()))) &&
         guard(upload(net, d_pull_lo, pull_lo.data(), pull_lo.size())) &&
         guard(upload(net, d_pull_lane, L.pull_lane.data(), L.pull_lane.size()));
   if (rc == SZSNN_OK) {
@@ -542,6 +545,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.pull_lane = d_pull_lane;
   n.pull_syn = d_pull_syn;
   n.pull_syn32 = d_pull_syn32;
+  n.pull_syn16 = d_pull_syn16;
   n.pull_wtab = d_pull_wtab;
   n.pull_fmt = L.pull_fmt;
   n.pull_wn = (uint32_t)L.pull_wtab.size();
@@ -560,7 +564,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   net->info.n_syn_stored = S;
   net->info.n_rows = nrows;
   net->info.pull_groups = d_pull_off ? groups : 0;
-  net->info.pull_entry_bytes = d_pull_off ? (L.pull_fmt != kPullWide ? 4 : 8) : 0;
+  net->info.pull_entry_bytes = !d_pull_off ? 0 : L.pull_fmt == kPullWide ? 8 : L.pull_fmt == kPullTable ? 4 : 2;
   net->info.raster_words = n.W;
   net->tiling.gather_grid = default_gather_grid(net, net->tiling.gather_block);
   *out = net;

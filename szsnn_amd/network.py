@@ -45,7 +45,7 @@ class Network:
     drive_cdf: np.ndarray = None  # uint64 [K]
     drive_w: float = 0.0
     queue_capacity: int = 0
-    pull_layout: int = 0          # 0: dense-gather layout when possible (4-byte entries when allowed); 1: 8-byte entries; -1: never
+    pull_layout: int = 0          # 0: dense-gather layout when possible (2- or 4-byte entries when allowed); 1: 8-byte entries; -1: never
     meta: dict = field(default_factory=dict)
 
     @property

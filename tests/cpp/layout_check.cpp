@@ -143,10 +143,11 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
   const uint32_t words = L.pull_words;
   std::vector<uint64_t> bits(words);
   std::vector<double> sums(2 * kPullTB);
-  const bool narrow = L.pull_fmt != kPullWide;
-  const uint32_t V = narrow ? 4 : 2;
+  const uint32_t fmt = L.pull_fmt;
+  const uint32_t V = fmt == kPullWide ? 2 : fmt == kPullTable ? 4 : 8;
   const uint32_t pad_rel = words * 64;
-  const size_t nent = narrow ? L.pull_syn32.size() : L.pull_syn.size();
+  const uint16_t skip0 = (uint16_t)(kPullSkipTag << kPullDeltaBits);
+  const size_t nent = fmt == kPullWide ? L.pull_syn.size() : fmt == kPullTable ? L.pull_syn32.size() : L.pull_syn16.size();
   CHECK(L.pull_off.size() == (size_t)L.NB * D * kPullSlices + 1, "pull_off size");
   CHECK(L.pull_lane.size() == (size_t)L.NB * D * kPullTB, "pull_lane size");
   CHECK(L.pull_off.back() == nent, "entries %zu vs offsets %u", nent, L.pull_off.back());
@@ -191,26 +192,32 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
           for (uint32_t j = mine; j < nv; ++j)
             for (uint32_t c = 0; c < V; ++c) {
               const uint64_t e = (uint64_t)(s0 / V + 64 * j + lane) * V + c;
-              const bool pad = narrow ? L.pull_syn32[e] == pad_rel : (L.pull_syn[e].x == pad_rel && L.pull_syn[e].y == 0);
+              const bool pad = fmt == kPullWide    ? (L.pull_syn[e].x == pad_rel && L.pull_syn[e].y == 0)
+                               : fmt == kPullTable ? L.pull_syn32[e] == pad_rel
+                                                   : L.pull_syn16[e] == skip0;
               CHECK(pad, "real entry beyond the lane's vectors");
             }
           double ae = 0.0, ai = 0.0;
+          uint32_t run = 0;  // count format: running sum of the deltas
           for (uint32_t j = 0; j < mine; ++j)
             for (uint32_t c = 0; c < V; ++c) {
               const uint64_t e = (uint64_t)(s0 / V + 64 * j + lane) * V + c;
               CHECK(e < s1, "entry beyond the slice");
               uint32_t rel;
               int32_t w;
-              if (narrow) {
+              if (fmt == kPullCount) {
+                const uint16_t x = L.pull_syn16[e];
+                run += x & kPullMaxDelta;
+                const uint32_t tag = x >> kPullDeltaBits;
+                CHECK((run >> 6) < nw, "LDS bit read rel=%u nw=%u", run, nw);
+                if (tag == kPullSkipTag) continue;  // gap or padding: no synapse
+                CHECK(tag < kPullCountFields && tag + 1 < L.pull_wtab.size(), "count tag %u", tag);
+                rel = run;
+                w = L.pull_wtab[tag + 1];
+              } else if (fmt == kPullTable) {
                 const uint32_t x = L.pull_syn32[e];
                 rel = x & kPullRelMask;
-                const uint32_t tag = x >> kPullRelBits;
-                uint32_t wi = tag;
-                if (L.pull_fmt == kPullCount) {
-                  CHECK(tag % kPullCountBits == 0 && tag / kPullCountBits < kPullCountFields, "count shift %u", tag);
-                  wi = tag / kPullCountBits + 1;
-                }
-                if (x == pad_rel) wi = 0;
+                const uint32_t wi = x == pad_rel ? 0u : x >> kPullRelBits;
                 CHECK(wi < L.pull_wtab.size(), "weight index %u of %zu", wi, L.pull_wtab.size());
                 w = L.pull_wtab[wi];
               } else {
@@ -346,7 +353,9 @@ int main() {
   struct C { uint32_t N, M, D; uint64_t S; bool gaps; };
   const C cases[] = {{1, 0, 1, 5, false},     {63, 2, 5, 400, false},   {64, 0, 3, 500, false},
                      {255, 7, 64, 3000, false}, {256, 0, 2, 2000, true}, {257, 31, 9, 4000, false},
-                     {1000, 100, 32, 30000, true}, {5000, 10, 16, 60000, false}, {3000, 500, 1, 20000, true}};
+                     {1000, 100, 32, 30000, true}, {5000, 10, 16, 60000, false}, {3000, 500, 1, 20000, true},
+                     // sparse columns over wide source ranges: count-format gaps beyond one delta (skips)
+                     {600, 60000, 4, 3000, false}, {300, 390000, 2, 2000, false}};
   for (const C& c : cases) {
     check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 0, 0);   // continuous weights: wide unless few synapses
     check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 0);   // three weights: count format

@@ -76,7 +76,7 @@ def test_largest_delay_ring_ragged_ids(ctx, fraction):
     steps, volley = 140, 600
     print(f"case: largest delay ring, policy {fraction}", flush=True)
     r, v, c, s = _run(ctx, net, steps, fraction=fraction, volley=volley)
-    assert s.info.pull_entry_bytes == 4 and s.info.max_delay == 64
+    assert s.info.pull_entry_bytes == 2 and s.info.max_delay == 64
     forced = [(t, i) for t in range(steps) for i in range(volley)]
     o = Oracle(net, 32)
     ro, _ = o.run(steps, forced=forced)

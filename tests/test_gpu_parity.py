@@ -372,35 +372,40 @@ def test_persistent_mode_pull_volley_equals_graph(ctx):
     assert runs[0][2]["events"] == runs[1][2]["events"] > 0
 
 
-@pytest.mark.parametrize("which", ["count", "table"])
+@pytest.mark.parametrize("which", ["count", "count_skips", "table"])
 def test_narrow_and_wide_pull_entries_agree(ctx, which):
-    """The same network pulled through 4-byte entries (weight table: few
-    weights counted per index, or up to 255 looked up) and through 8-byte
-    entries: rasters, state and event counts identical to pushing every step;
-    the pulled table network also matches the FP32 oracle."""
+    """The same network pulled through narrow entries (2-byte count format:
+    few weights, sources as running deltas, gaps wider than a delta taken by
+    skip entries; or the 4-byte table format, up to 255 weights looked up)
+    and through 8-byte entries: rasters, state and event counts identical to
+    pushing every step; the pulled table network also matches the FP32 oracle."""
     import dataclasses
     from szsnn_amd import configs
     from szsnn_amd.simulation import Simulation
     if which == "count":
         net, steps, volley = configs.if_cond_exp_100k(n_exc=4000, n_inh=1000, k_exc=200, k_inh=50), 70, 1500
+    elif which == "count_skips":
+        # 50000 sources, about 8 per (target, delay): gaps beyond 2^14 - 1 are common
+        net, steps, volley = configs.if_cond_exp_100k(n_exc=40000, n_inh=10000, k_exc=200, k_inh=50), 40, 12000
     else:
         net, steps, volley = configs.izhikevich2003(), 300, 0
         w = net.syn_w.copy()
         net = dataclasses.replace(net, syn_w=(np.round(w * 25.0) / 25.0).astype(np.float32))
         assert 20 < np.unique(net.syn_w[net.syn_w != 0]).size <= 255
     print(f"case: narrow vs wide pull entries ({which})", flush=True)
-    for layout, width in ((0, 4), (1, 8)):
+    narrow_bytes = 4 if which == "table" else 2
+    for layout, width in ((0, narrow_bytes), (1, 8)):
         s = Simulation(ctx, dataclasses.replace(net, pull_layout=layout))
         assert s.info.pull_entry_bytes == width
         del s
     rp, vp, cp = _run_policy(ctx, net, steps, 2.0, volley)                                    # push only
-    r4, v4, c4 = _run_policy(ctx, net, steps, 0.0, volley)                                    # pull, 4 B
+    rn, vn, cn = _run_policy(ctx, net, steps, 0.0, volley)                                    # pull, narrow
     r8, v8, c8 = _run_policy(ctx, dataclasses.replace(net, pull_layout=1), steps, 0.0, volley)  # pull, 8 B
-    for r, v, c in ((r4, v4, c4), (r8, v8, c8)):
+    for r, v, c in ((rn, vn, cn), (r8, v8, c8)):
         np.testing.assert_array_equal(r, rp)
         np.testing.assert_array_equal(v, vp)
         assert c["events"] == cp["events"] > 0
     if volley == 0:
         o, ro, _ = _oracle_run(net, steps, 32)
-        np.testing.assert_array_equal(r4, ro)
-        assert c4["events"] == o.events
+        np.testing.assert_array_equal(rn, ro)
+        assert cn["events"] == o.events
