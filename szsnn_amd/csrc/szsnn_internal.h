@@ -183,6 +183,11 @@ hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base,
 int persistent_max_blocks_per_cu(const DevNet& n, uint32_t block);
 // dynamic LDS bytes of a gather launch (pull path)
 size_t pull_dyn_bytes(const DevNet& n);
+// The same for a layout of `words` staged bitmap words, entry format `fmt`
+// and a weight table of `wn` entries.
+size_t pull_dyn_bytes_for(uint32_t words, uint32_t fmt, uint32_t wn);
+// Resident gather_kernel<block> blocks per CU with `dyn` bytes of LDS (0 on error).
+int gather_max_blocks_per_cu(size_t dyn, uint32_t block);
 hipError_t launch_advance(uint64_t* t_base, uint64_t k, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
 

@@ -639,7 +639,7 @@ template <int BLK, int FMT, bool AGENT>
 __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
   constexpr bool kNarrow = FMT != kPullWide;
   constexpr uint32_t kV = FMT == kPullWide ? 2 : FMT == kPullTable ? 4 : 8;  // entries per 16-byte vector
-  constexpr uint32_t kB = 4;                // vectors per lane and batch
+  constexpr uint32_t kB = 2;                // vectors per lane and batch
   constexpr uint32_t kWaves = BLK / 64;
   // count format: flush the fields before any can pass 2^kPullCountBits - 1
   constexpr uint32_t kFlushVectors = ((1u << kPullCountBits) - 1) / kV / kB * kB;
@@ -842,8 +842,10 @@ __device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32
 // gather(t): the pull of the deferred steps whose synapses are due at t, one
 // (target block, delay group) work item per block iteration.  Exits at once
 // when nothing is due (every block reads the same stamps).
+// At most 72 VGPRs (7 waves per SIMD): as many 256-thread blocks as the LDS
+// of a typical pull block admits stay resident.
 template <int BLK>
-__global__ void __launch_bounds__(BLK) gather_kernel(DevNet n, uint32_t j) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(7))) gather_kernel(DevNet n, uint32_t j) {
   extern __shared__ __align__(16) uint64_t dyn[];
   const uint64_t t = *n.t_base + j;
   const uint64_t due = due_mask<false>(n, t);
@@ -1046,10 +1048,22 @@ hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_
   return hipGetLastError();
 }
 
+size_t pull_dyn_bytes_for(uint32_t words, uint32_t fmt, uint32_t wn) {
+  return (size_t)pull_bitmap_u64(words) * 8 + (size_t)2 * kPullTB * 8 + (fmt != kPullWide ? (size_t)32 * wn : 0);
+}
+
 size_t pull_dyn_bytes(const DevNet& n) {
-  return n.pull_off ? (size_t)pull_bitmap_u64(n.pull_words) * 8 + (size_t)2 * kPullTB * 8 +
-                          (n.pull_fmt != kPullWide ? (size_t)32 * n.pull_wn : 0)
-                    : 0;
+  return n.pull_off ? pull_dyn_bytes_for(n.pull_words, n.pull_fmt, n.pull_wn) : 0;
+}
+
+int gather_max_blocks_per_cu(size_t dyn, uint32_t block) {
+  int nb = 0;
+  const void* f = block == 128   ? reinterpret_cast<const void*>(&gather_kernel<128>)
+                  : block == 256 ? reinterpret_cast<const void*>(&gather_kernel<256>)
+                  : block == 512 ? reinterpret_cast<const void*>(&gather_kernel<512>)
+                                 : reinterpret_cast<const void*>(&gather_kernel<1024>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)block, dyn) != hipSuccess) return 0;
+  return nb;
 }
 
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {

@@ -414,11 +414,20 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   float *params = nullptr, *vars = nullptr, *noise = nullptr, *src_rate = nullptr, *src_last = nullptr;
   long long *acc = nullptr, *acc_pull = nullptr;
   uint32_t* barrier = nullptr;
-  // delay groups of the pull gather: enough (block, group) work items to fill the chip
+  // delay groups of the pull gather: as many (target block, group) work items
+  // as one round of resident gather blocks (default block size) holds, the
+  // delays split evenly between the groups where that keeps at least half
+  // of them (equal items end together)
   uint32_t groups = 1;
   if (!pull_off.empty()) {
-    const uint64_t want = ((uint64_t)4 * std::max(ctx->cu_count, 1) + NB - 1) / std::max<uint64_t>(NB, 1);
-    groups = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, (uint64_t)D, (uint64_t)kMaxPullGroups}));
+    const int per_cu =
+        gather_max_blocks_per_cu(pull_dyn_bytes_for(L.pull_words, L.pull_fmt, (uint32_t)L.pull_wtab.size()), 256);
+    const uint64_t slots = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(ctx->cu_count, 1);
+    const uint64_t want =
+        std::max<uint64_t>(1, std::min<uint64_t>({slots / std::max<uint64_t>(NB, 1), (uint64_t)D, (uint64_t)kMaxPullGroups}));
+    uint64_t even = want;
+    while (D % even) --even;
+    groups = (uint32_t)(2 * even >= want ? even : want);
   }
   uint64_t *keys = nullptr, *src_thresh = nullptr, *drive_cdf = nullptr, *dbits = nullptr, *dstamp = nullptr;
   uint32_t* soff = nullptr;
