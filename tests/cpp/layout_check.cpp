@@ -363,6 +363,55 @@ int main() {
     check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 200, 0); // 200 weights: table format
     check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 1);   // wide entries asked for
   }
+  // count format at the delta boundaries: one target whose sources sit at
+  // gaps 0 (duplicate), 16383 (one delta), 16384, 32766 (one skip), 32767,
+  // 32768 (two skips); the lane's decoded sources and weights must come back
+  {
+    const uint32_t gaps[] = {0, 0, 16383, 16384, 32766, 32767, 32768};
+    Net n{1, 0, 1, {}, {}, {}, {}};
+    uint32_t id = 5;
+    uint64_t want_entries = 0;
+    for (uint32_t i = 0; i < sizeof(gaps) / sizeof(gaps[0]); ++i) {
+      id += gaps[i];
+      n.src.push_back(id);
+      n.tgt.push_back(0);
+      n.w.push_back(i % 2 ? 0.25f : -0.5f);
+      n.dl.push_back(0);
+      const uint32_t gap = i == 0 ? id : gaps[i];  // the first delta counts from rel 0 (block base 0)
+      want_entries += 1 + (gap > kPullMaxDelta ? (gap - 1) / kPullMaxDelta : 0);
+    }
+    n.M = id + 1;
+    const szsnn_net_desc d = desc_of(n);
+    HostLayout L;
+    CHECK(build_layout(d, L), "build: %s", L.error.c_str());
+    CHECK(L.pull_fmt == kPullCount && L.pull_lo.size() == 1 && L.pull_lo[0] == 0, "count format expected");
+    uint32_t tl = kPullTB, mine = 0, lane = 0, sl = 0;
+    for (uint32_t i = 0; i < kPullTB; ++i)
+      if ((L.pull_lane[i] & (kPullTB - 1)) == 0) {
+        tl = 0;
+        sl = i / 64;
+        lane = i % 64;
+        mine = L.pull_lane[i] >> kPullShift;
+      }
+    CHECK(tl == 0 && mine == (want_entries + 7) / 8, "lane of target 0: %u vectors for %llu entries", mine,
+          (unsigned long long)want_entries);
+    std::vector<std::pair<uint32_t, int32_t>> got;
+    uint32_t run = 0, seen = 0;
+    for (uint32_t j = 0; j < mine; ++j)
+      for (uint32_t c = 0; c < 8; ++c) {
+        const uint16_t x = L.pull_syn16[(size_t)(L.pull_off[sl] / 8 + 64 * j + lane) * 8 + c];
+        run += x & kPullMaxDelta;
+        ++seen;
+        if ((x >> kPullDeltaBits) != kPullSkipTag) got.emplace_back(run, L.pull_wtab[(x >> kPullDeltaBits) + 1]);
+      }
+    CHECK(got.size() == n.src.size(), "decoded %zu of %zu sources", got.size(), n.src.size());
+    for (size_t i = 0; i < got.size(); ++i) {
+      const int32_t wq = (int32_t)std::nearbyint(std::ldexp((double)n.w[i], d.frac_bits));
+      CHECK(got[i].first == n.src[i] && got[i].second == wq, "entry %zu: source %u weight %d", i, got[i].first,
+            got[i].second);
+    }
+    CHECK(seen >= want_entries && seen - want_entries < 8, "entries %u for %llu", seen, (unsigned long long)want_entries);
+  }
   // a network whose sources span more than the LDS budget gets no pull layout
   {
     Net n{200, 800000, 1, {0, 799999}, {0, 1}, {0.5f, -0.5f}, {0, 0}};
