@@ -141,7 +141,7 @@ class SpikeRecorder:
 
     def record_cycle_spikes(self, cycle, ids):
         cycle = int(cycle)
-        if cycle < self.start or (self.stop and cycle >= self.stop):
+        if cycle < self.start or (self.stop and cycle > self.stop):  # the stop cycle is recorded
             return
         if cycle in self.by_cycle:
             raise FormatError(f"cycle {cycle} already recorded")

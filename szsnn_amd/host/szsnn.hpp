@@ -11,7 +11,14 @@
 //   Simulation<floatT>              szsnn1.1/src/Simulation.hpp:39-131 (step,
 //                                   stepAndFired, getTime)
 //   Connection<floatT>              szsnn1.1/src/Connection.hpp:76-174 (.conn files)
-//   SpikeRecorder                   szsnn1.1/src/SpikeRecorder.cpp:13-104
+//   SpikeRecorder                   szsnn1.1/src/SpikeRecorder.cpp:13-104 (written
+//                                   as it records)
+//   construction::Population<M>     szsnn1.1/src/Population.hpp:14-397 (population
+//                                   and template files)
+//   construction::NetworkCreator    szsnn1.1/src/NetworkCreator.hpp:21-188
+//   construction::Connection        the Connection above
+//   record::SpikeRecorder           szsnn1.1/src/SpikeRecorder.{hpp,cpp} (in
+//                                   memory, saved by cycles or by neurons)
 //   SzsnnException                  szsnn1.1/src/SzsnnException.hpp:21-41
 // Every negative return code of the C ABI becomes an SzsnnException.
 //
@@ -26,8 +33,10 @@
 #include <cmath>
 #include <cstdint>
 #include <fstream>
+#include <iomanip>
 #include <iterator>
 #include <map>
+#include <set>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -66,6 +75,8 @@ class Context {
     check(szsnn_create(device, &c), nullptr, "Context::setup()");
     h_ = std::shared_ptr<szsnn_ctx>(c, [](szsnn_ctx* p) { szsnn_destroy(p); });
   }
+  // Context::setup (Context.cpp:136-224): the device was chosen by the constructor
+  void setup() {}
   szsnn_ctx* handle() const { return h_.get(); }
   std::string getDeviceName() const {
     char buf[256];
@@ -121,6 +132,7 @@ class ModelBase {
 template <typename floatT>
 class IF_curr_exp_NeuronModel : public ModelBase<floatT, SZSNN_MODEL_IF_CURR_EXP, 9, 4> {
  public:
+  static const char* name() { return "IF_curr_exp_NeuronModel"; }
   enum Params { p_V_REST = 0, p_CM, p_TAU_M, p_TAU_REFRAC, p_TAU_SYN_E, p_TAU_SYN_I, p_I_OFFSET, p_V_RESET, p_V_THRESH };
   enum Vars { v_V = 0, v_IE, v_II, v_FIRED };
   IF_curr_exp_NeuronModel() {
@@ -135,6 +147,7 @@ class IF_curr_exp_NeuronModel : public ModelBase<floatT, SZSNN_MODEL_IF_CURR_EXP
 template <typename floatT>
 class IF_cond_exp_NeuronModel : public ModelBase<floatT, SZSNN_MODEL_IF_COND_EXP, 11, 4> {
  public:
+  static const char* name() { return "IF_cond_exp_NeuronModel"; }
   enum Params { p_V_REST = 0, p_CM, p_TAU_M, p_TAU_REFRAC, p_TAU_SYN_E, p_TAU_SYN_I, p_I_OFFSET, p_V_RESET, p_V_THRESH,
                 p_E_REV_E, p_E_REV_I };
   enum Vars { v_V = 0, v_GE, v_GI, v_FIRED };
@@ -149,6 +162,7 @@ class IF_cond_exp_NeuronModel : public ModelBase<floatT, SZSNN_MODEL_IF_COND_EXP
 template <typename floatT>
 class Izhikevich_NeuronModel : public ModelBase<floatT, SZSNN_MODEL_IZHIKEVICH, 4, 2> {
  public:
+  static const char* name() { return "Izhikevich_NeuronModel"; }
   enum Params { p_A = 0, p_B, p_C, p_D };
   enum Vars { v_V = 0, v_U };
   Izhikevich_NeuronModel() {
@@ -162,6 +176,7 @@ class Izhikevich_NeuronModel : public ModelBase<floatT, SZSNN_MODEL_IZHIKEVICH, 
 template <typename floatT>
 class Const_rate_InputModel : public ModelBase<floatT, SZSNN_SOURCE_CONST_RATE, 1, 1> {
  public:
+  static const char* name() { return "Const_rate_InputModel"; }
   enum Params { p_RATE = 0 };
   enum Vars { v_LAST = 0 };
   Const_rate_InputModel() {
@@ -175,6 +190,7 @@ class Const_rate_InputModel : public ModelBase<floatT, SZSNN_SOURCE_CONST_RATE, 
 template <typename floatT>
 class Poisson_InputModel : public ModelBase<floatT, SZSNN_SOURCE_POISSON, 1, 0> {
  public:
+  static const char* name() { return "Poisson_InputModel"; }
   enum Params { p_PROB = 0 };
   Poisson_InputModel() { this->params[0] = 0; }
 };
@@ -210,6 +226,19 @@ class NetworkGraph {
     inputSynapses.emplace_back();
     return (unsigned)inputs.size() - 1;
   }
+  unsigned addInput(size_t idx, const InputModelT& in) {
+    if (idx < inputs.size()) {
+      inputs[idx] = in;
+      return (unsigned)inputs.size() - 1;
+    }
+    if (idx > inputs.size()) {
+      std::ostringstream os;
+      os << "SZSNN::NetworkGraph::addInput(idx, input) error: cant add input with ID=`" << idx
+         << "`, ID's must be consistance";
+      throw SzsnnException(os.str());
+    }
+    return addInput(in);
+  }
   void addSynapse(const Synapse<float_type>& s) {
     if (s.sourceNeuronIdx >= neurons.size() || s.targetNeuronIdx >= neurons.size()) {
       std::ostringstream os;
@@ -237,6 +266,23 @@ class NetworkGraph {
   size_t getInputCount() const { return inputs.size(); }
   unsigned getInputsIDOffset() const { return (unsigned)neurons.size(); }
   unsigned getMaxDelay() const { return maxDelay_; }
+  // debug listings (NetworkGraph.ipp:341-366)
+  std::string neuronsToString() const {
+    std::ostringstream ss;
+    for (size_t i = 0; i < neurons.size(); ++i) ss << "Idx [ " << i << " ], param1 = " << neurons[i].getParam(0) << "\n";
+    return ss.str();
+  }
+  std::string synapsesToString() const {
+    std::ostringstream ss;
+    for (size_t i = 0; i < synapses.size(); ++i) {
+      ss << "Idx [ " << i << " ] ";
+      for (size_t j = 0; j < synapses[i].size(); ++j)
+        ss << j << "[ " << synapses[i][j].sourceNeuronIdx << " -(" << synapses[i][j].weight << ", "
+           << synapses[i][j].delay << ")-> " << synapses[i][j].targetNeuronIdx << "]   ";
+      ss << "\n";
+    }
+    return ss.str();
+  }
 
   std::vector<NeuronModelT> neurons;
   std::vector<InputModelT> inputs;
@@ -413,6 +459,11 @@ class Connection {
       out << kv.first.first << "\t" << kv.first.second << "\t" << kv.second.first << "\t"
           << (float)kv.second.second << "\n";
   }
+  typedef typename std::map<std::pair<unsigned, unsigned>, std::pair<floatT, unsigned>>::const_iterator iterator;
+  iterator begin() const { return connections.begin(); }
+  iterator end() const { return connections.end(); }
+  unsigned getFromStartID() const { return from_; }
+  unsigned getToStartID() const { return to_; }
   template <typename N, typename I>
   void applyTo(NetworkGraph<N, I>& g) const {
     for (const auto& kv : connections) {
@@ -441,7 +492,7 @@ class SpikeRecorder {
     out_ << "\n";
   }
   void recordCycleSpikes(size_t cycle, const std::vector<unsigned>& spikes) {
-    if (cycle < start_ || (stop_ && cycle >= stop_)) return;
+    if (cycle < start_ || (stop_ && cycle > stop_)) return;  // the stop cycle is recorded
     if (have_last_ && cycle <= last_)
       throw SzsnnException("SZSNN::SpikeRecorder::recordCycleSpikes error: cycle already recorded");
     for (size_t i = 0; i < spikes.size(); ++i) {
@@ -459,5 +510,332 @@ class SpikeRecorder {
   size_t last_ = 0;
   bool have_last_ = false;
 };
+
+
+// ------------------------------------------------------------------ construction
+namespace construction {
+
+using szsnn::Connection;
+
+// Population<NeuronModelT> (Population.hpp:14-397): `size` neurons with ids
+// first .. first + size - 1.  Files: `#` comment lines, then the size, the
+// first id, the label (one word), then one parameter line and one variable
+// line per neuron (the template file: one pair for all neurons).  The
+// reference's writer left out the label its reader expects; the files
+// written here have it, so they read back.
+template <typename NeuronModelT>
+class Population {
+ public:
+  typedef float floatT;
+  typedef typename std::vector<NeuronModelT>::iterator iterator;
+  typedef typename std::vector<NeuronModelT>::const_iterator const_iterator;
+  static constexpr unsigned param_count = NeuronModelT::param_count;
+  static constexpr unsigned var_count = NeuronModelT::var_count;
+
+  Population(size_t size, size_t firstNeuronID, std::string label = "")
+      : firstNeuronID_(firstNeuronID), label_(std::move(label)), neurons_(size) {
+    if (!size) {
+      std::ostringstream os;
+      os << "SZSNN::CONSTRUCTION::Population() error: Population size=" << size << ", must be > 0";
+      throw SzsnnException(os.str());
+    }
+  }
+  Population() = default;
+
+  void clear() {
+    firstNeuronID_ = 0;
+    label_.clear();
+    neurons_.clear();
+  }
+  // params[param_count], vars[var_count]
+  void setNeurons(const floatT* params, const floatT* vars) {
+    for (auto& n : neurons_) n.setAll(params, vars);
+  }
+  void setNeuronParameter(unsigned long idx, unsigned param_idx, floatT val) { at(idx).setParam(param_idx, val); }
+  void setNeuronParameters(unsigned param_idx, floatT val) {
+    for (auto& n : neurons_) n.setParam(param_idx, val);
+  }
+  void setNeuronVariables(unsigned var_idx, floatT val) {
+    for (auto& n : neurons_) n.setVar(var_idx, val);
+  }
+
+  void loadFromFile(const std::string& filename) { load(filename, false, "loadFromFile"); }
+  void loadFromTemplateFile(const std::string& filename) { load(filename, true, "loadFromTemplateFile"); }
+  void saveToFile(const std::string& filename) const { save(filename, false, "saveToFile"); }
+  void saveToTemplateFile(const std::string& filename) const { save(filename, true, "saveToTemplateFile"); }
+
+  iterator begin() { return neurons_.begin(); }
+  iterator end() { return neurons_.end(); }
+  const_iterator begin() const { return neurons_.begin(); }
+  const_iterator end() const { return neurons_.end(); }
+  size_t getFirstId() const { return firstNeuronID_; }
+  size_t getLastId() const { return firstNeuronID_ + neurons_.size() - 1; }
+  std::string getLabel() const { return label_; }
+  size_t getSize() const { return neurons_.size(); }
+
+ private:
+  NeuronModelT& at(unsigned long idx) {
+    if (idx >= neurons_.size()) {
+      std::ostringstream os;
+      os << "SZSNN::CONSTRUCTION::Population::setNeuronParameter() error: neuron " << idx << " of " << neurons_.size();
+      throw SzsnnException(os.str());
+    }
+    return neurons_[idx];
+  }
+  [[noreturn]] static void fail(const char* what, const std::string& filename, const std::string& why) {
+    throw SzsnnException(std::string("SZSNN::CONSTRUCTION::Population::") + what + "() error: " + why + " in file `" +
+                         filename + "`");
+  }
+  // the next non-empty line that is not a comment
+  static bool dataLine(std::istream& in, std::string& line, size_t& lineIdx) {
+    while (std::getline(in, line)) {
+      ++lineIdx;
+      const size_t b = line.find_first_not_of(" \t\r");
+      if (b == std::string::npos || line[b] == '#') continue;
+      line = line.substr(b);
+      return true;
+    }
+    return false;
+  }
+  template <unsigned K>
+  static void values(const std::string& line, floatT (&out)[K], const char* what, const std::string& filename,
+                     size_t lineIdx) {
+    std::istringstream ss(line);
+    std::vector<std::string> v{std::istream_iterator<std::string>(ss), std::istream_iterator<std::string>()};
+    if (v.size() != K) {
+      std::ostringstream os;
+      os << "wrong format at line #" << lineIdx << ": " << v.size() << " values, expected " << K;
+      fail(what, filename, os.str());
+    }
+    for (unsigned k = 0; k < K; ++k) {
+      try {
+        out[k] = std::stof(v[k]);
+      } catch (const std::exception&) {
+        std::ostringstream os;
+        os << "wrong format at line #" << lineIdx << " at value #" << k + 1;
+        fail(what, filename, os.str());
+      }
+    }
+  }
+  void load(const std::string& filename, bool tmpl, const char* what) {
+    std::ifstream in(filename);
+    if (!in.is_open()) fail(what, filename, "couldn't open file");
+    clear();
+    std::string line;
+    size_t lineIdx = 0;
+    long long size = 0;
+    try {
+      if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing size");
+      size = std::stoll(line);
+      if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing first neuron ID");
+      firstNeuronID_ = (size_t)std::stoull(line);
+    } catch (const std::invalid_argument&) {
+      fail(what, filename, "wrong header");
+    }
+    if (size <= 0) fail(what, filename, "Population size must be > 0");
+    if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing label");
+    label_ = line.substr(0, line.find_first_of(" \t\r"));
+    neurons_.assign((size_t)size, NeuronModelT());
+    floatT p[param_count], v[var_count > 0 ? var_count : 1];
+    for (size_t i = 0; i < (tmpl ? 1 : (size_t)size); ++i) {
+      if (!dataLine(in, line, lineIdx)) {
+        std::ostringstream os;
+        os << "not enough data: " << i << " of " << size << " neurons";
+        fail(what, filename, os.str());
+      }
+      values<param_count>(line, p, what, filename, lineIdx);
+      if constexpr (var_count > 0) {
+        if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing variable line");
+        values<var_count>(line, v, what, filename, lineIdx);
+      }
+      if (tmpl)
+        for (auto& n : neurons_) n.setAll(p, v);
+      else
+        neurons_[i].setAll(p, v);
+    }
+  }
+  void save(const std::string& filename, bool tmpl, const char* what) const {
+    if (neurons_.empty()) fail(what, filename, "trying to save an empty Population");
+    std::ofstream out(filename);
+    if (!out.is_open()) fail(what, filename, "couldn't open file");
+    out << "# Population " << (tmpl ? "Template " : "") << "file for : " << label_ << "\n# Model: "
+        << NeuronModelT::name() << "\n# Size: " << neurons_.size() << "\n# First neuron ID: " << firstNeuronID_
+        << "\n" << neurons_.size() << "\n" << firstNeuronID_ << "\n" << (label_.empty() ? "population" : label_) << "\n";
+    out << std::setprecision(9);
+    for (size_t i = 0; i < (tmpl ? 1 : neurons_.size()); ++i) {
+      for (unsigned k = 0; k < param_count; ++k) out << (k ? " " : "") << neurons_[i].getParam(k);
+      out << "\n";
+      if (var_count) {
+        for (unsigned k = 0; k < var_count; ++k) out << (k ? " " : "") << neurons_[i].getVar(k);
+        out << "\n";
+      }
+    }
+    if (!out) fail(what, filename, "output failure");
+  }
+
+  size_t firstNeuronID_ = 0;
+  std::string label_;
+  std::vector<NeuronModelT> neurons_;
+};
+
+// NetworkCreator (NetworkCreator.hpp:21-188): populations and projections
+// gathered, then written into a NetworkGraph.  Populations are taken in
+// order of their first id and must leave no gap; the input projections
+// become input synapses (the reference sorted with a comparator that
+// compared a population with itself, left the input size uninitialised and
+// added its neuron projections a second time as input synapses: the
+// intended behaviour is kept here).
+template <typename NeuronModelT, typename InputModelT>
+class NetworkCreator {
+ public:
+  typedef float float_type;
+  NetworkCreator() = default;
+  NetworkCreator(const NetworkCreator&) = delete;
+  NetworkCreator& operator=(const NetworkCreator&) = delete;
+
+  void addPopulation(const Population<NeuronModelT>& population) {
+    if (population.getFirstId() > networkSize_) {
+      std::ostringstream os;
+      os << "SZSNN::CONSTRUCTION::NetworkCreator::addPopulation error: adding population `" << population.getLabel()
+         << "` neurons must be consistance: networkSize=" << networkSize_ << " firstId=" << population.getFirstId();
+      throw SzsnnException(os.str());
+    }
+    networkSize_ = std::max(networkSize_, population.getLastId() + 1);
+    populations_.push_back(population);
+    std::stable_sort(populations_.begin(), populations_.end(),
+                     [](const Population<NeuronModelT>& a, const Population<NeuronModelT>& b) {
+                       return a.getFirstId() < b.getFirstId();
+                     });
+  }
+  void addInputPopulation(const Population<InputModelT>& population) {
+    if (population.getFirstId() > inputSize_) {
+      std::ostringstream os;
+      os << "SZSNN::CONSTRUCTION::NetworkCreator::addInputPopulation error: adding input population `"
+         << population.getLabel() << "` inputs must be consistance: inputSize=" << inputSize_
+         << " firstId=" << population.getFirstId();
+      throw SzsnnException(os.str());
+    }
+    inputSize_ = std::max(inputSize_, population.getLastId() + 1);
+    inputPopulations_.push_back(population);
+    std::stable_sort(inputPopulations_.begin(), inputPopulations_.end(),
+                     [](const Population<InputModelT>& a, const Population<InputModelT>& b) {
+                       return a.getFirstId() < b.getFirstId();
+                     });
+  }
+  size_t addConnection(const Connection<float_type>& connection) {
+    connections_.push_back(connection);
+    return connections_.size() - 1;
+  }
+  size_t addInputConnection(const Connection<float_type>& connection) {
+    inputConnections_.push_back(connection);
+    return inputConnections_.size() - 1;
+  }
+  // connection_<i>.conn for every neuron projection
+  void saveAllConnections() const {
+    for (size_t i = 0; i < connections_.size(); ++i)
+      connections_[i].saveToFile("connection_" + std::to_string(i) + ".conn");
+  }
+  void fillNetworkGraph(NetworkGraph<NeuronModelT, InputModelT>& graph) const {
+    for (const auto& pop : populations_) {
+      size_t id = pop.getFirstId();
+      for (const auto& n : pop) graph.addNeuron(id++, n);
+    }
+    for (const auto& pop : inputPopulations_) {
+      size_t id = pop.getFirstId();
+      for (const auto& in : pop) graph.addInput(id++, in);
+    }
+    for (const auto& con : connections_)
+      for (const auto& kv : con) graph.addSynapse(synapse(con, kv));
+    for (const auto& con : inputConnections_)
+      for (const auto& kv : con) graph.addInputSynapse(synapse(con, kv));
+  }
+
+ private:
+  template <class KV>
+  static Synapse<float_type> synapse(const Connection<float_type>& con, const KV& kv) {
+    Synapse<float_type> s;
+    s.sourceNeuronIdx = con.getFromStartID() + kv.first.first;
+    s.targetNeuronIdx = con.getToStartID() + kv.first.second;
+    s.weight = kv.second.first;
+    s.delay = kv.second.second;
+    return s;
+  }
+  size_t networkSize_ = 0, inputSize_ = 0;
+  std::vector<Population<NeuronModelT>> populations_;
+  std::vector<Population<InputModelT>> inputPopulations_;
+  std::vector<Connection<float_type>> connections_, inputConnections_;
+};
+
+}  // namespace construction
+
+// ------------------------------------------------------------------ record
+namespace record {
+
+// SpikeRecorder (SpikeRecorder.{hpp,cpp}): spikes of cycles recordStart ..
+// recordStop (0: no end) kept in memory; a cycle may be recorded once and a
+// neuron once per cycle.  Files: `#spikes by cycles` or `#spikes by
+// neurons`, `#record start at: S[   stop at: E]` (always ended by a newline
+// here; the reference's only when a stop was given), then `cycle<TAB>neuron`
+// lines ordered by cycle or by neuron.
+class SpikeRecorder {
+ public:
+  typedef unsigned cycleNr_t;
+  typedef unsigned neuronID_t;
+  explicit SpikeRecorder(cycleNr_t recordStart, cycleNr_t recordStop = 0)
+      : start_(recordStart), stop_(recordStop), current_(recordStart) {}
+
+  void recordCycleSpikes(const std::vector<neuronID_t>& spikes) { recordCycleSpikes(current_, spikes); }
+  void recordCycleSpikes(cycleNr_t cycle, const std::vector<neuronID_t>& spikes) {
+    if (cycle < start_ || (stop_ && cycle > stop_)) return;
+    if (byCycle_.count(cycle)) {
+      std::ostringstream os;
+      os << "SZSNN::RECORD::SpikeRecorder::recordCycleSpikes error: cycle `" << cycle << "` already recorder";
+      throw SzsnnException(os.str());
+    }
+    std::set<neuronID_t> ids(spikes.begin(), spikes.end());
+    if (ids.size() != spikes.size()) {
+      std::ostringstream os;
+      os << "SZSNN::RECORD::SpikeRecorder::recordCycleSpikes error: in cycle `" << cycle
+         << "` some spikes were duplicated";
+      throw SzsnnException(os.str());
+    }
+    for (neuronID_t n : ids) byNeuron_[n].insert(cycle);
+    byCycle_.emplace(cycle, std::move(ids));
+    ++current_;
+  }
+  void saveToFileByCycles(const std::string& filename) const {
+    std::ofstream out = open(filename, "saveToFileByCycles");
+    out << "#spikes by cycles\n" << header();
+    for (const auto& c : byCycle_)
+      for (neuronID_t n : c.second) out << c.first << "\t" << n << "\n";
+  }
+  void saveToFileByNeurons(const std::string& filename) const {
+    std::ofstream out = open(filename, "saveToFileByNeurons");
+    out << "#spikes by neurons\n" << header();
+    for (const auto& n : byNeuron_)
+      for (cycleNr_t c : n.second) out << c << "\t" << n.first << "\n";
+  }
+
+ private:
+  std::string header() const {
+    std::ostringstream os;
+    os << "#record start at: " << start_;
+    if (stop_) os << "   stop at: " << stop_;
+    os << "\n";
+    return os.str();
+  }
+  static std::ofstream open(const std::string& filename, const char* what) {
+    std::ofstream out(filename);
+    if (!out.is_open())
+      throw SzsnnException(std::string("SZSNN::RECORD::SpikeRecorder::") + what + "() error: couldn't open file `" +
+                           filename + "`");
+    return out;
+  }
+  cycleNr_t start_, stop_, current_;
+  std::map<neuronID_t, std::set<cycleNr_t>> byNeuron_;
+  std::map<cycleNr_t, std::set<neuronID_t>> byCycle_;
+};
+
+}  // namespace record
 
 }  // namespace szsnn

@@ -70,3 +70,8 @@ def test_spike_recorder_files(tmp_path):
     assert q.read_text().splitlines()[2:] == ["1\t2", "3\t2", "1\t7"]
     with pytest.raises(FormatError):
         SpikeRecorder().record_cycle_spikes(0, [1, 1])
+    # the stop cycle itself is recorded (SpikeRecorder.cpp: returns for cycle > stop)
+    s = SpikeRecorder(record_start=1, record_stop=2)
+    for c in range(4):
+        s.record_cycle_spikes(c, [c])
+    assert sorted(s.by_cycle) == [1, 2]

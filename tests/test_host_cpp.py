@@ -100,3 +100,17 @@ def test_cpp_driver_matches_oracle(tmp_path):
     bits = np.unpackbits(raster.view(np.uint8).reshape(steps, -1), axis=1, bitorder="little")[:, :100]
     np.testing.assert_array_equal(got, bits.astype(bool))
     assert got.sum() > 20  # the network is active (forced volleys plus Const_rate drive)
+
+
+def test_cpp_construction_and_record_classes(tmp_path):
+    """Population files, NetworkCreator -> NetworkGraph and record::SpikeRecorder
+    of the C++ layer (tests/cpp/construction_check.cpp), compiled on the host
+    with ASan/UBSan; no device is touched."""
+    exe = tmp_path / "construction_check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-fsanitize=address,undefined", "-I",
+                    os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "construction_check.cpp"),
+                    "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe), str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "construction_check OK" in r.stdout
