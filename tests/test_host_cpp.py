@@ -114,3 +114,34 @@ def test_cpp_construction_and_record_classes(tmp_path):
                        timeout=120)
     assert r.returncode == 0, r.stderr
     assert "construction_check OK" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_creator_driver_matches_oracle(tmp_path):
+    """tests/cpp/creator_driver.cpp (populations, a Const_rate input projection
+    through NetworkCreator, record::SpikeRecorder) run on the device: its spike
+    file equals the FP32 oracle's raster of the same one-neuron network."""
+    from oracle.pyoracle import Oracle
+    from szsnn_amd.network import Network, quantize_weights
+    exe = os.path.join(ROOT, "tests", "cpp", "creator_driver.bin")
+    if not os.path.exists(exe):
+        pytest.skip("creator_driver.bin not built (run __graft_entry__.build())")
+    steps = 600
+    out = tmp_path / "creator.txt"
+    r = subprocess.run([exe, str(steps), str(out)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = out.read_text().splitlines()
+    assert lines[:2] == ["#spikes by cycles", "#record start at: 0   stop at: 1000"]
+    got = sorted(int(ln.split("\t")[0]) for ln in lines[2:] if ln.split("\t")[1] == "0")
+    # the same network: weight 20 at the C++ layer's fixed point (2^25 = the
+    # largest power of two with 20 * 2^F < 2^30), API delay 1
+    net = Network(model=1, n_neurons=1, params=np.array([[0.02], [0.2], [-65.0], [8.0]], np.float32),
+                  vars=np.array([[-80.0], [0.0]], np.float32), syn_src=np.array([1], np.uint32),
+                  syn_tgt=np.array([0], np.uint32), syn_w=quantize_weights(np.array([20.0], np.float32), 25),
+                  syn_delay=np.array([1], np.uint16), max_delay=2, frac_bits=25, seed=0, n_sources=1,
+                  source_model=1, src_rate=np.array([2.0], np.float32),
+                  src_last=np.zeros(1, np.float32)).validate()
+    raster, _ = Oracle(net, 32).run(steps)
+    want = [t for t in range(steps) if int(raster[t, 0]) & 1]
+    assert got == want and len(want) > 8  # one spike about every 44 steps
