@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--natural-pass", action="store_true",
                     help="profiler pass: natural dynamics, --warmup steps then --steps steps as ONE persistent "
                          "launch (block 512, grid 256), then --volley-steps single-step gather+update launches")
+    ap.add_argument("--batched-pass", action="store_true",
+                    help="profiler pass: the batched Izhikevich instances only, --warmup then --batch-steps steps")
+    ap.add_argument("--adjoint-pass", action="store_true",
+                    help="profiler pass: the adjoint forward + backward only (2 untimed, 5 timed batches)")
     ap.add_argument("--gather-block", type=int, default=256, help="profile pass: pull gather block size")
     ap.add_argument("--gather-grid", type=int, default=4096, help="profile pass: pull gather grid")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_pull_volley_traffic.json"),
@@ -216,6 +220,29 @@ def main():
     import torch
     from szsnn_amd.configs import if_cond_exp_100k
     from szsnn_amd.simulation import Context, Simulation
+
+    if args.batched_pass:
+        from szsnn_amd.batch import BatchSimulation, batched_izhikevich
+        ctx = Context(0)
+        nets = batched_izhikevich(n_instances=args.batch_instances)
+        bs = BatchSimulation(ctx, nets)
+        bs.run(args.warmup)
+        ctx.synchronize()
+        c0 = bs.counters()
+        t1 = time.perf_counter()
+        bs.run(args.batch_steps)
+        ctx.synchronize()
+        wall = time.perf_counter() - t1
+        ev = bs.counters()["events"] - c0["events"]
+        print(json.dumps({"batched_pass": True, "instances": len(nets), "neurons": sum(n.n_neurons for n in nets),
+                          "steps": args.batch_steps, "us_per_step": wall * 1e6 / args.batch_steps,
+                          "events_per_step": ev / args.batch_steps, "tiling": bs.tiling()}), flush=True)
+        return
+    if args.adjoint_pass:
+        args.no_cpu = True
+        out = run_adjoint(args, log)
+        print(json.dumps({"adjoint_pass": True, **out}), flush=True)
+        return
 
     t0 = time.time()
     if args.small:

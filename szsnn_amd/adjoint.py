@@ -12,11 +12,15 @@ Network (one trial b, step t; dt = 1 ms):
            driven only by spikes would have dL/dW1 = 0 under a fixed pattern)
   loss     mean over (t, b, k) of (y_t - target_t)^2
 
-The forward stores the hidden state every K steps and the spike bits; the
-backward recomputes each segment from its checkpoint (refractory masks and
-h), then runs the hand-written adjoint kernel backwards through the segment.
-GEMMs (x W1, h W2 and the gradient products) go to the vendor BLAS through
-torch.matmul; the recurrences are the HIP kernels of szsnn_adjoint.hip.
+The forward computes the input currents of all T steps in two GEMMs (x W1
+split by sign), runs the recurrence segment by segment (the hidden state is
+stored every K steps, with the spike bits and h of every step), and forms the
+readout drive in one GEMM.  The backward runs the hand-written adjoint
+kernel backwards through each segment from its checkpoint, then forms the
+weight gradients of all steps in one GEMM each (about 0.4 GB of per-step
+currents, h and adjoints stay resident for T = 200, B = 64, H = 2000).
+GEMMs go to the vendor BLAS through torch.matmul; the recurrences are the
+HIP kernels of szsnn_adjoint.hip.
 """
 import ctypes
 import math
@@ -108,28 +112,27 @@ def _make_function():
             R = W2.shape[1]
             dev = W1.device
             Xprev = torch.cat([torch.zeros(1, B, n_in, device=dev, dtype=X.dtype), X[:-1]], 0).contiguous()
-            W1p = W1.clamp(min=0).contiguous()
-            W1n = W1.clamp(max=0).contiguous()
+            xs = Xprev.reshape(T * B, n_in)
+            # the input currents of every step in two GEMMs, kept for the backward
+            AE = (xs @ W1.clamp(min=0)).reshape(T, B * H)
+            AI = (xs @ W1.clamp(max=0)).reshape(T, B * H)
             words = (B * H + 63) // 64
             bits = torch.zeros(T, words, dtype=torch.int64, device=dev)
-            drive = torch.empty(T, B, R, dtype=torch.float32, device=dev)
+            h = torch.empty(T, B * H, dtype=torch.float32, device=dev)
             state = initial_state(torch, B, H, params, dev)
             ckpts = []
             st = _stream(torch)
             for s0 in range(0, T, K):
                 k = min(K, T - s0)
                 ckpts.append(state.clone())
-                xs = Xprev[s0:s0 + k].reshape(k * B, n_in)
-                AE = (xs @ W1p).reshape(k, B * H).contiguous()
-                AI = (xs @ W1n).reshape(k, B * H).contiguous()
-                h = torch.empty(k, B * H, dtype=torch.float32, device=dev)
-                _check(lib.szsnn_adj_forward_segment(st, B, H, k, _ptr(params), _ptr(state), _ptr(AE), _ptr(AI),
-                                                     _ptr(h), _ptr(bits[s0])), "adj_forward_segment")
-                drive[s0:s0 + k] = (h.reshape(k * B, H) @ W2).reshape(k, B, R)
+                _check(lib.szsnn_adj_forward_segment(st, B, H, k, _ptr(params), _ptr(state), _ptr(AE[s0]),
+                                                     _ptr(AI[s0]), _ptr(h[s0]), _ptr(bits[s0])),
+                       "adj_forward_segment")
+            drive = (h.reshape(T * B, H) @ W2).reshape(T, B, R).contiguous()
             y = torch.empty(T, B, R, dtype=torch.float32, device=dev)
             _check(lib.szsnn_readout_forward(st, B * R, T, alpha, None, _ptr(drive), _ptr(y)), "readout_forward")
             loss = ((y - target) ** 2).mean()
-            ctx.save_for_backward(W1, W2, Xprev, target, params, y, bits, *ckpts)
+            ctx.save_for_backward(W1, W2, xs, target, params, y, bits, h, *ckpts)
             ctx.alpha = alpha
             ctx.K = K
             ctx.mark_non_differentiable(y, bits)
@@ -137,47 +140,33 @@ def _make_function():
 
         @staticmethod
         def backward(ctx, gloss, gy, gbits):
-            W1, W2, Xprev, target, params, y, bits, *ckpts = ctx.saved_tensors
+            W1, W2, xs, target, params, y, bits, h, *ckpts = ctx.saved_tensors
             alpha, K = ctx.alpha, ctx.K
-            T, B, n_in = Xprev.shape
+            T, R = y.shape[0], y.shape[2]
+            B = y.shape[1]
             H = W1.shape[1]
-            R = W2.shape[1]
             dev = W1.device
             st = _stream(torch)
-            W1p = W1.clamp(min=0).contiguous()
-            W1n = W1.clamp(max=0).contiguous()
             lam = torch.empty(T, B, R, dtype=torch.float32, device=dev)
             scale = float(gloss) * 2.0 / (T * B * R)
             _check(lib.szsnn_readout_backward(st, B * R, T, alpha, scale, _ptr(y), _ptr(target), _ptr(lam)),
                    "readout_backward")
-            dW1p = torch.zeros(n_in, H, dtype=torch.float32, device=dev)
-            dW1n = torch.zeros(n_in, H, dtype=torch.float32, device=dev)
-            dW2 = torch.zeros(H, R, dtype=torch.float32, device=dev)
+            lam2 = lam.reshape(T * B, R)
+            dW2 = h.reshape(T * B, H).t() @ lam2
+            g = (lam2 @ W2.t()).reshape(T, B * H).contiguous()
             carry = torch.zeros(3, B * H, dtype=torch.float32, device=dev)
-            words = bits.shape[1]
-            seg_bits = torch.empty(K, words, dtype=torch.int64, device=dev)
+            LE = torch.empty(T, B * H, dtype=torch.float32, device=dev)
+            LI = torch.empty(T, B * H, dtype=torch.float32, device=dev)
             starts = list(range(0, T, K))
             for i in reversed(range(len(starts))):
                 s0 = starts[i]
                 k = min(K, T - s0)
-                state = ckpts[i].clone()
-                xs = Xprev[s0:s0 + k].reshape(k * B, n_in)
-                AE = (xs @ W1p).reshape(k, B * H).contiguous()
-                AI = (xs @ W1n).reshape(k, B * H).contiguous()
-                h = torch.empty(k, B * H, dtype=torch.float32, device=dev)
-                # recompute the segment from its checkpoint (h for dW2)
-                _check(lib.szsnn_adj_forward_segment(st, B, H, k, _ptr(params), _ptr(state), _ptr(AE), _ptr(AI),
-                                                     _ptr(h), _ptr(seg_bits)), "adj_forward_segment")
-                lam_s = lam[s0:s0 + k].reshape(k * B, R)
-                dW2 += h.reshape(k * B, H).t() @ lam_s
-                g = (lam_s @ W2.t()).reshape(k, B * H).contiguous()
-                LE = torch.empty(k, B * H, dtype=torch.float32, device=dev)
-                LI = torch.empty(k, B * H, dtype=torch.float32, device=dev)
                 _check(lib.szsnn_adj_backward_segment(st, B, H, k, _ptr(params), _ptr(ckpts[i]), _ptr(bits[s0]),
-                                                      _ptr(g), _ptr(carry), _ptr(LE), _ptr(LI)),
+                                                      _ptr(g[s0]), _ptr(carry), _ptr(LE[s0]), _ptr(LI[s0])),
                        "adj_backward_segment")
-                dW1p += xs.t() @ LE.reshape(k * B, H)
-                dW1n += xs.t() @ LI.reshape(k * B, H)
+            # the weight gradients of every step in two GEMMs
+            dW1p = xs.t() @ LE.reshape(T * B, H)
+            dW1n = xs.t() @ LI.reshape(T * B, H)
             dW1 = torch.where(W1 >= 0, dW1p, dW1n)
             return dW1, dW2, None, None, None, None, None
 
