@@ -213,6 +213,10 @@ int szsnn_autotune(szsnn_net* net, uint32_t steps);
  *   backward_segment: adjoint over the segment; ckpt = state at its start;
  *                     g = dL/dh [K][B*H]; carry [3][B*H] in/out; writes
  *                     LE/LI = dL/dse, dL/dsi [K][B*H].
+ *   input_currents  : AE[r][j] = sum_i X[r][i] max(W1[i][j], 0) and AI[r][j] the
+ *                     same with min(W1[i][j], 0), for rows r < rows of X
+ *                     [rows][n_in] and W1 [n_in][H], over the nonzero X in
+ *                     ascending i (spiking inputs are sparse).
  *   readout_forward : y_t = alpha y_{t-1} + drive_t over [T][BR]; y0 may be NULL.
  *   readout_backward: lam_t = scale (y_t - target_t) + alpha lam_{t+1}. */
 int szsnn_adj_forward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params, float* state,
@@ -220,6 +224,8 @@ int szsnn_adj_forward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, 
 int szsnn_adj_backward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params,
                                const float* ckpt, const uint64_t* spike_bits, const float* g, float* carry,
                                float* LE, float* LI);
+int szsnn_adj_input_currents(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
+                             float* AE, float* AI);
 int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,
                           float* y);
 int szsnn_readout_backward(void* stream, uint32_t BR, uint32_t T, float alpha, float scale, const float* y,

@@ -129,6 +129,8 @@ PROTOTYPES = [
                                                  _vp, _vp, _vp, _vp, _vp, _vp]),
     ("szsnn_adj_backward_segment", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("szsnn_adj_input_currents", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp,
+                                                 _vp, _vp]),
     ("szsnn_readout_forward", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, _vp, _vp,
                                              _vp]),
     ("szsnn_readout_backward", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,

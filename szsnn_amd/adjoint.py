@@ -12,8 +12,9 @@ Network (one trial b, step t; dt = 1 ms):
            driven only by spikes would have dL/dW1 = 0 under a fixed pattern)
   loss     mean over (t, b, k) of (y_t - target_t)^2
 
-The forward computes the input currents of all T steps in two GEMMs (x W1
-split by sign), runs the recurrence segment by segment (the hidden state is
+The forward computes the input currents of all T steps with one HIP kernel
+that adds only the spiking inputs' rows of W1 (split by sign), runs the
+recurrence segment by segment (the hidden state is
 stored every K steps, with the spike bits and h of every step), and forms the
 readout drive in one GEMM.  The backward runs the hand-written adjoint
 kernel backwards through each segment from its checkpoint, then forms the
@@ -112,16 +113,20 @@ def _make_function():
             R = W2.shape[1]
             dev = W1.device
             Xprev = torch.cat([torch.zeros(1, B, n_in, device=dev, dtype=X.dtype), X[:-1]], 0).contiguous()
-            xs = Xprev.reshape(T * B, n_in)
-            # the input currents of every step in two GEMMs, kept for the backward
-            AE = (xs @ W1.clamp(min=0)).reshape(T, B * H)
-            AI = (xs @ W1.clamp(max=0)).reshape(T, B * H)
+            xs = Xprev.reshape(T * B, n_in).to(torch.float32).contiguous()
+            # the input currents of every step (only the spiking inputs are
+            # added), kept for the backward
+            AE = torch.empty(T, B * H, dtype=torch.float32, device=dev)
+            AI = torch.empty(T, B * H, dtype=torch.float32, device=dev)
+            W1c = W1.contiguous()
+            st = _stream(torch)
+            _check(lib.szsnn_adj_input_currents(st, T * B, n_in, H, _ptr(xs), _ptr(W1c), _ptr(AE), _ptr(AI)),
+                   "adj_input_currents")
             words = (B * H + 63) // 64
             bits = torch.zeros(T, words, dtype=torch.int64, device=dev)
             h = torch.empty(T, B * H, dtype=torch.float32, device=dev)
             state = initial_state(torch, B, H, params, dev)
             ckpts = []
-            st = _stream(torch)
             for s0 in range(0, T, K):
                 k = min(K, T - s0)
                 ckpts.append(state.clone())

@@ -19,7 +19,8 @@
 //   Cv = lv'_t * (r_t ? 1 : 1 - 1/tau_m);  Ce = (1-1/tau_e) lse_t;  Ci = (1-1/tau_i) lsi_t
 // The carries (Cv, Ce, Ci) cross segment boundaries through `carry`.
 // dL/dW1 = sum_t x_{t-1} (lse_t or lsi_t by the weight's sign) is formed by
-// the caller with library GEMMs.
+// the caller with library GEMMs; the input currents AE, AI of all steps come
+// from input_currents_kernel, which adds only the nonzero inputs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -163,6 +164,68 @@ __global__ void readout_backward_kernel(uint32_t BR, uint32_t T, float alpha, fl
   }
 }
 
+// Input currents of one input row r (a (step, trial) pair): AE[r][j] =
+// sum_i X[r][i] * max(W1[i][j], 0), AI[r][j] likewise with min(W1[i][j], 0),
+// over the nonzero X[r][i] only (input spikes: a few percent of the inputs),
+// in ascending i.  One block per row; the inputs are scanned in chunks of
+// the block size, each chunk's nonzeros listed in LDS in order, then every
+// thread adds them into its kCols columns (strided by the block size, so
+// each W1 row is read coalesced).
+constexpr int kInBlock = 256;
+constexpr int kCols = 8;
+
+__global__ void __launch_bounds__(kInBlock) input_currents_kernel(uint32_t n_in, uint32_t H, uint32_t col0,
+                                                                  const float* __restrict__ X,
+                                                                  const float* __restrict__ W1, float* __restrict__ AE,
+                                                                  float* __restrict__ AI) {
+  __shared__ uint32_t s_idx[kInBlock];
+  __shared__ float s_val[kInBlock];
+  __shared__ uint32_t s_wave[kInBlock / 64];
+  const uint32_t r = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const float* x = X + (size_t)r * n_in;
+  float e[kCols], in[kCols];
+#pragma unroll
+  for (int c = 0; c < kCols; ++c) e[c] = in[c] = 0.0f;
+  for (uint32_t base = 0; base < n_in; base += kInBlock) {
+    const uint32_t i = base + tid;
+    const float xv = i < n_in ? x[i] : 0.0f;
+    const uint64_t m = __ballot(xv != 0.0f);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t pos = 0, n = 0;
+    for (uint32_t w = 0; w < kInBlock / 64; ++w) {
+      pos += w < wave ? s_wave[w] : 0u;
+      n += s_wave[w];
+    }
+    if (xv != 0.0f) {
+      pos += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      s_idx[pos] = i;
+      s_val[pos] = xv;
+    }
+    __syncthreads();
+    for (uint32_t q = 0; q < n; ++q) {
+      const float* wr = W1 + (size_t)s_idx[q] * H;
+      const float xq = s_val[q];
+#pragma unroll
+      for (int c = 0; c < kCols; ++c) {
+        const uint32_t j = col0 + c * kInBlock + tid;
+        const float w = j < H ? wr[j] : 0.0f;
+        e[c] += xq * (w > 0.0f ? w : 0.0f);
+        in[c] += xq * (w < 0.0f ? w : 0.0f);
+      }
+    }
+    __syncthreads();  // the list is rewritten by the next chunk
+  }
+#pragma unroll
+  for (int c = 0; c < kCols; ++c) {
+    const uint32_t j = col0 + c * kInBlock + tid;
+    if (j < H) {
+      AE[(size_t)r * H + j] = e[c];
+      AI[(size_t)r * H + j] = in[c];
+    }
+  }
+}
+
 int launch_check(hipError_t e) { return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP; }
 
 }  // namespace
@@ -187,6 +250,17 @@ int szsnn_adj_backward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K,
   const uint32_t blocks = (BH + 255) / 256;
   adj_backward_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(BH, H, K, params, ckpt, spike_bits, g, carry, LE, LI);
   return launch_check(hipGetLastError());
+}
+
+int szsnn_adj_input_currents(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
+                             float* AE, float* AI) {
+  if (!rows || !n_in || !H || !X || !W1 || !AE || !AI) return SZSNN_ERR_INVALID;
+  for (uint32_t col0 = 0; col0 < H; col0 += kCols * kInBlock) {
+    input_currents_kernel<<<rows, kInBlock, 0, (hipStream_t)stream>>>(n_in, H, col0, X, W1, AE, AI);
+    const int rc = launch_check(hipGetLastError());
+    if (rc != SZSNN_OK) return rc;
+  }
+  return SZSNN_OK;
 }
 
 int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,

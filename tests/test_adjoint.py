@@ -179,3 +179,34 @@ def test_hip_adjoint_matches_finite_differences(cuda):
         assert abs(fd - g) <= 1e-3 * max(abs(fd), 1e-3), (which, i, j, fd, g)
         checked += 1
     assert checked >= 15
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,n_in,H", [(300, 1000, 2000), (37, 300, 2100), (5, 513, 4500)])
+def test_input_currents_equal_sign_split_products(cuda, rows, n_in, H):
+    """szsnn_adj_input_currents (nonzero inputs only, ascending order) against
+    the FP64 products X max(W1, 0) and X min(W1, 0); inputs include values
+    other than 0/1, widths not multiples of the block (H up to 4500: several
+    column passes) and an all-zero row."""
+    import ctypes
+    import torch
+    from szsnn_amd import _lib
+    lib = _lib.load()
+    g = torch.Generator().manual_seed(rows * 7 + H)
+    X = (torch.rand(rows, n_in, generator=g) < 0.03).float() * (1.0 + (torch.rand(rows, n_in, generator=g) < 0.2))
+    X[rows // 2] = 0.0
+    W = torch.randn(n_in, H, generator=g) * 0.1
+    Xd, Wd = X.to(cuda).contiguous(), W.to(cuda).contiguous()
+    AE = torch.empty(rows, H, device=cuda)
+    AI = torch.empty(rows, H, device=cuda)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    rc = lib.szsnn_adj_input_currents(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), rows, n_in, H,
+                                      ptr(Xd), ptr(Wd), ptr(AE), ptr(AI))
+    assert rc == 0
+    torch.cuda.synchronize()
+    X64, W64 = X.double(), W.double()
+    re, ri = X64 @ W64.clamp(min=0), X64 @ W64.clamp(max=0)
+    # FP32 sums of at most ~3 % of n_in terms: within a few ulps of the FP64 value
+    assert torch.allclose(AE.cpu().double(), re, rtol=1e-5, atol=1e-6)
+    assert torch.allclose(AI.cpu().double(), ri, rtol=1e-5, atol=1e-6)
+    assert float(AE[rows // 2].abs().max()) == 0.0 and float(AI[rows // 2].abs().max()) == 0.0
