@@ -9,9 +9,15 @@
 * Population files (szsnn1.1/src/Population.hpp:118-397): `#` comment lines,
   then the size, the first neuron id and the label, then per neuron one line
   of parameters and one line of variables.  The template variant holds one
-  parameter line and one variable line for every neuron.  (The reference's
-  own writer omitted the label its reader expects, and its Brian template
-  writer used another field order: both files here read back.)
+  parameter line and one variable line for every neuron.  The reference's
+  own writer omitted the label its reader expects, and the pyszsnn Brian
+  dumper (pyszsnn/brian/dumper.py:12-126) put the label first (templates:
+  label, size, first id; populations: model name, size, size again, first
+  id); the reference could read neither.  The readers here take all three
+  header orders: a header that starts with a number is size, first id,
+  label; one that starts with a word is that word followed by the integers
+  up to the first parameter line, of which the first is the size and the
+  last the first id.
 * Spike files (szsnn1.1/src/SpikeRecorder.cpp:13-104): `#spikes by cycles` or
   `#spikes by neurons`, `#record start at: S[   stop at: E]` (always ended by
   a newline here), then `cycle<TAB>neuron` lines.
@@ -70,15 +76,42 @@ def _data_lines(path):
                 yield s
 
 
+def _is_int(s):
+    try:
+        int(s)
+        return True
+    except ValueError:
+        return False
+
+
+def _header(lines, path):
+    """(size, first id, label, index of the first line after the header) of
+    a population or template file's data lines, in any of the three orders."""
+    try:
+        if _is_int(lines[0]):
+            # a label is one word; a longer third line is already the first
+            # parameter line (a Brian template written with an empty label)
+            if len(lines) > 2 and len(lines[2].split()) == 1:
+                return int(lines[0]), int(lines[1]), lines[2].strip(), 3
+            return int(lines[0]), int(lines[1]), "", 2
+        label, k, ints = lines[0].split()[0], 1, []
+        while k < len(lines) and len(lines[k].split()) == 1 and _is_int(lines[k]):
+            ints.append(int(lines[k]))
+            k += 1
+        if len(ints) not in (2, 3) or (len(ints) == 3 and ints[0] != ints[1]):
+            raise FormatError(f"{path}: bad header: {len(ints)} integers after `{label}`")
+        return ints[0], ints[-1], label, k
+    except IndexError:
+        raise FormatError(f"{path}: bad header: file too short") from None
+    except ValueError as e:
+        raise FormatError(f"{path}: bad header: {e}") from None
+
+
 def load_population(path, model):
     """Returns (params [n_params, n], vars [n_vars, n], first_id, label)."""
-    it = _data_lines(path)
-    try:
-        n = int(next(it))
-        first = int(next(it))
-        label = next(it)
-    except (StopIteration, ValueError) as e:
-        raise FormatError(f"{path}: bad header: {e}") from None
+    lines = list(_data_lines(path))
+    n, first, label, k = _header(lines, path)
+    it = iter(lines[k:])
     if n <= 0:
         raise FormatError(f"{path}: population size must be > 0")
     npar, nvar = N_PARAMS[model], N_VARS[model]
@@ -112,14 +145,12 @@ def save_population(path, model, params, vars_, first_id=0, label="population", 
 
 def load_population_template(path, model):
     """Template variant: one params line and one vars line for all neurons."""
-    it = _data_lines(path)
+    lines = list(_data_lines(path))
+    n, first, label, k = _header(lines, path)
     try:
-        n = int(next(it))
-        first = int(next(it))
-        label = next(it)
-        p = [float(x) for x in next(it).split()]
-        v = [float(x) for x in next(it).split()]
-    except (StopIteration, ValueError) as e:
+        p = [float(x) for x in lines[k].split()]
+        v = [float(x) for x in lines[k + 1].split()] if N_VARS[model] else []
+    except (IndexError, ValueError) as e:
         raise FormatError(f"{path}: bad template: {e}") from None
     if n <= 0 or len(p) != N_PARAMS[model] or len(v) != N_VARS[model]:
         raise FormatError(f"{path}: bad template sizes")

@@ -522,7 +522,9 @@ using szsnn::Connection;
 // first id, the label (one word), then one parameter line and one variable
 // line per neuron (the template file: one pair for all neurons).  The
 // reference's writer left out the label its reader expects; the files
-// written here have it, so they read back.
+// written here have it, so they read back.  The files of the pyszsnn Brian
+// dumper (pyszsnn/brian/dumper.py:12-126: label or model name first) read
+// too; see header().
 template <typename NeuronModelT>
 class Population {
  public:
@@ -617,37 +619,80 @@ class Population {
       }
     }
   }
+  static bool isInt(const std::string& t) {
+    if (t.empty()) return false;
+    size_t i = (t[0] == '-' || t[0] == '+') ? 1 : 0;
+    if (i == t.size()) return false;
+    for (; i < t.size(); ++i)
+      if (t[i] < '0' || t[i] > '9') return false;
+    return true;
+  }
+  static std::vector<std::string> words(const std::string& line) {
+    std::istringstream ss(line);
+    return {std::istream_iterator<std::string>(ss), std::istream_iterator<std::string>()};
+  }
+  // The header in any of three orders: size, first id, label (the
+  // reference's); or a word (the label of a pyszsnn Brian template, or the
+  // model name of a Brian population file) followed by the integers up to
+  // the first parameter line, of which the first is the size and the last
+  // the first id (templates: size, id; populations: size, size, id).  A
+  // reference-order header whose third line has several values has no
+  // label (a Brian template written with an empty one).
+  void header(const std::vector<std::string>& lines, size_t& k, long long& size, const char* what,
+              const std::string& filename) {
+    auto num = [&](size_t i) -> long long {
+      if (i >= lines.size() || !isInt(lines[i])) fail(what, filename, "wrong header");
+      return std::stoll(lines[i]);
+    };
+    if (lines.empty()) fail(what, filename, "empty file");
+    if (isInt(lines[0])) {
+      size = num(0);
+      firstNeuronID_ = (size_t)num(1);
+      if (lines.size() > 2 && words(lines[2]).size() == 1) {
+        label_ = words(lines[2])[0];
+        k = 3;
+      } else {
+        k = 2;
+      }
+      return;
+    }
+    label_ = words(lines[0])[0];
+    std::vector<long long> ints;
+    for (k = 1; k < lines.size() && words(lines[k]).size() == 1 && isInt(words(lines[k])[0]); ++k)
+      ints.push_back(std::stoll(lines[k]));
+    if (!(ints.size() == 2 || (ints.size() == 3 && ints[0] == ints[1]))) fail(what, filename, "wrong header");
+    size = ints.front();
+    firstNeuronID_ = (size_t)ints.back();
+  }
   void load(const std::string& filename, bool tmpl, const char* what) {
     std::ifstream in(filename);
     if (!in.is_open()) fail(what, filename, "couldn't open file");
     clear();
-    std::string line;
-    size_t lineIdx = 0;
-    long long size = 0;
-    try {
-      if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing size");
-      size = std::stoll(line);
-      if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing first neuron ID");
-      firstNeuronID_ = (size_t)std::stoull(line);
-    } catch (const std::invalid_argument&) {
-      fail(what, filename, "wrong header");
+    std::vector<std::string> lines;
+    std::vector<size_t> lineNo;
+    {
+      std::string line;
+      size_t lineIdx = 0;
+      while (dataLine(in, line, lineIdx)) {
+        lines.push_back(line);
+        lineNo.push_back(lineIdx);
+      }
     }
+    size_t k = 0;
+    long long size = 0;
+    header(lines, k, size, what, filename);
     if (size <= 0) fail(what, filename, "Population size must be > 0");
-    if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing label");
-    label_ = line.substr(0, line.find_first_of(" \t\r"));
     neurons_.assign((size_t)size, NeuronModelT());
     floatT p[param_count], v[var_count > 0 ? var_count : 1];
-    for (size_t i = 0; i < (tmpl ? 1 : (size_t)size); ++i) {
-      if (!dataLine(in, line, lineIdx)) {
+    const size_t per = var_count > 0 ? 2 : 1;
+    for (size_t i = 0; i < (tmpl ? 1 : (size_t)size); ++i, k += per) {
+      if (k + per > lines.size()) {
         std::ostringstream os;
         os << "not enough data: " << i << " of " << size << " neurons";
         fail(what, filename, os.str());
       }
-      values<param_count>(line, p, what, filename, lineIdx);
-      if constexpr (var_count > 0) {
-        if (!dataLine(in, line, lineIdx)) fail(what, filename, "missing variable line");
-        values<var_count>(line, v, what, filename, lineIdx);
-      }
+      values<param_count>(lines[k], p, what, filename, lineNo[k]);
+      if constexpr (var_count > 0) values<var_count>(lines[k + 1], v, what, filename, lineNo[k + 1]);
       if (tmpl)
         for (auto& n : neurons_) n.setAll(p, v);
       else

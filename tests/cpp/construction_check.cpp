@@ -68,6 +68,26 @@ int main(int argc, char** argv) {
     std::ofstream(dir + "/short.pop") << "# two announced, one given\n2\n0\nx\n0.02 0.2 -65 8\n-65 -13\n";
   }
   CHECK(throws([&] { Population<N> p; p.loadFromFile(dir + "/short.pop"); }), "missing neuron must throw");
+  // the pyszsnn Brian dumper's orders: template (label, size, first id), with
+  // and without a label; population (model name, size, size, first id)
+  {
+    std::ofstream(dir + "/b1.tmpl") << "# population template saved by pyszsnn.brian\n# Izhikevich\nexc\n3\n5\n"
+                                       "0.02 0.2 -65 8\n-65 -13\n";
+    std::ofstream(dir + "/b2.tmpl") << "# population template saved by pyszsnn.brian\n# Izhikevich\n\n3\n5\n"
+                                       "0.02 0.2 -65 8\n-65 -13\n";
+    std::ofstream(dir + "/b.pop") << "# population saved by pyszsnn.brian\nIzhikevich\n2\n2\n7\n0.02 0.2 -65 8\n"
+                                     "-65 -13\n0.03 0.2 -65 8\n-64 -13\n";
+  }
+  Population<N> b1, b2, bp;
+  b1.loadFromTemplateFile(dir + "/b1.tmpl");
+  b2.loadFromTemplateFile(dir + "/b2.tmpl");
+  bp.loadFromFile(dir + "/b.pop");
+  CHECK(b1.getSize() == 3 && b1.getFirstId() == 5 && b1.getLabel() == "exc" && (b1.begin() + 2)->getParam(3) == 8.0f,
+        "Brian template");
+  CHECK(b2.getSize() == 3 && b2.getFirstId() == 5 && b2.getLabel().empty() && b2.begin()->getVar(1) == -13.0f,
+        "Brian template without a label");
+  CHECK(bp.getSize() == 2 && bp.getFirstId() == 7 && (bp.begin() + 1)->getParam(0) == 0.03f &&
+            (bp.begin() + 1)->getVar(0) == -64.0f, "Brian population");
   CHECK(throws([&] { Population<N> p; p.loadFromFile(dir + "/none.pop"); }), "missing file must throw");
   CHECK(throws([] { Population<N> p(0, 0); }), "empty population must throw");
 

@@ -75,3 +75,27 @@ def test_spike_recorder_files(tmp_path):
     for c in range(4):
         s.record_cycle_spikes(c, [c])
     assert sorted(s.by_cycle) == [1, 2]
+
+
+def test_population_files_in_the_brian_dumper_orders(tmp_path):
+    """The pyszsnn Brian dumper (pyszsnn/brian/dumper.py:12-126) wrote the label
+    (templates) or the model name and the size twice (populations) before the
+    numbers; the readers take those orders as well as the reference's."""
+    from szsnn_amd.formats import FormatError, load_population, load_population_template
+    t1 = tmp_path / "b1.tmpl"
+    t1.write_text("# population template saved by pyszsnn.brian\n# Izhikevich\nexc\n3\n5\n0.02 0.2 -65 8\n-65 -13\n")
+    P, V, first, label = load_population_template(t1, 1)
+    assert P.shape == (4, 3) and first == 5 and label == "exc" and np.all(P[3] == 8) and np.all(V[1] == -13)
+    t2 = tmp_path / "b2.tmpl"   # written with the default empty label: a blank line
+    t2.write_text("# population template saved by pyszsnn.brian\n# Izhikevich\n\n3\n5\n0.02 0.2 -65 8\n-65 -13\n")
+    P, V, first, label = load_population_template(t2, 1)
+    assert P.shape == (4, 3) and first == 5 and label == ""
+    pp = tmp_path / "b.pop"
+    pp.write_text("# population saved by pyszsnn.brian\nIzhikevich\n2\n2\n7\n"
+                  "0.02 0.2 -65 8\n-65 -13\n0.03 0.2 -65 8\n-64 -13\n")
+    P, V, first, label = load_population(pp, 1)
+    assert P.shape == (4, 2) and first == 7 and P[0, 1] == np.float32(0.03) and V[0, 1] == -64
+    bad = tmp_path / "bad.pop"   # the size twice, but different
+    bad.write_text("Izhikevich\n2\n3\n7\n0.02 0.2 -65 8\n-65 -13\n")
+    with pytest.raises(FormatError):
+        load_population(bad, 1)
