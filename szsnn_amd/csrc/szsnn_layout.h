@@ -17,12 +17,13 @@
 //     lane, rounded up to whole 16-byte vectors), stored as vectors
 //     [L / V][64 lanes][V entries], so the wave's j-th 16-byte load is
 //     contiguous; each lane also records how many vectors hold its own
-//     entries (the rest of its column is padding and is not read).  Entries name the source relative to the block's first
-//     staged id.  Wide entries (8 bytes, V = 2): (source offset, weight).
-//     Narrow entries (4 bytes, V = 4), when the network has at most
-//     kPullMaxWeights distinct weights and the offsets fit: source offset <<
-//     wb | weight index, with a weight table whose index 0 is the inert 0.
-//     Padding entries are (offset 0, weight 0).
+//     entries (the rest of its column is padding and is not read).  Entries
+//     name the source relative to the block's first staged id, in one of
+//     three formats (kPullWide, kPullCount, kPullTable in szsnn_constants.h):
+//     8-byte (offset, weight); with at most kPullCountFields distinct weights
+//     2-byte running deltas of the offset with a weight tag (skip entries for
+//     wider gaps); with at most kPullMaxWeights, 4-byte offset | weight
+//     index.  The weight table's index 0 is the inert 0.
 // Shared by the runtime and by the CPU layout check (tests/cpp/layout_check.cpp).
 #pragma once
 
