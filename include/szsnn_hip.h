@@ -190,6 +190,18 @@ int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_bl
 int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction);
 int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_pull);
 
+/* Heavy pushes (networks with a pull layout; no counterpart in the
+ * reference, whose push is one work-item per spiking id): on a pushed step,
+ * an update block whose spiking ids carry at least min_entries synapse
+ * entries posts their synapse ranges instead of adding them itself, and the
+ * next step's gather launch adds them with its whole grid, so a burst
+ * concentrated in a few blocks is not limited by one CU's atomic rate.  The
+ * sums are identical either way.  Default 8192; 0 = every block pushes its
+ * own ids.  Setting it on a network without a pull layout is an error
+ * unless min_entries is 0. */
+int szsnn_set_heavy_push(szsnn_net* net, uint32_t min_entries);
+int szsnn_get_heavy_push(szsnn_net* net, uint32_t* min_entries);
+
 /* Execution mode of szsnn_run.  GRAPH: each step is a gather launch (a no-op
  * unless something is due for the pull) and an update launch, replayed from a
  * captured HIP graph.  PERSISTENT: one cooperative launch runs all steps, a

@@ -125,6 +125,8 @@ PROTOTYPES = [
     ("szsnn_set_mode", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]),
     ("szsnn_set_gather_policy", ctypes.c_int, [_vp, ctypes.c_float]),
     ("szsnn_get_gather_policy", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]),
+    ("szsnn_set_heavy_push", ctypes.c_int, [_vp, ctypes.c_uint32]),
+    ("szsnn_get_heavy_push", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32)]),
     ("szsnn_adj_forward_segment", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                  _vp, _vp, _vp, _vp, _vp, _vp]),
     ("szsnn_adj_backward_segment", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,

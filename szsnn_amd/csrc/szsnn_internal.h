@@ -78,6 +78,15 @@ struct DevNet {
   unsigned long long* counters;  // striped, see counter(): events spikes drive overflow rows
   const uint64_t* t_base;   // device cycle counter
   uint32_t volley;          // ids [0, volley) are forced every step
+  // Heavy pushes: in single-step launches a block whose spiking ids carry at
+  // least heavy_min synapse entries posts their ranges instead of pushing
+  // them (one CU alone pushes only a small share of the device's atomic
+  // rate), and the next step's gather launch (or the launch before a
+  // persistent loop) pushes them with its whole grid.  Step s posts into
+  // list s & 1: heavy_cnt[s & 1] ranges at heavy_list[(s & 1) * T ..].
+  uint32_t heavy_min;       // 0: every block pushes its own ids (default kHeavyPushMin)
+  uint32_t* heavy_cnt;      // [2]
+  uint2* heavy_list;        // [2][T] (first synapse, synapses)
   const uint64_t* forced_bits;   // [W] forced ids this step, or null
   uint64_t* hist;           // [R][W] spike bitmaps of the last R steps (slot t % R)
   // pull layout (dense gather), a sliced ELLPACK: target blocks of kPullTB
@@ -171,9 +180,16 @@ __host__ __device__ inline unsigned long long* spk_counter(unsigned long long* b
   return base + ((size_t)(step & 3) * kRingStripes + (stripe % kRingStripes)) * kCntStride;
 }
 
+// Default heavy-push threshold: synapse entries of one update block's spiking
+// ids (16 per thread of a 512-thread block).
+constexpr uint32_t kHeavyPushMin = 8192;
+
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
 // gather: the pull of step t (launches nothing without a pull layout).
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);
+// The heavy pushes posted by step t_base - 1 (before a persistent launch; the
+// gather launch of a step does the same for its previous step).
+hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s);
 // update: block 64, 128, 256, 512 or 1024.
 hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s);
 // One cooperative launch running nsteps steps (block 256/512/1024; grid must

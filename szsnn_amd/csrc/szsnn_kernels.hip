@@ -233,6 +233,50 @@ __device__ inline void push_entries(const DevNet& n, const uint2 (&sw)[NB], uint
     if (sw[u].y != 0u) atomicAdd((unsigned long long*)dst[u], (unsigned long long)(long long)(int32_t)sw[u].y);
 }
 
+// Whether a block whose spiking ids carry `total` synapse entries posts them
+// as heavy pushes (single-step launches only: inside the persistent loop
+// every block pushes its own ids).
+__device__ inline bool heavy_block(const DevNet& n, uint32_t total, bool agent) {
+  return !agent && n.heavy_min && total >= n.heavy_min;
+}
+
+// Posts the synapse range (first, count) of a spiking id of step t.
+__device__ inline void post_heavy(const DevNet& n, uint64_t t, uint2 range) {
+  const uint32_t q = atomicAdd(&n.heavy_cnt[t & 1], 1u);
+  n.heavy_list[(size_t)(t & 1) * n.T + q] = range;
+}
+
+// Pushes the hc heavy ranges posted by step s: range q by block q (mod the
+// grid), all its threads; events counted per (block, delay) into the
+// delivery ring like the update's own pushes.  Every thread of the block
+// must call it.
+template <int BLK>
+__device__ inline void heavy_push(const DevNet& n, uint64_t s, uint32_t hc, uint32_t* ev) {
+  for (uint32_t i = threadIdx.x; i < 64; i += BLK) ev[i] = 0;
+  __syncthreads();
+  const uint32_t slot = (uint32_t)(s % n.R);
+  const uint32_t first_slot = slot + 1 == n.R ? 0u : slot + 1;
+  const uint2* list = n.heavy_list + (size_t)(s & 1) * n.T;
+  for (uint32_t q = blockIdx.x; q < hc; q += gridDim.x) {
+    const uint2 range = list[q];
+    for (uint32_t e0 = 0; e0 < range.y; e0 += BLK * kPushBatch) {
+      uint2 sw[kPushBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kPushBatch; ++u) {
+        const uint32_t e = e0 + u * BLK + threadIdx.x;
+        sw[u] = e < range.y ? n.syn[range.x + e] : make_uint2(0u, 0u);
+      }
+      push_entries<kPushBatch>(n, sw, first_slot, ev);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < n.D) {
+    uint32_t ds = first_slot + threadIdx.x;
+    if (ds >= n.R) ds -= n.R;
+    if (ev[threadIdx.x]) atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)ev[threadIdx.x]);
+  }
+}
+
 // LDS of one update block.
 template <int BLK>
 struct UpdateSmem {
@@ -525,6 +569,10 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
         len[q] = v.y;
         total += v.y;
       }
+      if (heavy_block(n, total, AGENT)) {
+        if (threadIdx.x < nlist) post_heavy(n, t, sm.list[threadIdx.x]);
+        total = 0;
+      }
       for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
         uint2 sw[kPushBatch];
 #pragma unroll
@@ -546,7 +594,11 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     } else {
       // many ids: block-wide scan of the lanes' counts, binary search per entry
       sm.start[threadIdx.x] = r0;
-      const uint32_t total = block_exclusive_scan<BLK>(has ? r1 - r0 : 0u, sm.pref, sm.wave);
+      uint32_t total = block_exclusive_scan<BLK>(has ? r1 - r0 : 0u, sm.pref, sm.wave);
+      if (heavy_block(n, total, AGENT)) {
+        if (has) post_heavy(n, t, make_uint2(r0, r1 - r0));
+        total = 0;
+      }
       for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
         uint2 sw[kPushBatch];
 #pragma unroll
@@ -581,6 +633,8 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
   if (threadIdx.x == 0) {
     // the deferral stamp, by one block (every block takes the same decision)
     if (defer && blockIdx.x == 0) st_agent(&n.dstamp[slot], t + 1);
+    // the heavy list step t + 1 posts into was drained by this step's gather
+    if (!AGENT && n.heavy_min && blockIdx.x == 0) st_agent(&n.heavy_cnt[(t + 1) & 1], 0u);
     // CARRY: the caller sums the block's spikes and drive events over the
     // launch and adds them (and the last step's spike slot) at its end
     if (!CARRY) {
@@ -847,7 +901,11 @@ __device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32
 template <int BLK>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(7))) gather_kernel(DevNet n, uint32_t j) {
   extern __shared__ __align__(16) uint64_t dyn[];
+  __shared__ uint32_t hev[64];
   const uint64_t t = *n.t_base + j;
+  // the heavy pushes of step t - 1 (their delay-0 synapses are due at t)
+  const uint32_t hc = n.heavy_min && t > 0 ? n.heavy_cnt[(t - 1) & 1] : 0u;
+  if (hc) heavy_push<BLK>(n, t - 1, hc, hev);
   const uint64_t due = due_mask<false>(n, t);
   if (!due) return;
   for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
@@ -954,6 +1012,8 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   }
   extern __shared__ __align__(16) uint64_t dyn[];
   const uint32_t id = blockIdx.x * BLK + threadIdx.x;
+  // heavy_kernel has pushed the ranges step t0 - 1 posted
+  if (n.heavy_min && blockIdx.x == 0 && threadIdx.x == 0) st_agent(&n.heavy_cnt[(t0 - 1) & 1], 0u);
   LaneState<MODEL> ls;
   if (carry) {
     load_lane<MODEL>(n, id, ls);
@@ -1018,6 +1078,16 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   if (blockIdx.x == 0 && threadIdx.x == 0) *t_base = t0 + nsteps;
 }
 
+// The heavy pushes posted by step t_base - 1, before a persistent launch
+// (whose blocks all push their own ids); the persistent kernel then clears
+// the list's count.
+__global__ void __launch_bounds__(256) heavy_kernel(DevNet n) {
+  __shared__ uint32_t hev[64];
+  const uint64_t t = *n.t_base;
+  const uint32_t hc = t > 0 ? n.heavy_cnt[(t - 1) & 1] : 0u;
+  if (hc) heavy_push<256>(n, t - 1, hc, hev);
+}
+
 __global__ void advance_kernel(uint64_t* t_base, uint64_t k) { *t_base += k; }
 
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t value, uint64_t count) {
@@ -1064,6 +1134,12 @@ int gather_max_blocks_per_cu(size_t dyn, uint32_t block) {
                                  : reinterpret_cast<const void*>(&gather_kernel<1024>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)block, dyn) != hipSuccess) return 0;
   return nb;
+}
+
+hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s) {
+  if (!n.heavy_min) return hipSuccess;
+  heavy_kernel<<<grid ? grid : 1, 256, 0, s>>>(n);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {

@@ -191,6 +191,7 @@ int run_steps(szsnn_net* net, uint64_t n) {
     if (n == 0) return SZSNN_OK;
     const uint32_t g = persistent_grid(net, net->tiling.persist_block, net->tiling.persist_grid);
     if (g == 0) return fail(ctx, SZSNN_ERR_INVALID, "persistent mode: block size has no co-resident occupancy");
+    HIP_TRY(ctx, launch_heavy(net->dn, net->tiling.gather_grid, ctx->stream));
     HIP_TRY(ctx, launch_persistent(net->dn, n, net->t_base, net->tiling.persist_block, g, ctx->stream));
     net->t_host += n;
     return SZSNN_OK;
@@ -452,7 +453,13 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint16_t* d_pull_syn16 = nullptr;
   int32_t* d_pull_wtab = nullptr;
   uint32_t* d_pull_lane = nullptr;
+  uint32_t* heavy_cnt = nullptr;
+  uint2* heavy_list = nullptr;
   if (rc == SZSNN_OK) guard(dev_alloc(net, &hist, (size_t)n.R * n.W));
+  // heavy pushes are drained by the gather launch of the next step, so they
+  // exist with a pull layout only
+  if (rc == SZSNN_OK && !pull_off.empty())
+    guard(dev_alloc(net, &heavy_cnt, 2)) && guard(dev_alloc(net, &heavy_list, (size_t)2 * T));
   if (rc == SZSNN_OK && !pull_off.empty())
     guard(dev_alloc(net, &d_pull_off, pull_off.size())) &&
         (L.pull_fmt == kPullWide    ? guard(dev_alloc(net, &d_pull_syn, pull_syn_size))
@@ -516,6 +523,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     if (he == hipSuccess) he = hipMemsetAsync(net->t_base, 0, 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(net->forced_bits, 0, (size_t)n.W * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(hist, 0, (size_t)n.R * n.W * 8, s);
+    if (he == hipSuccess && heavy_cnt) he = hipMemsetAsync(heavy_cnt, 0, 2 * 4, s);
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     if (he != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_net_upload: ") + hipGetErrorString(he));
   }
@@ -561,6 +569,9 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.pull_lo = d_pull_lo;
   n.pull_nb = d_pull_off ? (uint32_t)NB : 0;
   n.pull_words = pull_words;
+  n.heavy_min = heavy_cnt ? kHeavyPushMin : 0;
+  n.heavy_cnt = heavy_cnt;
+  n.heavy_list = heavy_list;
   net->active_ids = active;
   n.defer_spikes = defer_threshold(net, net->dense_fraction);
 
@@ -897,6 +908,28 @@ int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction) {
   return SZSNN_OK;
 }
 
+int szsnn_set_heavy_push(szsnn_net* net, uint32_t min_entries) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_heavy_push: net is NULL");
+  szsnn_ctx* ctx = net->ctx;
+  if (!net->dn.heavy_cnt) {
+    if (min_entries) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_heavy_push: the network has no pull layout");
+    return SZSNN_OK;
+  }
+  // the ranges the last step posted are pushed under the old threshold,
+  // then both lists start empty
+  HIP_TRY(ctx, launch_heavy(net->dn, net->tiling.gather_grid, ctx->stream));
+  HIP_TRY(ctx, hipMemsetAsync(net->dn.heavy_cnt, 0, 2 * 4, ctx->stream));
+  net->dn.heavy_min = min_entries;
+  net->graph_dirty = true;
+  return SZSNN_OK;
+}
+
+int szsnn_get_heavy_push(szsnn_net* net, uint32_t* min_entries) {
+  if (!net || !min_entries) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_heavy_push: bad argument");
+  *min_entries = net->dn.heavy_min;
+  return SZSNN_OK;
+}
+
 int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_pull) {
   if (!net || !dense_fraction || !has_pull)
     return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_gather_policy: bad argument");
@@ -963,6 +996,10 @@ std::vector<Region> state_regions(szsnn_net* net) {
   };
   if (n.src_last) r.push_back({n.src_last, (size_t)n.M * 4});
   if (n.acc_pull) r.push_back({n.acc_pull, (size_t)n.pull_groups * 2 * n.npad * 8});
+  if (n.heavy_cnt) {
+    r.push_back({n.heavy_cnt, 2 * 4});
+    r.push_back({n.heavy_list, (size_t)2 * n.T * 8});
+  }
   return r;
 }
 

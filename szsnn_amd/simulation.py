@@ -202,6 +202,18 @@ class Simulation:
         check(self.lib.szsnn_get_gather_policy(self.handle, ctypes.byref(f), ctypes.byref(h)), self.ctx.handle)
         return {"dense_fraction": f.value, "has_pull": bool(h.value)}
 
+    def set_heavy_push(self, min_entries):
+        """On a pushed step, an update block whose spiking ids carry at least
+        min_entries synapse entries hands them to the next gather launch,
+        which adds them with its whole grid (0: every block pushes its own).
+        Needs the pull layout unless min_entries is 0."""
+        check(self.lib.szsnn_set_heavy_push(self.handle, int(min_entries)), self.ctx.handle)
+
+    def heavy_push(self):
+        m = ctypes.c_uint32()
+        check(self.lib.szsnn_get_heavy_push(self.handle, ctypes.byref(m)), self.ctx.handle)
+        return m.value
+
     def set_mode(self, mode, block=0, grid=0):
         check(self.lib.szsnn_set_mode(self.handle, int(mode), int(block), int(grid)), self.ctx.handle)
 
