@@ -54,6 +54,10 @@ def parse():
                     help="profiler pass: the batched Izhikevich instances only, --warmup then --batch-steps steps")
     ap.add_argument("--adjoint-pass", action="store_true",
                     help="profiler pass: the adjoint forward + backward only (2 untimed, 5 timed batches)")
+    ap.add_argument("--burst-pass", action="store_true",
+                    help="timing pass: ids 0..S-1 forced every step (a concentrated burst), graph mode, pushed "
+                         "with heavy pushes at the default threshold, pushed by each block itself, and pulled")
+    ap.add_argument("--burst-ids", default="50,250,1000,4000", help="burst pass: the values of S")
     ap.add_argument("--gather-block", type=int, default=256, help="profile pass: pull gather block size")
     ap.add_argument("--gather-grid", type=int, default=4096, help="profile pass: pull gather grid")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_pull_volley_traffic.json"),
@@ -283,6 +287,32 @@ def main():
                           "launches": n, "gather_us": g_ms * 1e3 / n, "update_us": u_ms * 1e3 / n,
                           "events_per_launch": ev, "rows_per_launch": rw, "alg_bytes_per_launch": per,
                           "gbs": per / (g_ms * 1e-3 / n) / 1e9}), flush=True)
+        return
+    if args.burst_pass:
+        sim.set_mode(0)
+        heavy = sim.heavy_push()
+        rows = []
+        for S in (int(x) for x in args.burst_ids.split(",")):
+            for name, policy, threshold in (("push_heavy", 2.0, heavy), ("push_block", 2.0, 0), ("pull", 0.0, heavy)):
+                sim.set_heavy_push(threshold)
+                sim.set_gather_policy(policy)
+                sim.set_volley(S)
+                sim.run(args.warmup)
+                ctx.synchronize()
+                c0 = sim.counters()
+                t1 = time.perf_counter()
+                sim.run(args.steps)
+                ctx.synchronize()
+                wall = time.perf_counter() - t1
+                c1 = sim.counters()
+                g_ms, u_ms = sim.time_kernels(args.volley_steps)
+                rows.append({"ids": S, "mode": name, "us_per_step": wall * 1e6 / args.steps,
+                             "events_per_step": (c1["events"] - c0["events"]) / args.steps,
+                             "gather_us": g_ms * 1e3 / args.volley_steps, "update_us": u_ms * 1e3 / args.volley_steps})
+                log(f"[bench] burst {rows[-1]}")
+            sim.set_volley(0)
+        print(json.dumps({"burst_pass": True, "heavy_threshold": heavy, "steps": args.steps, "runs": rows}),
+              flush=True)
         return
     if args.natural_pass:
         sim.set_mode(1, 512, 256)
