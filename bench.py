@@ -394,9 +394,11 @@ def main():
         g_s = gv_ms * 1e-3 / args.volley_steps
         u_s = uv_ms * 1e-3 / args.volley_steps
         return {"gather_us": g_s * 1e6, "update_us": u_s * 1e6, "events_per_launch": ev, "rows_per_launch": rw}
-    # push only: the update kernel pushes 1e8 synapses per step with atomics
+    # push only: 1e8 synapses per step added with atomics (every update block
+    # holds 512k synapse entries, so it posts them as heavy pushes and the
+    # next gather launch adds them; the step is both kernels)
     v_push = volley(2.0)
-    v_push["push_events_per_s"] = v_push["events_per_launch"] / (v_push["update_us"] * 1e-6)
+    v_push["push_events_per_s"] = v_push["events_per_launch"] / ((v_push["update_us"] + v_push["gather_us"]) * 1e-6)
     log(f"[bench] volley push-only: {v_push}")
     # default policy: every step is deferred and pulled
     v_auto = volley(default_policy)
