@@ -187,8 +187,9 @@ constexpr uint32_t kHeavyPushMin = 8192;
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
 // gather: the pull of step t (launches nothing without a pull layout).
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);
-// The heavy pushes posted by step t_base - 1 (before a persistent launch; the
-// gather launch of a step does the same for its previous step).
+// The heavy pushes posted by step t_base - 1, then both lists emptied (before
+// a persistent launch and at threshold changes; the gather launch of a step
+// pushes its previous step's list).
 hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s);
 // update: block 64, 128, 256, 512 or 1024.
 hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s);

@@ -1012,8 +1012,6 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   }
   extern __shared__ __align__(16) uint64_t dyn[];
   const uint32_t id = blockIdx.x * BLK + threadIdx.x;
-  // heavy_kernel has pushed the ranges step t0 - 1 posted
-  if (n.heavy_min && blockIdx.x == 0 && threadIdx.x == 0) st_agent(&n.heavy_cnt[(t0 - 1) & 1], 0u);
   LaneState<MODEL> ls;
   if (carry) {
     load_lane<MODEL>(n, id, ls);
@@ -1079,13 +1077,17 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
 }
 
 // The heavy pushes posted by step t_base - 1, before a persistent launch
-// (whose blocks all push their own ids); the persistent kernel then clears
-// the list's count.
+// (whose blocks all push their own ids); heavy_clear_kernel then empties
+// both lists.
 __global__ void __launch_bounds__(256) heavy_kernel(DevNet n) {
   __shared__ uint32_t hev[64];
   const uint64_t t = *n.t_base;
   const uint32_t hc = t > 0 ? n.heavy_cnt[(t - 1) & 1] : 0u;
   if (hc) heavy_push<256>(n, t - 1, hc, hev);
+}
+
+__global__ void heavy_clear_kernel(uint32_t* heavy_cnt) {
+  if (threadIdx.x < 2) heavy_cnt[threadIdx.x] = 0;
 }
 
 __global__ void advance_kernel(uint64_t* t_base, uint64_t k) { *t_base += k; }
@@ -1139,6 +1141,7 @@ int gather_max_blocks_per_cu(size_t dyn, uint32_t block) {
 hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s) {
   if (!n.heavy_min) return hipSuccess;
   heavy_kernel<<<grid ? grid : 1, 256, 0, s>>>(n);
+  heavy_clear_kernel<<<1, 64, 0, s>>>(n.heavy_cnt);
   return hipGetLastError();
 }
 

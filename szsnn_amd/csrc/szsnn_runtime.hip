@@ -918,7 +918,6 @@ int szsnn_set_heavy_push(szsnn_net* net, uint32_t min_entries) {
   // the ranges the last step posted are pushed under the old threshold,
   // then both lists start empty
   HIP_TRY(ctx, launch_heavy(net->dn, net->tiling.gather_grid, ctx->stream));
-  HIP_TRY(ctx, hipMemsetAsync(net->dn.heavy_cnt, 0, 2 * 4, ctx->stream));
   net->dn.heavy_min = min_entries;
   net->graph_dirty = true;
   return SZSNN_OK;
