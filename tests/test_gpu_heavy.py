@@ -121,3 +121,31 @@ def test_network_without_pull_layout_rejects_a_threshold(ctx):
     s.set_heavy_push(0)
     with pytest.raises(SzsnnError):
         s.set_heavy_push(100)
+
+
+def test_forced_burst_through_step_matches_oracle(ctx):
+    """Bursts of 250 contiguous ids forced through single steps (every third
+    step, ids 300..549: 25k synapse entries, posted as heavy pushes at the
+    default threshold) against the FP32 oracle given the same forced ids."""
+    from oracle.pyoracle import Oracle
+    from szsnn_amd import configs
+    from szsnn_amd.simulation import Simulation
+    net, steps = configs.izhikevich2003(), 120
+    forced = [(t, i) for t in range(0, steps, 3) for i in range(300, 550)]
+    by_t = {}
+    for t, i in forced:
+        by_t.setdefault(t, []).append(i)
+    print("case: forced bursts through step(), default heavy threshold", flush=True)
+    s = Simulation(ctx, net)
+    assert s.heavy_push() > 0 and 250 * 100 >= s.heavy_push()
+    s.set_gather_policy(2.0)
+    s.set_recording(0, steps)
+    for t in range(steps):
+        s.step(by_t.get(t))
+    r, v, c = s.raster(), s.get_vars(), s.counters()
+    o = Oracle(net, 32)
+    ro, _ = o.run(steps, forced=forced)
+    bad = np.flatnonzero((r != ro).any(axis=1))
+    assert bad.size == 0, f"raster differs first at step {bad[0]}"
+    np.testing.assert_array_equal(v, o.vars().astype(np.float32))
+    assert c["events"] == o.events > steps // 3 * 250 * 100
