@@ -240,10 +240,17 @@ __device__ inline bool heavy_block(const DevNet& n, uint32_t total, bool agent) 
   return !agent && n.heavy_min && total >= n.heavy_min;
 }
 
-// Posts the synapse range (first, count) of a spiking id of step t.
-__device__ inline void post_heavy(const DevNet& n, uint64_t t, uint2 range) {
-  const uint32_t q = atomicAdd(&n.heavy_cnt[t & 1], 1u);
-  n.heavy_list[(size_t)(t & 1) * n.T + q] = range;
+// Posts the synapse ranges (first, count) of the wave's lanes with `has`
+// set, for step t: one counter atomic per wave.  Every lane of the wave
+// must call it.
+__device__ inline void post_heavy(const DevNet& n, uint64_t t, bool has, uint2 range) {
+  const uint64_t m = __ballot(has);
+  if (!m) return;
+  uint32_t base = 0;
+  if (__lane_id() == 0) base = atomicAdd(&n.heavy_cnt[t & 1], (uint32_t)__popcll(m));
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+  const uint32_t q = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  if (has) n.heavy_list[(size_t)(t & 1) * n.T + q] = range;
 }
 
 // Pushes the hc heavy ranges posted by step s: range q by block q (mod the
@@ -570,7 +577,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
         total += v.y;
       }
       if (heavy_block(n, total, AGENT)) {
-        if (threadIdx.x < nlist) post_heavy(n, t, sm.list[threadIdx.x]);
+        if (threadIdx.x < 64) post_heavy(n, t, threadIdx.x < nlist, sm.list[threadIdx.x < nlist ? threadIdx.x : 0]);
         total = 0;
       }
       for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
@@ -596,7 +603,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       sm.start[threadIdx.x] = r0;
       uint32_t total = block_exclusive_scan<BLK>(has ? r1 - r0 : 0u, sm.pref, sm.wave);
       if (heavy_block(n, total, AGENT)) {
-        if (has) post_heavy(n, t, make_uint2(r0, r1 - r0));
+        post_heavy(n, t, has, make_uint2(r0, r1 - r0));
         total = 0;
       }
       for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
