@@ -186,7 +186,9 @@ int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_bl
  * atomics).  A step is deferred when its predicted spikes (the previous
  * step's plus the forced ids) reach dense_fraction x the ids with synapses;
  * dense_fraction <= 0 always defers, > 1 never does.  Both give identical
- * sums.  has_pull reports whether the layout was built. */
+ * sums.  The default follows the pull entry width (where pushing a step
+ * costs as much as pulling): 0.012 for 2-byte entries, 0.02 for 4-byte,
+ * 0.04 for 8-byte.  has_pull reports whether the layout was built. */
 int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction);
 int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_pull);
 
@@ -196,7 +198,7 @@ int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_
  * entries posts their synapse ranges instead of adding them itself, and the
  * next step's gather launch adds them with its whole grid, so a burst
  * concentrated in a few blocks is not limited by one CU's atomic rate.  The
- * sums are identical either way.  Default 8192; 0 = every block pushes its
+ * sums are identical either way.  Default 1024; 0 = every block pushes its
  * own ids.  Setting it on a network without a pull layout is an error
  * unless min_entries is 0. */
 int szsnn_set_heavy_push(szsnn_net* net, uint32_t min_entries);

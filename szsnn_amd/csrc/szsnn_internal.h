@@ -72,6 +72,9 @@ struct DevNet {
   unsigned long long* spk_ring;  // [4][kRingStripes] spikes of step s in slot s % 4
   uint32_t defer_spikes;    // defer step t when its predicted spikes >= this (0xffffffff: never)
   uint32_t forced_count;    // ids forced by a host step (for the prediction)
+  uint32_t forced_prev;     // ids forced by the host step forced_prev_t (0: none)
+  uint64_t forced_prev_t;
+  uint64_t volley_t0;       // first step forced with the current volley
   uint64_t* raster;         // [rec_steps][W] or null
   uint64_t rec_first;
   uint64_t rec_steps;
@@ -180,9 +183,26 @@ __host__ __device__ inline unsigned long long* spk_counter(unsigned long long* b
   return base + ((size_t)(step & 3) * kRingStripes + (stripe % kRingStripes)) * kCntStride;
 }
 
+// Whether step t is deferred to the pull: its predicted spikes reach
+// defer_spikes.  The prediction is the previous step's spikes less the ids
+// forced in it (they are forced again or not, as this step says) plus the
+// ids forced in this step.  Inside a persistent launch (no host-forced ids,
+// the volley forced in every step) it is the previous step's spikes from
+// the launch's second step on, which the loop uses directly.
+__device__ inline bool defer_step(const DevNet& n, uint64_t t, unsigned long long spk_prev) {
+  if (!n.pull_off) return false;
+  const unsigned long long fprev =
+      (t > n.volley_t0 ? n.volley : 0u) + (n.forced_prev && t == n.forced_prev_t + 1 ? n.forced_prev : 0u);
+  const unsigned long long own = spk_prev > fprev ? spk_prev - fprev : 0ull;
+  return own + n.volley + n.forced_count >= n.defer_spikes;
+}
+
 // Default heavy-push threshold: synapse entries of one update block's spiking
-// ids (16 per thread of a 512-thread block).
-constexpr uint32_t kHeavyPushMin = 8192;
+// ids (one push batch of a 256-thread block).  Lower thresholds move the
+// pushes of networks with short rows into gather blocks that each take one
+// row per pass; higher ones leave bursts of a few thousand spread ids to the
+// update blocks, whose pushes run at a small share of the device's atomic rate.
+constexpr uint32_t kHeavyPushMin = 1024;
 
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
 // gather: the pull of step t (launches nothing without a pull layout).

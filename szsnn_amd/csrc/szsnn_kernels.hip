@@ -307,6 +307,7 @@ struct UpdateSmem {
 struct StepFacts {
   uint64_t due;
   unsigned long long spk_prev;
+  bool defer;  // the step's deferral decision (carried: taken by the loop)
 };
 
 // Block-uniform facts of step t, gathered by one wave (the block's first)
@@ -440,9 +441,9 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
   __syncthreads();
   const uint64_t due = CARRY ? facts.due : sm.due;
   const unsigned long long spk_prev = CARRY ? facts.spk_prev : sm.spk_prev;
-  // step t is deferred when the spikes of step t - 1 plus the forced ids
-  // reach the threshold (the same decision in every block)
-  const bool defer = n.pull_off && spk_prev + n.volley + n.forced_count >= n.defer_spikes;
+  // step t is deferred when its predicted spikes reach the threshold (the
+  // same decision in every block)
+  const bool defer = CARRY ? facts.defer : defer_step(n, t, spk_prev);
   const uint32_t r0 = ls.r0, r1 = ls.r1;
   const uint64_t dbits = ls.dbits;
   bool spike = false;
@@ -663,7 +664,7 @@ template <int MODEL, int BLK, bool AGENT>
 __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
   LaneState<MODEL> ls;
   unsigned long long drive = 0;
-  update_step<MODEL, BLK, false, AGENT>(n, t, id, sm, ls, StepFacts{0, 0}, drive);
+  update_step<MODEL, BLK, false, AGENT>(n, t, id, sm, ls, StepFacts{0, 0, false}, drive);
 }
 
 template <int MODEL, int BLK>
@@ -1026,6 +1027,10 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   }
   __syncthreads();
   StepFacts f = f0;
+  // the first step's decision; after it a persistent launch forces only its
+  // volley, whose ids spike every step, so the prediction is the previous
+  // step's spikes
+  f.defer = defer_step(n, t0, f.spk_prev);
   const uint64_t dmask = n.D >= 64 ? ~0ull : (1ull << n.D) - 1ull;
   // counter fold of the carry path: lane L of wave 0 owns stripe q
   const uint32_t q = blockIdx.x + __lane_id() * gridDim.x;
@@ -1055,9 +1060,9 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       const uint32_t slot = (uint32_t)(t % n.R);
       if (folds) fold_v = atomicExch(ring_counter(n.ring_cnt, slot, q / kRingStripes, q % kRingStripes), 0ull);
       if (!grid_barrier(n.barrier, bar_sh, err, threadIdx.x == 0 ? spikes : 0ull, &sum)) return;
-      const bool deferred = n.pull_off && f.spk_prev + n.volley + n.forced_count >= n.defer_spikes;
-      f.due = ((f.due << 1) | (deferred ? 1ull : 0ull)) & dmask;
+      f.due = ((f.due << 1) | (f.defer ? 1ull : 0ull)) & dmask;
       f.spk_prev = sum;
+      f.defer = n.pull_off && sum >= n.defer_spikes;
     } else {
       for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
         update_block<MODEL, BLK, true>(n, t, base + threadIdx.x, usm);

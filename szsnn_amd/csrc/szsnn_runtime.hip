@@ -67,7 +67,9 @@ struct szsnn_net {
   // autotune snapshot buffers
   std::vector<std::pair<void*, size_t>> snap;
   uint64_t active_ids = 0;        // ids with at least one synapse
-  float dense_fraction = 0.04f;
+  uint32_t last_forced = 0;       // ids forced by the last szsnn_step, at step last_forced_t
+  uint64_t last_forced_t = 0;
+  float dense_fraction = 0.04f;   // set per pull entry format at upload
   // batched networks: per-instance id ranges (empty for a plain network)
   std::vector<uint32_t> inst_noff, inst_soff, inst_n, inst_m;
 };
@@ -135,6 +137,14 @@ uint32_t default_gather_grid(const szsnn_net* net, uint32_t block) {
   const uint64_t items = (uint64_t)net->dn.pull_nb * net->dn.pull_groups;
   const uint64_t cap = (uint64_t)std::max(net->ctx->cu_count, 1) * 8;
   return (uint32_t)std::max<uint64_t>(1, std::min(items, cap));
+}
+
+// Default gather policy per pull entry format: the fraction of ids spiking
+// at which pushing a step's synapses (atomics, about 2.3e10 per second with
+// heavy pushes) costs as much as pulling all synapses (measured on 1e8
+// synapses: 2-byte entries 1.6e12 per second, 4-byte 1.1e12, 8-byte 6.1e11).
+float default_dense_fraction(uint32_t fmt) {
+  return fmt == kPullCount ? 0.012f : fmt == kPullTable ? 0.02f : 0.04f;
 }
 
 // Deferral threshold of the gather policy: a step is deferred to the pull
@@ -550,6 +560,9 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.ring_cnt = ring_cnt;
   n.spk_ring = spk_ring;
   n.forced_count = 0;
+  n.forced_prev = 0;
+  n.forced_prev_t = 0;
+  n.volley_t0 = 0;
   n.raster = nullptr;
   n.rec_first = 0;
   n.rec_steps = 0;
@@ -573,6 +586,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.heavy_cnt = heavy_cnt;
   n.heavy_list = heavy_list;
   net->active_ids = active;
+  net->dense_fraction = default_dense_fraction(d_pull_off ? L.pull_fmt : kPullWide);
   n.defer_spikes = defer_threshold(net, net->dense_fraction);
 
   net->info.n_neurons = (uint32_t)N;
@@ -735,6 +749,10 @@ int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, ui
     dn.forced_count = n_forced;
   }
   const uint64_t t = net->t_host;
+  // the previous host step's forced ids, taken out of its spike count in
+  // this step's deferral prediction
+  dn.forced_prev = net->last_forced;
+  dn.forced_prev_t = net->last_forced_t;
   uint64_t* row = nullptr;
   if (dn.raster && t >= dn.rec_first && t - dn.rec_first < dn.rec_steps) {
     row = dn.raster + (t - dn.rec_first) * dn.W;
@@ -747,6 +765,8 @@ int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, ui
   int rc = launch_step(net, dn, 0);
   if (rc) return rc;
   HIP_TRY(ctx, launch_advance(net->t_base, 1, ctx->stream));
+  net->last_forced = n_forced;
+  net->last_forced_t = t;
   net->t_host += 1;
   if (out_bitmap)
     HIP_TRY(ctx, hipMemcpyAsync(out_bitmap, row, dn.W * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -768,6 +788,7 @@ int szsnn_run(szsnn_net* net, uint64_t n_steps) {
 int szsnn_set_volley(szsnn_net* net, uint32_t n_forced_prefix) {
   if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_set_volley: net is NULL");
   if (n_forced_prefix > net->dn.T) return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_volley: prefix beyond the network");
+  if (n_forced_prefix != net->dn.volley) net->dn.volley_t0 = net->t_host;
   net->dn.volley = n_forced_prefix;
   net->graph_dirty = true;
   return SZSNN_OK;

@@ -309,11 +309,13 @@ def _run_policy(ctx, net, steps, fraction, volley=0):
 
 def _mixed_fraction(net, raster, volley):
     """A policy fraction under which about half of the steps are deferred:
-    step t is deferred when spikes(t-1) + volley >= ceil(fraction x ids with
-    synapses).  Returns (fraction, deferred steps)."""
+    step t is deferred when its predicted spikes, spikes(t-1) less the
+    volley forced in step t-1 plus the volley of step t, reach ceil(fraction
+    x ids with synapses) (the volley runs from step 0).  Returns (fraction,
+    deferred steps)."""
     from szsnn_amd.simulation import raster_to_bool
     counts = raster_to_bool(raster, net.n_neurons + net.n_sources).sum(axis=1).astype(np.int64)
-    pred = np.concatenate([[0], counts[:-1]]) + volley
+    pred = np.concatenate([[volley], counts[:-1]])
     active = np.unique(net.syn_src[net.syn_w != 0]).size
     thr = int(np.median(pred[1:])) + 1
     frac = (thr - 0.5) / active
