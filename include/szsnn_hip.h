@@ -184,7 +184,8 @@ int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_bl
  * later step then pulls, per target block, the synapses of the deferred steps
  * due at it against their staged spike bitmaps (cost ~ synapses, no global
  * atomics).  A step is deferred when its predicted spikes (the previous
- * step's plus the forced ids) reach dense_fraction x the ids with synapses;
+ * step's less the ids forced in it, plus the ids forced in this one) reach
+ * dense_fraction x the ids with synapses;
  * dense_fraction <= 0 always defers, > 1 never does.  Both give identical
  * sums.  The default follows the pull entry width (where pushing a step
  * costs as much as pulling): 0.012 for 2-byte entries, 0.02 for 4-byte,

@@ -193,8 +193,10 @@ class Simulation:
 
     def set_gather_policy(self, dense_fraction):
         """Defer a step's spikes to the pull gather when the spikes predicted
-        for it (the previous step's plus the forced ones) reach
-        dense_fraction x the ids with synapses (<= 0: always, > 1: never)."""
+        for it (the previous step's less the ids forced in it, plus the ids
+        forced in this one) reach dense_fraction x the ids with synapses
+        (<= 0: always, > 1: never).  The default follows the pull entry
+        width: 0.012 (2-byte), 0.02 (4-byte), 0.04 (8-byte)."""
         check(self.lib.szsnn_set_gather_policy(self.handle, float(dense_fraction)), self.ctx.handle)
 
     def gather_policy(self):
