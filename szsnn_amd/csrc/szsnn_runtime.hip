@@ -279,7 +279,9 @@ int szsnn_create(int device, szsnn_ctx** out) {
     return fail(nullptr, SZSNN_ERR_HIP, std::string("szsnn_create: ") + hipGetErrorString(e));
   }
   ctx->cu_count = prop.multiProcessorCount;
-  ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+  // the marketing name can be empty (no libdrm name table on the host)
+  const std::string pname = prop.name[0] ? std::string(prop.name) : "HIP device " + std::to_string(device);
+  ctx->name = pname + " (" + prop.gcnArchName + ")";
   *out = ctx;
   return SZSNN_OK;
 }
