@@ -35,11 +35,13 @@ def _ragged_net(seed=7, n=1001, m=37, max_delay=64, fanout=40):
                    src_thresh=poisson_thresholds(np.full(m, 0.05))).validate()
 
 
-def _run(ctx, net, steps, fraction=None, volley=0):
+def _run(ctx, net, steps, fraction=None, volley=0, heavy=None):
     from szsnn_amd.simulation import Simulation
     s = Simulation(ctx, net)
     if fraction is not None:
         s.set_gather_policy(fraction)
+    if heavy is not None:
+        s.set_heavy_push(heavy)
     if volley:
         s.set_volley(volley)
     s.set_recording(0, steps)
@@ -66,16 +68,18 @@ def test_network_without_synapses(ctx):
     assert c["events"] == 0 and o.events == 0 and c["spikes"] > 0
 
 
-@pytest.mark.parametrize("fraction", [2.0, 0.0])
-def test_largest_delay_ring_ragged_ids(ctx, fraction):
+@pytest.mark.parametrize("fraction,heavy", [(2.0, None), (2.0, 1), (2.0, 0), (0.0, None)])
+def test_largest_delay_ring_ragged_ids(ctx, fraction, heavy):
     """64 delay slots, 1001 neurons + 37 sources, a volley of 600 forced ids:
-    pushing every step and pulling every step (4-byte entries) both equal the
-    oracle bit for bit, including synapses of delay 63."""
+    pushing every step (by the update blocks, as heavy pushes at the default
+    threshold, or as heavy pushes from every block) and pulling every step
+    (2-byte entries) all equal the oracle bit for bit, including synapses of
+    delay 63, whose delivery slots wrap around the ring."""
     from oracle.pyoracle import Oracle
     net = _ragged_net()
     steps, volley = 140, 600
-    print(f"case: largest delay ring, policy {fraction}", flush=True)
-    r, v, c, s = _run(ctx, net, steps, fraction=fraction, volley=volley)
+    print(f"case: largest delay ring, policy {fraction}, heavy threshold {heavy}", flush=True)
+    r, v, c, s = _run(ctx, net, steps, fraction=fraction, volley=volley, heavy=heavy)
     assert s.info.pull_entry_bytes == 2 and s.info.max_delay == 64
     forced = [(t, i) for t in range(steps) for i in range(volley)]
     o = Oracle(net, 32)
