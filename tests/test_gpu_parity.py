@@ -396,9 +396,11 @@ def test_narrow_and_wide_pull_entries_agree(ctx, which):
         assert 20 < np.unique(net.syn_w[net.syn_w != 0]).size <= 255
     print(f"case: narrow vs wide pull entries ({which})", flush=True)
     narrow_bytes = 4 if which == "table" else 2
+    default_fraction = {2: 0.012, 4: 0.02, 8: 0.04}   # where a pushed step costs as much as a pull
     for layout, width in ((0, narrow_bytes), (1, 8)):
         s = Simulation(ctx, dataclasses.replace(net, pull_layout=layout))
         assert s.info.pull_entry_bytes == width
+        assert s.gather_policy()["dense_fraction"] == pytest.approx(default_fraction[width])
         del s
     rp, vp, cp = _run_policy(ctx, net, steps, 2.0, volley)                                    # push only
     rn, vn, cn = _run_policy(ctx, net, steps, 0.0, volley)                                    # pull, narrow
