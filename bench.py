@@ -1,21 +1,32 @@
 """Benchmark: synaptic events/s and realtime factor of the plain IF_cond_exp
 100k-neuron network (1e8 synapses, dt = 0.1 ms) on one MI355X.
 
-A "step" is one simulation step (gather -> update -> scatter) of the whole
-network.  Inputs (network, state) are resident in HBM before the timed region.
+A bench "step" is one --chunk of simulation steps (default 1000 steps of
+gather -> update -> scatter, i.e. 100 ms of network time): the network fires
+in population bursts of about 100 ms separated by quiet spells of about 50 ms,
+so single simulation steps are not a representative unit.  Inputs (network,
+state) are resident in HBM before the timed region.  The network is first run
+for --settle-steps simulation steps (default 10000: the whole 1000 ms of the
+configuration from its initial state, timed and reported as `full_run`), then
+--warmup untimed chunks, then --steps timed chunks; the run fails if the timed
+window delivers no synaptic event.
 Prints one JSON line (the driver's contract) with:
   value          synaptic events/s over the timed steps (natural dynamics)
   roofline       the pull gather kernel at the volley load (all neurons forced
-                 every step: every synapse delivered each step), HIP-event
-                 timed; algorithmic bytes = 8 B per delivered synapse (f32
-                 weight + u32 target) + 8 B per delivered (source, delay)
-                 row; `traffic` = HBM bytes per launch from the rocprofv3
-                 FETCH_SIZE / WRITE_SIZE passes recorded in profiles/ (the
-                 pull layout stores 2-byte entries when the network has at
-                 most 3 distinct weights, so traffic is below the
-                 algorithmic bytes; hbm_gbs is traffic / launch time)
-  cpu_baseline   the FP32 CPU restatement on the GPU run's timed steps, on
-                 every host core this process may use (and a one-thread sample)
+                 every step: every synapse delivered each step), each launch
+                 preceded by a 2 GiB write sweep that evicts L2 and the
+                 Infinity Cache (cold), HIP-event timed on the launch stream;
+                 `traffic` = HBM bytes per launch from the rocprofv3
+                 FETCH_SIZE / WRITE_SIZE passes of the same cold workload
+                 recorded in profiles/, `achieved` = traffic / launch time,
+                 `frac` = achieved / 8 TB/s.  The survey's algorithmic unit
+                 (8 B per delivered synapse + 8 B per delivered (source,
+                 delay) row) is reported beside it as `alg_*`: the pull layout
+                 stores 2-byte entries, so that unit is not a byte count of
+                 this kernel
+  cpu_baseline   the FP32 CPU restatement over the same settle run and the
+                 same timed window, on every host core this process may use
+                 (and a one-thread sample)
 """
 import argparse
 import json
@@ -33,10 +44,15 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=10, help="timed chunks")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed chunks before the timed ones")
+    ap.add_argument("--chunk", type=int, default=1000, help="simulation steps per bench step")
+    ap.add_argument("--settle-steps", type=int, default=10000,
+                    help="untimed-for-value steps run first (the full 1000 ms network run, reported as full_run)")
+    ap.add_argument("--flush-mib", type=int, default=2048,
+                    help="cold-cache gather timing: MiB written between launches to evict L2 and the Infinity Cache")
     ap.add_argument("--volley-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=2000, help="one-thread CPU baseline sample (steps of the same net)")
+    ap.add_argument("--cpu-steps", type=int, default=1000, help="one-thread CPU baseline sample (steps of the same net)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="scaled-down network (smoke use only)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
@@ -47,6 +63,8 @@ def parse():
     ap.add_argument("--profile-pass", action="store_true",
                     help="profiler pass: a 1 GiB device copy (calibration), the volley's delay slots filled by "
                          "one persistent launch, then exactly --volley-steps gather_kernel launches")
+    ap.add_argument("--cold", action="store_true",
+                    help="profile pass: a --flush-mib write sweep before every gather launch")
     ap.add_argument("--natural-pass", action="store_true",
                     help="profiler pass: natural dynamics, --warmup steps then --steps steps as ONE persistent "
                          "launch (block 512, grid 256), then --volley-steps single-step gather+update launches")
@@ -58,10 +76,15 @@ def parse():
                     help="timing pass: ids 0..S-1 forced every step (a concentrated burst), graph mode, pushed "
                          "with heavy pushes at the default threshold, pushed by each block itself, and pulled")
     ap.add_argument("--burst-ids", default="50,250,1000,4000", help="burst pass: the values of S")
+    ap.add_argument("--burst-steps", type=int, default=200, help="burst pass: timed simulation steps per case")
     ap.add_argument("--gather-block", type=int, default=256, help="profile pass: pull gather block size")
     ap.add_argument("--gather-grid", type=int, default=4096, help="profile pass: pull gather grid")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "gather_pull_volley_traffic.json"),
-                    help="HBM bytes per pull gather launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)")
+                    help="HBM bytes per pull gather launch (warm: launches back to back) measured with "
+                         "rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)")
+    ap.add_argument("--cold-traffic-file",
+                    default=os.path.join(ROOT, "profiles", "gather_pull_volley_cold_traffic.json"),
+                    help="the same for cold launches (a cache-evicting sweep before each)")
     return ap.parse_args()
 
 
@@ -95,6 +118,44 @@ def copy_bandwidth_gbs(torch, nbytes=1 << 30, reps=10):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+class CacheFlush:
+    """A device buffer several times the Infinity Cache (256 MB) and the L2s;
+    writing it evicts whatever a following kernel would otherwise find there."""
+
+    def __init__(self, torch, mib):
+        self.torch = torch
+        self.buf = torch.empty(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        self.k = 0.0
+
+    def __call__(self):
+        self.k += 1.0
+        self.buf.fill_(self.k)
+        self.torch.cuda.synchronize()
+
+    def close(self):
+        del self.buf
+        self.torch.cuda.empty_cache()
+
+
+def time_cold(sim, flush, n):
+    """n single steps, each after a cache-evicting sweep: summed gather and
+    update kernel times (ms, HIP events on the library's stream)."""
+    g = u = 0.0
+    for _ in range(n):
+        flush()
+        a, b = sim.time_kernels(1)
+        g += a
+        u += b
+    return g, u
+
+
+def read_json(path):
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
 
 
 def run_adjoint(args, log):
@@ -175,7 +236,7 @@ def run_batched(args, ctx, log):
     ev_b = bs.counters()["events"] - c0["events"]
     log(f"[bench] batched run: {wall_b * 1e3:.1f} ms for {T} steps, {ev_b / wall_b:.3e} events/s")
     per_inst = bs.instance_rasters()
-    del bs
+    bs.close()
     # each instance run alone, once in graph mode and once as a persistent
     # cooperative launch; the comparison uses the faster of the two per instance
     wall = {"graph": 0.0, "persistent": 0.0, "best": 0.0}
@@ -198,7 +259,7 @@ def run_batched(args, ctx, log):
             if mode == "graph":
                 ev_p += s.counters()["events"]
             same = same and np.array_equal(raster_to_bool(s.raster(), net.n_neurons), per_inst[b])
-            del s
+            s.close()
         wall["best"] += best
         if b % 32 == 31:
             log(f"[bench] plain runs: {b + 1}/{len(nets)}, {wall} s so far")
@@ -220,139 +281,201 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    import numpy as np
     import torch
-    from szsnn_amd.configs import if_cond_exp_100k
-    from szsnn_amd.simulation import Context, Simulation
+    from szsnn_amd.simulation import Context, close_all
 
-    if args.batched_pass:
-        from szsnn_amd.batch import BatchSimulation, batched_izhikevich
+    try:
+        if args.batched_pass:
+            return batched_pass(args)
+        if args.adjoint_pass:
+            args.no_cpu = True
+            out = run_adjoint(args, log)
+            print(json.dumps({"adjoint_pass": True, **out}), flush=True)
+            return
         ctx = Context(0)
-        nets = batched_izhikevich(n_instances=args.batch_instances)
-        bs = BatchSimulation(ctx, nets)
-        bs.run(args.warmup)
-        ctx.synchronize()
-        c0 = bs.counters()
-        t1 = time.perf_counter()
-        bs.run(args.batch_steps)
-        ctx.synchronize()
-        wall = time.perf_counter() - t1
-        ev = bs.counters()["events"] - c0["events"]
-        print(json.dumps({"batched_pass": True, "instances": len(nets), "neurons": sum(n.n_neurons for n in nets),
-                          "steps": args.batch_steps, "us_per_step": wall * 1e6 / args.batch_steps,
-                          "events_per_step": ev / args.batch_steps, "tiling": bs.tiling()}), flush=True)
-        return
-    if args.adjoint_pass:
-        args.no_cpu = True
-        out = run_adjoint(args, log)
-        print(json.dumps({"adjoint_pass": True, **out}), flush=True)
-        return
+        net, sim = upload_headline(args, ctx)
+        if args.profile_pass:
+            return profile_pass(args, torch, ctx, net, sim)
+        if args.burst_pass:
+            return burst_pass(args, ctx, sim)
+        if args.natural_pass:
+            return natural_pass(args, ctx, sim)
+        return headline(args, torch, ctx, net, sim)
+    finally:
+        # everything on the device is released here, in order, before the
+        # interpreter (and a profiler attached to the HIP runtime) tears down
+        close_all()
+        torch.cuda.synchronize()
 
+
+def upload_headline(args, ctx):
+    from szsnn_amd.configs import if_cond_exp_100k
+    from szsnn_amd.simulation import Simulation
     t0 = time.time()
     if args.small:
         net = if_cond_exp_100k(n_exc=8000, n_inh=2000, k_exc=400, k_inh=100)
     else:
         net = if_cond_exp_100k()
     log(f"[bench] network built in {time.time() - t0:.1f}s: {net.n_neurons} neurons, {net.n_syn} synapses")
-    dt_ms = net.meta["dt_ms"]
-    ctx = Context(0)
     t0 = time.time()
     sim = Simulation(ctx, net)
     log(f"[bench] uploaded in {time.time() - t0:.1f}s to {ctx.name}; device bytes {sim.info.device_bytes}")
+    return net, sim
 
-    if args.profile_pass:
-        # calibration: a plain copy of known size (FETCH_SIZE / WRITE_SIZE of 1 GiB each)
-        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
-        # fill every delay slot with one persistent launch (no gather_kernel
-        # dispatch), then launch gather_kernel exactly --volley-steps times
-        sim.set_tiling(gather_block=args.gather_block, gather_grid=args.gather_grid)
-        sim.set_mode(1)
-        sim.set_volley(net.n_neurons)
-        sim.run(net.max_delay + 2)
-        ctx.synchronize()
-        c0 = sim.counters()
+
+def batched_pass(args):
+    from szsnn_amd.batch import BatchSimulation, batched_izhikevich
+    from szsnn_amd.simulation import Context
+    ctx = Context(0)
+    nets = batched_izhikevich(n_instances=args.batch_instances)
+    bs = BatchSimulation(ctx, nets)
+    bs.run(args.warmup * args.chunk)
+    ctx.synchronize()
+    c0 = bs.counters()
+    t1 = time.perf_counter()
+    bs.run(args.batch_steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t1
+    ev = bs.counters()["events"] - c0["events"]
+    print(json.dumps({"batched_pass": True, "instances": len(nets), "neurons": sum(n.n_neurons for n in nets),
+                      "steps": args.batch_steps, "us_per_step": wall * 1e6 / args.batch_steps,
+                      "events_per_step": ev / args.batch_steps, "tiling": bs.tiling()}), flush=True)
+
+
+def profile_pass(args, torch, ctx, net, sim):
+    """Exactly --volley-steps gather_kernel launches at the volley load
+    (after one persistent launch has filled every delay slot), each after a
+    cache-evicting sweep with --cold; a 1 GiB device copy first calibrates
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    flush = CacheFlush(torch, args.flush_mib) if args.cold else None
+    sim.set_tiling(gather_block=args.gather_block, gather_grid=args.gather_grid)
+    sim.set_mode(1)
+    sim.set_volley(net.n_neurons)
+    sim.run(net.max_delay + 2)
+    ctx.synchronize()
+    c0 = sim.counters()
+    if flush:
+        g_ms, u_ms = time_cold(sim, flush, args.volley_steps)
+        flush.close()
+    else:
         g_ms, u_ms = sim.time_kernels(args.volley_steps)
-        c1 = sim.counters()
-        n = args.volley_steps
-        ev = (c1["events"] - c0["events"]) / n
-        rw = (c1["rows"] - c0["rows"]) / n
-        per = gather_alg_bytes(ev, rw)
-        print(json.dumps({"profile_pass": True, "gather_block": args.gather_block, "gather_grid": args.gather_grid,
-                          "launches": n, "gather_us": g_ms * 1e3 / n, "update_us": u_ms * 1e3 / n,
-                          "events_per_launch": ev, "rows_per_launch": rw, "alg_bytes_per_launch": per,
-                          "gbs": per / (g_ms * 1e-3 / n) / 1e9}), flush=True)
-        return
-    if args.burst_pass:
-        sim.set_mode(0)
-        heavy = sim.heavy_push()
-        rows = []
-        for S in (int(x) for x in args.burst_ids.split(",")):
-            for name, policy, threshold in (("push_heavy", 2.0, heavy), ("push_block", 2.0, 0), ("pull", 0.0, heavy)):
-                sim.set_heavy_push(threshold)
-                sim.set_gather_policy(policy)
-                sim.set_volley(S)
-                sim.run(args.warmup)
-                ctx.synchronize()
-                c0 = sim.counters()
-                t1 = time.perf_counter()
-                sim.run(args.steps)
-                ctx.synchronize()
-                wall = time.perf_counter() - t1
-                c1 = sim.counters()
-                g_ms, u_ms = sim.time_kernels(args.volley_steps)
-                rows.append({"ids": S, "mode": name, "us_per_step": wall * 1e6 / args.steps,
-                             "events_per_step": (c1["events"] - c0["events"]) / args.steps,
-                             "gather_us": g_ms * 1e3 / args.volley_steps, "update_us": u_ms * 1e3 / args.volley_steps})
-                log(f"[bench] burst {rows[-1]}")
-            sim.set_volley(0)
-        print(json.dumps({"burst_pass": True, "heavy_threshold": heavy, "steps": args.steps, "runs": rows}),
-              flush=True)
-        return
-    if args.natural_pass:
-        sim.set_mode(1, 512, 256)
-        sim.run(args.warmup)
-        ctx.synchronize()
-        c0 = sim.counters()
-        t1 = time.perf_counter()
-        sim.run(args.steps)
-        ctx.synchronize()
-        wall = time.perf_counter() - t1
-        c1 = sim.counters()
-        g_ms, u_ms = sim.time_kernels(args.volley_steps)
-        print(json.dumps({"natural_pass": True, "steps": args.steps, "us_per_step": wall * 1e6 / args.steps,
-                          "events_per_step": (c1["events"] - c0["events"]) / args.steps,
-                          "spikes_per_step": (c1["spikes"] - c0["spikes"]) / args.steps,
-                          "single_step_gather_us": g_ms * 1e3 / args.volley_steps,
-                          "single_step_update_us": u_ms * 1e3 / args.volley_steps}), flush=True)
-        return
-    if not args.no_autotune:
-        log(f"[bench] autotuned for the natural dynamics: {sim.autotune(200)}")
-    # warmup (also builds the step graph)
-    sim.run(args.warmup)
+    c1 = sim.counters()
+    n = args.volley_steps
+    ev = (c1["events"] - c0["events"]) / n
+    rw = (c1["rows"] - c0["rows"]) / n
+    per = gather_alg_bytes(ev, rw)
+    print(json.dumps({"profile_pass": True, "cold": bool(args.cold), "gather_block": args.gather_block,
+                      "gather_grid": args.gather_grid, "launches": n, "gather_us": g_ms * 1e3 / n,
+                      "update_us": u_ms * 1e3 / n, "events_per_launch": ev, "rows_per_launch": rw,
+                      "alg_bytes_per_launch": per, "alg_gbs": per / (g_ms * 1e-3 / n) / 1e9}), flush=True)
+
+
+def burst_pass(args, ctx, sim):
+    sim.set_mode(0)
+    heavy = sim.heavy_push()
+    rows = []
+    for S in (int(x) for x in args.burst_ids.split(",")):
+        for name, policy, threshold in (("push_heavy", 2.0, heavy), ("push_block", 2.0, 0), ("pull", 0.0, heavy)):
+            sim.set_heavy_push(threshold)
+            sim.set_gather_policy(policy)
+            sim.set_volley(S)
+            sim.run(args.burst_steps // 10)
+            ctx.synchronize()
+            c0 = sim.counters()
+            t1 = time.perf_counter()
+            sim.run(args.burst_steps)
+            ctx.synchronize()
+            wall = time.perf_counter() - t1
+            c1 = sim.counters()
+            g_ms, u_ms = sim.time_kernels(args.volley_steps)
+            rows.append({"ids": S, "mode": name, "us_per_step": wall * 1e6 / args.burst_steps,
+                         "events_per_step": (c1["events"] - c0["events"]) / args.burst_steps,
+                         "gather_us": g_ms * 1e3 / args.volley_steps, "update_us": u_ms * 1e3 / args.volley_steps})
+            log(f"[bench] burst {rows[-1]}")
+        sim.set_volley(0)
+    print(json.dumps({"burst_pass": True, "heavy_threshold": heavy, "steps": args.burst_steps, "runs": rows}),
+          flush=True)
+
+
+def natural_pass(args, ctx, sim):
+    sim.set_mode(1, 512, 256)
+    sim.run(args.settle_steps + args.warmup * args.chunk)
+    ctx.synchronize()
+    n = args.steps * args.chunk
+    c0 = sim.counters()
+    t1 = time.perf_counter()
+    sim.run(n)  # one persistent launch
+    ctx.synchronize()
+    wall = time.perf_counter() - t1
+    c1 = sim.counters()
+    g_ms, u_ms = sim.time_kernels(args.volley_steps)
+    print(json.dumps({"natural_pass": True, "settle_steps": args.settle_steps, "steps": n,
+                      "us_per_step": wall * 1e6 / n,
+                      "events_per_step": (c1["events"] - c0["events"]) / n,
+                      "spikes_per_step": (c1["spikes"] - c0["spikes"]) / n,
+                      "single_step_gather_us": g_ms * 1e3 / args.volley_steps,
+                      "single_step_update_us": u_ms * 1e3 / args.volley_steps}), flush=True)
+
+
+def timed_run(sim, ctx, n, chunk=None):
+    """(wall seconds, counter deltas) of sim.run(n) (or n calls of
+    sim.run(chunk)), bracketed by synchronizations."""
+    import torch
     torch.cuda.synchronize()
     ctx.synchronize()
     c0 = sim.counters()
     t1 = time.perf_counter()
-    sim.run(args.steps)
+    if chunk is None:
+        sim.run(n)
+    else:
+        for _ in range(n):
+            sim.run(chunk)
     ctx.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
     c1 = sim.counters()
-    events = c1["events"] - c0["events"]
-    spikes = c1["spikes"] - c0["spikes"]
-    rows = c1["rows"] - c0["rows"]
+    return wall, {k: c1[k] - c0[k] for k in ("events", "spikes", "rows", "drive_events")}
+
+
+def headline(args, torch, ctx, net, sim):
+    dt_ms = net.meta["dt_ms"]
+    if not args.no_autotune:
+        log(f"[bench] autotuned for the natural dynamics from the initial state: {sim.autotune(200)}")
+    # the whole configured run (1000 ms) from the initial state: the settle
+    # phase of the timed window, timed itself as the full-run figure
+    wall_f, cf = timed_run(sim, ctx, args.settle_steps)
+    full = {"steps": args.settle_steps, "simulated_ms": args.settle_steps * dt_ms, "wall_s": wall_f,
+            "events": cf["events"], "spikes": cf["spikes"],
+            "events_per_s": cf["events"] / wall_f if wall_f > 0 else None,
+            "realtime_factor": args.settle_steps * dt_ms / (wall_f * 1e3) if wall_f > 0 else None,
+            "firing_rate_hz": cf["spikes"] / net.n_neurons / (args.settle_steps * dt_ms * 1e-3)
+            if args.settle_steps else None}
+    log(f"[bench] full run: {full}")
+    if not args.no_autotune:
+        # tuned again on the settled dynamics (the state is restored)
+        log(f"[bench] autotuned for the settled dynamics: {sim.autotune(500)}")
+    sim.run(args.warmup * args.chunk)
+    wall, cw = timed_run(sim, ctx, args.steps, args.chunk)
+    events, spikes = cw["events"], cw["spikes"]
+    n_sim = args.steps * args.chunk
+    w0 = args.settle_steps + args.warmup * args.chunk
+    window = f"simulation steps {w0}..{w0 + n_sim}"
+    if events <= 0:
+        raise SystemExit(f"bench: the timed window ({window}) delivered no synaptic event")
     ms_per_step = wall * 1e3 / args.steps
     ev_per_s = events / wall
-    rtf = args.steps * dt_ms / (wall * 1e3)
-    rate_hz = spikes / net.n_neurons / (args.steps * dt_ms * 1e-3)
-    log(f"[bench] natural: {ms_per_step * 1e3:.2f} us/step, {ev_per_s:.3e} events/s, RTF {rtf:.2f}, "
-        f"{rate_hz:.2f} Hz, {events / args.steps:.0f} events/step")
+    rtf = n_sim * dt_ms / (wall * 1e3)
+    rate_hz = spikes / net.n_neurons / (n_sim * dt_ms * 1e-3)
+    log(f"[bench] natural: {ms_per_step:.3f} ms per {args.chunk}-step chunk ({wall * 1e6 / n_sim:.2f} us per "
+        f"simulation step), {ev_per_s:.3e} events/s, RTF {rtf:.2f}, {rate_hz:.2f} Hz, "
+        f"{events / n_sim:.0f} events per simulation step")
 
     # per-kernel time on the natural dynamics (HIP events, same stream); the
     # update kernel also pushes the step's spikes
@@ -365,20 +488,17 @@ def main():
     # synapse range and delay bits; per pushed synapse: its 8-byte entry
     nat_u_bytes = ((4 * info.n_params + 8 * info.n_vars + 16 + 24) * net.n_neurons * kn
                    + 8 * (cg1["events"] - cg0["events"]))
-    natural = {
-        "gather_us": g_ms * 1e3 / kn, "update_us": u_ms * 1e3 / kn,
-        "update_gbs": nat_u_bytes / (u_ms * 1e-3) / 1e9 if u_ms > 0 else None,
-    }
+    natural = {"gather_us": g_ms * 1e3 / kn, "update_us": u_ms * 1e3 / kn,
+               "update_gbs": nat_u_bytes / (u_ms * 1e-3) / 1e9 if u_ms > 0 else None}
     log(f"[bench] natural kernels: {natural}")
-
-    # volley: every neuron forced every step -> every (source, delay) row is due
     natural_tiling = (sim.tiling(), sim.mode())
 
     default_policy = sim.gather_policy()["dense_fraction"]
     groups = int(sim.info.pull_groups)
     pull_entry_bytes = int(sim.info.pull_entry_bytes)
+    flush = CacheFlush(torch, args.flush_mib)
 
-    def volley(fraction):
+    def volley(fraction, cold=False):
         sim.set_gather_policy(fraction)
         sim.set_volley(net.n_neurons)
         sim.run(net.max_delay + 2)  # fill all delay slots
@@ -386,71 +506,97 @@ def main():
             log(f"[bench] autotuned for the volley (policy {fraction}): {sim.autotune(10)}")
         sim.set_mode(0)  # time the kernels one launch at a time
         cv0 = sim.counters()
-        gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
+        if cold:
+            gv_ms, uv_ms = time_cold(sim, flush, args.volley_steps)
+        else:
+            gv_ms, uv_ms = sim.time_kernels(args.volley_steps)
         cv1 = sim.counters()
         sim.set_volley(0)
         ev = (cv1["events"] - cv0["events"]) / args.volley_steps
         rw = (cv1["rows"] - cv0["rows"]) / args.volley_steps
+        per = gather_alg_bytes(ev, rw)
         g_s = gv_ms * 1e-3 / args.volley_steps
-        u_s = uv_ms * 1e-3 / args.volley_steps
-        return {"gather_us": g_s * 1e6, "update_us": u_s * 1e6, "events_per_launch": ev, "rows_per_launch": rw}
+        return {"gather_us": g_s * 1e6, "update_us": uv_ms * 1e3 / args.volley_steps, "events_per_launch": ev,
+                "rows_per_launch": rw, "alg_bytes_per_launch": per, "alg_gbs": per / g_s / 1e9,
+                "tiling": sim.tiling()}
     # push only: 1e8 synapses per step added with atomics (every update block
     # holds 512k synapse entries, so it posts them as heavy pushes and the
     # next gather launch adds them; the step is both kernels)
     v_push = volley(2.0)
     v_push["push_events_per_s"] = v_push["events_per_launch"] / ((v_push["update_us"] + v_push["gather_us"]) * 1e-6)
     log(f"[bench] volley push-only: {v_push}")
-    # default policy: every step is deferred and pulled
-    v_auto = volley(default_policy)
-    per = gather_alg_bytes(v_auto["events_per_launch"], v_auto["rows_per_launch"])
-    v_auto["alg_bytes_per_launch"] = per
-    v_auto["pull_groups"] = groups
-    v_auto["gbs"] = per / (v_auto["gather_us"] * 1e-6) / 1e9
-    log(f"[bench] volley pull (policy {default_policy}): {v_auto}")
+    # default policy: every step is deferred and pulled; warm (launches back
+    # to back: the 244 MB pull layout largely stays in the Infinity Cache)
+    # and cold (a cache-evicting sweep before every launch)
+    v_warm = volley(default_policy)
+    v_warm["pull_groups"] = groups
+    log(f"[bench] volley pull, warm (policy {default_policy}): {v_warm}")
+    v_cold = volley(default_policy, cold=True)
+    log(f"[bench] volley pull, cold: {v_cold}")
     sim.set_gather_policy(default_policy)
-    v_auto["tiling"] = sim.tiling()
-    per_launch = v_auto["alg_bytes_per_launch"]
-    g_avg_s = v_auto["gather_us"] * 1e-6
-    achieved = v_auto["gbs"]
+    flush.close()
+    sim.close()
 
-    del sim
     copy_gbs = copy_bandwidth_gbs(torch)
     log(f"[bench] device copy: {copy_gbs:.0f} GB/s (read + write)")
-    traffic = None
-    if os.path.exists(args.traffic_file):
-        with open(args.traffic_file) as f:
-            traffic = json.load(f)
+    t_warm = read_json(args.traffic_file)
+    t_cold = read_json(args.cold_traffic_file)
     batched = None if args.no_batched else run_batched(args, ctx, log)
     adjoint = None if args.no_adjoint else run_adjoint(args, log)
 
     cpu = None
     if not args.no_cpu:
-        # the FP32 CPU restatement of the same network: warm-up over the same
-        # steps as the GPU's, then the GPU's timed step range on every core
-        # this process may use, then a one-thread sample of the steps after it
+        # the FP32 CPU restatement of the same network: the same settle run
+        # (timed: the CPU's full-run rate), warm-up, and the GPU's timed
+        # window, on every core this process may use; then a one-thread
+        # sample of the steps after it
         from oracle.pyoracle import Oracle
         threads = len(os.sched_getaffinity(0))
         t0 = time.time()
         o = Oracle(net, 32)
         log(f"[bench] CPU oracle built in {time.time() - t0:.1f}s")
-        o.time_steps(args.warmup, threads=threads)
+        secs_f = o.time_steps(args.settle_steps, threads=threads)
+        ef = o.events
+        o.time_steps(args.warmup * args.chunk, threads=threads)
         e0 = o.events
-        secs_mt = o.time_steps(args.steps, threads=threads)
+        secs_w = o.time_steps(n_sim, threads=threads)
         e1 = o.events
         secs_1 = o.time_steps(args.cpu_steps)
         e2 = o.events
-        cpu = {"value": (e1 - e0) / secs_mt, "unit": "events/s", "cores": threads, "kind": "port",
-               "sample": f"same network and steps as the timed GPU run (steps {args.warmup}..{args.warmup + args.steps}, "
-                         f"{args.steps * dt_ms:.0f} ms simulated after {args.warmup * dt_ms:.0f} ms of warm-up), FP32 CPU "
-                         f"restatement on {threads} OpenMP threads: {secs_mt:.2f} s wall, RTF "
-                         f"{args.steps * dt_ms / (secs_mt * 1e3):.3f}",
+        cpu = {"value": (e1 - e0) / secs_w, "unit": "events/s", "cores": threads, "kind": "port",
+               "sample": f"the GPU's timed window ({window}, {n_sim * dt_ms:.0f} ms simulated) of the same network "
+                         f"after the same settle run and warm-up, FP32 CPU restatement on {threads} OpenMP threads: "
+                         f"{secs_w:.2f} s wall, RTF {n_sim * dt_ms / (secs_w * 1e3):.3f}",
+               "events_match_gpu": int(e1 - e0) == int(events),
+               "full_run_events_per_s": ef / secs_f,
+               "full_run_wall_s": secs_f,
+               "full_run_events_match_gpu": int(ef) == int(full["events"]),
                "value_1thread": (e2 - e1) / secs_1,
                "sample_1thread": f"the next {args.cpu_steps} steps on one thread: {secs_1:.2f} s wall, RTF "
                                  f"{args.cpu_steps * dt_ms / (secs_1 * 1e3):.3f}",
-               "events_match_gpu": int(e1 - e0) == int(events),
                "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count()}
         log(f"[bench] CPU: {cpu}")
+        del o
 
+    g_cold_s = v_cold["gather_us"] * 1e-6
+    g_warm_s = v_warm["gather_us"] * 1e-6
+    traffic = t_cold["hbm_bytes_per_launch"] if t_cold else None
+    achieved = traffic / g_cold_s / 1e9 if traffic else None
+    roofline = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+        "kernel": "gather_kernel pull, 2-byte count-format entries (volley: all 100k neurons forced each step), "
+                  "cold: a 2 GiB write sweep before each launch",
+        "avg_launch_us": g_cold_s * 1e6, "traffic_detail": t_cold,
+        "alg_bytes_per_launch": v_cold["alg_bytes_per_launch"], "alg_gbs": v_cold["alg_gbs"],
+        "alg_unit": "8 B per delivered synapse + 8 B per delivered (source, delay) row",
+        "pull_entry_bytes": pull_entry_bytes, "copy_gbs": copy_gbs,
+        "warm": {"avg_launch_us": g_warm_s * 1e6,
+                 "traffic": t_warm["hbm_bytes_per_launch"] if t_warm else None,
+                 "achieved": t_warm["hbm_bytes_per_launch"] / g_warm_s / 1e9 if t_warm else None,
+                 "frac": t_warm["hbm_bytes_per_launch"] / g_warm_s / 1e9 / HBM_PEAK_GBS if t_warm else None,
+                 "alg_gbs": v_warm["alg_gbs"]},
+    }
     out = {
         "metric": "synaptic events/s (plain IF_cond_exp 100k net); realtime factor; gather % HBM peak",
         "value": ev_per_s,
@@ -466,29 +612,25 @@ def main():
         "data": "synthetic (seeded random network and drive)",
         "config": {"workload": "if_cond_exp_100k" if not args.small else "if_cond_exp_10k_small",
                    "neurons": net.n_neurons, "synapses": net.n_syn, "dt_ms": dt_ms,
-                   "max_delay_steps": net.max_delay, "drive": "Binomial(100, 5 Hz * dt) per neuron-step"},
+                   "max_delay_steps": net.max_delay, "drive": "Binomial(100, 5 Hz * dt) per neuron-step",
+                   "settle_steps": args.settle_steps, "simulation_steps_per_bench_step": args.chunk},
         "realtime_factor": rtf,
         "firing_rate_hz": rate_hz,
         "events_per_step": events / args.steps,
+        "events_per_simulation_step": events / n_sim,
+        "us_per_simulation_step": wall * 1e6 / n_sim,
+        "full_run": full,
         "natural_kernels": natural,
         "natural_tiling": {**natural_tiling[0], **natural_tiling[1]},
         "volley_push_only": v_push,
-        "volley_pull": v_auto,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
-                     "traffic_detail": traffic,
-                     "kernel": "gather_kernel pull (volley: all 100k neurons forced each step)",
-                     "alg_bytes_per_launch": per_launch, "avg_launch_us": g_avg_s * 1e6,
-                     "pull_entry_bytes": int(pull_entry_bytes),
-                     "hbm_gbs": traffic["hbm_bytes_per_launch"] / g_avg_s / 1e9 if traffic else None,
-                     "hbm_frac": traffic["hbm_bytes_per_launch"] / g_avg_s / 1e9 / HBM_PEAK_GBS if traffic else None,
-                     "copy_gbs": copy_gbs},
+        "volley_pull_warm": v_warm,
+        "volley_pull_cold": v_cold,
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -22,7 +22,7 @@ struct Handle {
 
 template <typename E>
 int run_engine(E& e, uint64_t steps, const uint64_t* forced_t, const uint32_t* forced_id,
-               uint64_t n_forced, uint64_t* raster, double* v_trace, uint32_t n_neurons) {
+               uint64_t n_forced, uint64_t* raster, double* v_trace, uint32_t n_neurons, int threads = 1) {
   uint64_t fpos = 0;
   // skip schedule entries before the current cycle
   while (fpos < n_forced && forced_t[fpos] < e.cycle()) ++fpos;
@@ -30,8 +30,12 @@ int run_engine(E& e, uint64_t steps, const uint64_t* forced_t, const uint32_t* f
     const uint64_t t = e.cycle();
     const uint64_t f0 = fpos;
     while (fpos < n_forced && forced_t[fpos] == t) ++fpos;
-    e.step(forced_id + f0, fpos - f0, raster ? raster + s * e.words() : nullptr,
-           v_trace ? v_trace + s * (uint64_t)n_neurons : nullptr);
+    uint64_t* row = raster ? raster + s * e.words() : nullptr;
+    double* vrow = v_trace ? v_trace + s * (uint64_t)n_neurons : nullptr;
+    if (threads > 1)
+      e.step_parallel(threads, forced_id + f0, fpos - f0, row, vrow);
+    else
+      e.step(forced_id + f0, fpos - f0, row, vrow);
   }
   return 0;
 }
@@ -66,6 +70,23 @@ int oracle_run(void* p, uint64_t steps, const uint64_t* forced_t, const uint32_t
   try {
     if (h->f) return run_engine(*h->f, steps, forced_t, forced_id, n_forced, raster, v_trace, n_neurons);
     return run_engine(*h->d, steps, forced_t, forced_id, n_forced, raster, v_trace, n_neurons);
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return -1;
+  }
+}
+
+// oracle_run with the FP32 engine's parallel step on `threads` OpenMP
+// threads (the same results bit for bit); -1 for the FP64 engine.
+int oracle_run_parallel(void* p, uint64_t steps, const uint64_t* forced_t, const uint32_t* forced_id,
+                        uint64_t n_forced, uint64_t* raster, double* v_trace, uint32_t n_neurons, int threads) {
+  auto h = static_cast<Handle*>(p);
+  if (!h->f || threads < 1) {
+    g_err = "oracle_run_parallel: FP32 engine only";
+    return -1;
+  }
+  try {
+    return run_engine(*h->f, steps, forced_t, forced_id, n_forced, raster, v_trace, n_neurons, threads);
   } catch (const std::exception& ex) {
     g_err = ex.what();
     return -1;

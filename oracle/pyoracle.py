@@ -53,6 +53,9 @@ def load():
     lib.oracle_run.restype = ctypes.c_int
     lib.oracle_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, c_u64p, c_u32p, ctypes.c_uint64, c_u64p,
                                c_f64p, ctypes.c_uint32]
+    lib.oracle_run_parallel.restype = ctypes.c_int
+    lib.oracle_run_parallel.argtypes = [ctypes.c_void_p, ctypes.c_uint64, c_u64p, c_u32p, ctypes.c_uint64, c_u64p,
+                                        c_f64p, ctypes.c_uint32, ctypes.c_int]
     lib.oracle_time_steps.restype = ctypes.c_double
     lib.oracle_time_steps.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
     lib.oracle_time_steps_parallel.restype = ctypes.c_double
@@ -83,6 +86,11 @@ def _p(a, t):
 
 
 N_VARS = {0: 4, 1: 2, 2: 4}
+
+
+def host_threads():
+    """Cores this process may use (the GPU box gives a share of its host)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
 
 
 class Oracle:
@@ -133,20 +141,30 @@ class Oracle:
         self.words = int(lib.oracle_words(self.h))
         self.precision = precision
 
-    def run(self, steps, forced=None, record_v=False):
-        """forced: list of (step, id).  Returns (raster [steps, words] u64,
-        v [steps, N] float64 or None)."""
+    def run(self, steps, forced=None, record_v=False, threads=1):
+        """forced: list of (step, id), or a pair of arrays (steps, ids).
+        threads > 1: the FP32 engine's OpenMP step (identical results).
+        Returns (raster [steps, words] u64, v [steps, N] float64 or None)."""
         raster = np.zeros((steps, self.words), np.uint64)
         v = np.zeros((steps, self.n), np.float64) if record_v else None
-        if forced:
+        if isinstance(forced, tuple):
+            ft = np.asarray(forced[0], np.uint64)
+            fi = np.asarray(forced[1], np.uint32)
+            order = np.lexsort((fi, ft))
+            ft = np.ascontiguousarray(ft[order])
+            fi = np.ascontiguousarray(fi[order])
+        elif forced:
             f = sorted(forced)
             ft = np.array([a for a, _ in f], np.uint64)
             fi = np.array([b for _, b in f], np.uint32)
         else:
             ft = fi = None
-        rc = self.lib.oracle_run(self.h, steps, _p(ft, ctypes.c_uint64), _p(fi, ctypes.c_uint32),
-                                 0 if ft is None else ft.shape[0], _p(raster, ctypes.c_uint64),
-                                 _p(v, ctypes.c_double), self.n)
+        args = (self.h, steps, _p(ft, ctypes.c_uint64), _p(fi, ctypes.c_uint32), 0 if ft is None else ft.shape[0],
+                _p(raster, ctypes.c_uint64), _p(v, ctypes.c_double), self.n)
+        if threads > 1:
+            rc = self.lib.oracle_run_parallel(*args, int(threads))
+        else:
+            rc = self.lib.oracle_run(*args)
         if rc != 0:
             raise RuntimeError("oracle: " + self.lib.oracle_last_error().decode())
         return raster, v

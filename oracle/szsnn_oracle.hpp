@@ -318,14 +318,16 @@ class Engine {
     ++t_;
   }
 
-  // The same step on `threads` OpenMP threads, for timing the CPU baseline
-  // (no forced input, no outputs).  FP32 engine only: its accumulators are
-  // integers, so delivering the spikes with relaxed atomic adds in any order
-  // gives the serial step's sums exactly; neurons update independently and
-  // the spike list keeps ascending ids (each thread a contiguous range).
-  void step_parallel(int threads) {
+  // The same step on `threads` OpenMP threads: the CPU baseline's step, and
+  // the checker's step for large networks.  FP32 engine only: its
+  // accumulators are integers, so delivering the spikes with relaxed atomic
+  // adds in any order gives the serial step's sums exactly; neurons update
+  // independently and the spike list keeps ascending ids (each thread a
+  // contiguous range).  Arguments as step().
+  void step_parallel(int threads, const uint32_t* forced = nullptr, uint64_t n_forced = 0,
+                     uint64_t* raster_row = nullptr, double* v_row = nullptr) {
     if constexpr (!std::is_same<Acc, int64_t>::value) {
-      (void)threads;
+      (void)threads, (void)forced, (void)n_forced, (void)raster_row, (void)v_row;
       throw std::runtime_error("step_parallel: FP32 engine only");
     } else {
     const size_t slot = (size_t)(t_ % RS_);
@@ -333,6 +335,14 @@ class Engine {
     Acc* accI = &ring_[(slot * 2 + 1) * N_];
     events_ += ring_events_[slot];
     ring_events_[slot] = 0;
+    if (n_forced) {
+      forced_.assign(T_, 0);
+      for (uint64_t f = 0; f < n_forced; ++f) {
+        if (forced[f] >= T_) throw std::runtime_error("forced spike id not in the network");
+        forced_[forced[f]] = 1;
+      }
+    }
+    const bool any_forced = n_forced != 0;
     std::vector<std::vector<uint32_t>> tsp(threads);
     std::vector<uint64_t> tdrive(threads, 0);
 #pragma omp parallel num_threads(threads)
@@ -354,9 +364,10 @@ class Engine {
         }
         const R e = from_acc(ae), in = from_acc(accI[i]);
         const R iinj = d_.noise_amp ? (R)d_.noise_amp[i] * gauss<R>(keys_[i], t_) : R(0);
-        const bool sp = update_neuron<R>(d_.model, p.data(), x.data(), e, in, iinj, false);
+        const bool sp = update_neuron<R>(d_.model, p.data(), x.data(), e, in, iinj, any_forced && forced_[i] != 0);
         for (int k = 0; k < nv_; ++k) vars_[(size_t)k * N_ + i] = x[k];
         if (sp) tsp[th].push_back(i);
+        if (v_row) v_row[i] = (double)x[0];
         accE[i] = Acc(0);
         accI[i] = Acc(0);
       }
@@ -369,27 +380,38 @@ class Engine {
     }
     for (uint32_t j = 0; j < M_; ++j) {
       const uint32_t id = N_ + j;
-      bool sp = false;
+      bool sp = any_forced && forced_[id] != 0;
       if (d_.source_model == SRC_CONST_RATE) {
         R last = src_last_[j] + R(1);
-        sp = last == (R)d_.src_rate[j];
+        sp = sp || last == (R)d_.src_rate[j];
         if (sp) last = R(0);
         src_last_[j] = last;
       } else if (d_.source_model == SRC_POISSON) {
-        sp = draw(keys_[id], t_, 3) < d_.src_thresh[j];
+        sp = sp || draw(keys_[id], t_, 3) < d_.src_thresh[j];
       }
       if (sp) spikes_.push_back(id);
     }
     const int64_t ns = (int64_t)spikes_.size();
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 4)
-    for (int64_t si = 0; si < ns; ++si) {
-      const uint32_t id = spikes_[si];
-      for (uint64_t q = off_[id]; q < off_[id + 1]; ++q) {
-        const size_t sl = (size_t)((t_ + dl_[q] + 1) % RS_);
-        const int ch = w_[q] < 0.0f ? 1 : 0;
-        __atomic_fetch_add(&ring_[(sl * 2 + ch) * N_ + tgt_[q]], wq_[q], __ATOMIC_RELAXED);
-        __atomic_fetch_add(&ring_events_[sl], (uint64_t)1, __ATOMIC_RELAXED);
+    std::vector<uint64_t> tev((size_t)threads * RS_, 0);  // events per thread and delivery slot
+#pragma omp parallel num_threads(threads)
+    {
+      uint64_t* ev = &tev[(size_t)omp_get_thread_num() * RS_];
+#pragma omp for schedule(dynamic, 4)
+      for (int64_t si = 0; si < ns; ++si) {
+        const uint32_t id = spikes_[si];
+        for (uint64_t q = off_[id]; q < off_[id + 1]; ++q) {
+          const size_t sl = (size_t)((t_ + dl_[q] + 1) % RS_);
+          const int ch = w_[q] < 0.0f ? 1 : 0;
+          __atomic_fetch_add(&ring_[(sl * 2 + ch) * N_ + tgt_[q]], wq_[q], __ATOMIC_RELAXED);
+          ev[sl] += 1;
+        }
       }
+    }
+    for (int th = 0; th < threads; ++th)
+      for (uint32_t sl = 0; sl < RS_; ++sl) ring_events_[sl] += tev[(size_t)th * RS_ + sl];
+    if (raster_row) {
+      std::memset(raster_row, 0, words() * 8);
+      for (uint32_t id : spikes_) raster_row[id / 64] |= (uint64_t)1 << (id % 64);
     }
     ++t_;
     }

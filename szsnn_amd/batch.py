@@ -13,7 +13,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, ptr
 from .configs import DEFAULT_SEED, izhikevich2003
-from .simulation import Simulation, _desc, raster_to_bool
+from .simulation import Simulation, _desc, _live_nets, raster_to_bool
 
 
 def _bits(raster, first, count):
@@ -74,6 +74,7 @@ class BatchSimulation(Simulation):
         self._rec = None
         self._bitmap = np.zeros(self.words, np.uint64)
         self.network = None
+        _live_nets.add(self)
         self.offsets = []
         for b in range(len(self.networks)):
             no, n, so, m = (ctypes.c_uint32() for _ in range(4))

@@ -12,13 +12,39 @@ plus the batched-run calls of the HIP library (run, recording, counters,
 volley input, tiling).  Errors raise SzsnnError, the reference's
 SzsnnException contract (utils.h:144-155).
 """
+import atexit
 import ctypes
 import time
+import weakref
 
 import numpy as np
 
 from . import _lib
 from ._lib import SzsnnError, check, ptr
+
+
+# Live contexts and networks.  At interpreter exit they are closed in
+# dependency order (networks, then contexts), before the HIP runtime's own
+# exit handlers run: left to garbage collection at shutdown, a network could
+# free device memory and graphs after the runtime (or a profiler attached to
+# it) has already been torn down.
+_live_nets = weakref.WeakSet()
+_live_ctxs = weakref.WeakSet()
+
+
+@atexit.register
+def close_all():
+    """Close every live Simulation, then every live Context."""
+    for s in list(_live_nets):
+        try:
+            s.close()
+        except Exception:
+            pass
+    for c in list(_live_ctxs):
+        try:
+            c.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -30,6 +56,7 @@ class Context:
         check(self.lib.szsnn_create(int(device), ctypes.byref(h)))
         self.handle = h
         self.device = device
+        _live_ctxs.add(self)
 
     @property
     def name(self):
@@ -41,7 +68,11 @@ class Context:
         check(self.lib.szsnn_synchronize(self.handle), self.handle)
 
     def close(self):
+        """Destroy the stream; networks still open on this context are closed first."""
         if self.handle:
+            for s in list(_live_nets):
+                if s.ctx is self:
+                    s.close()
             self.lib.szsnn_destroy(self.handle)
             self.handle = None
 
@@ -113,6 +144,7 @@ class Simulation:
         self._wall_ms = 0.0
         self._rec = None
         self._bitmap = np.zeros(self.words, np.uint64)
+        _live_nets.add(self)
 
     # -- reference API -------------------------------------------------
     def step(self, fInput=None):

@@ -294,3 +294,39 @@ def test_parallel_step_equals_serial(oracle_mod, which):
     np.testing.assert_array_equal(serial.vars(), parallel.vars())
     assert serial.events == parallel.events > 0
     assert serial.drive_events == parallel.drive_events
+
+
+@pytest.mark.parametrize("which", ["cond_forced", "const_rate_forced"])
+def test_parallel_run_with_forced_ids_equals_serial(oracle_mod, which):
+    """oracle_run_parallel (the checker's step for full-size networks) with
+    forced ids, rasters and voltage rows equals the serial run bit for bit,
+    including forced sources and ids forced during refractory periods."""
+    from szsnn_amd import configs
+    from szsnn_amd.network import Network, if_curr_exp_params, if_vars, quantize_weights
+    rng = np.random.default_rng(17)
+    if which == "cond_forced":
+        net = configs.if_cond_exp_100k(n_exc=4000, n_inh=1000, k_exc=200, k_inh=50)
+        steps = 80
+    else:
+        n, m, D, S = 300, 20, 9, 6000
+        net = Network(model=0, n_neurons=n, params=if_curr_exp_params(n, tau_refrac=3.0, v_thresh=-55.0),
+                      vars=if_vars(n, rng.uniform(-65, -55, n).astype(np.float32)),
+                      syn_src=rng.integers(0, n + m, S).astype(np.uint32),
+                      syn_tgt=rng.integers(0, n, S).astype(np.uint32),
+                      syn_w=quantize_weights(rng.uniform(-0.6, 1.0, S), 26),
+                      syn_delay=rng.integers(0, D, S).astype(np.uint16), max_delay=D, frac_bits=26, seed=9,
+                      n_sources=m, source_model=1, src_rate=rng.integers(2, 9, m).astype(np.float32),
+                      src_last=np.zeros(m, np.float32))
+        steps = 200
+    T = net.n_neurons + net.n_sources
+    ft = rng.integers(0, steps, 4 * T).astype(np.uint64)
+    fi = rng.integers(0, T, 4 * T).astype(np.uint32)
+    serial = oracle_mod.Oracle(net, 32)
+    parallel = oracle_mod.Oracle(net, 32)
+    rs, vs = serial.run(steps, forced=(ft, fi), record_v=True)
+    rp, vp = parallel.run(steps, forced=(ft, fi), record_v=True, threads=4)
+    np.testing.assert_array_equal(rs, rp)
+    np.testing.assert_array_equal(vs, vp)
+    np.testing.assert_array_equal(serial.vars(), parallel.vars())
+    assert serial.events == parallel.events > 0
+    assert serial.drive_events == parallel.drive_events
