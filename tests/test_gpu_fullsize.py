@@ -109,7 +109,7 @@ def test_if_curr_exp_10k_10000_steps_fp32_bit_exact(ctx, drive):
     b = raster_to_bool(gpu[0], net.n_neurons + net.n_sources)
     assert b[:, net.n_neurons:].sum() > 5000    # the Poisson sources
     if drive == "driven":
-        assert b[:, :net.n_neurons].sum() > 1e6
+        assert b[:, :net.n_neurons].sum() > 4e5   # about 57 Hz x 10000 neurons x 1 s
 
 
 def test_batched_izhikevich_256_instances(ctx):
@@ -173,7 +173,7 @@ def test_full_size_volley_pull_equals_push_and_oracle(ctx, net100k):
     _assert_same(pull, _oracle(net100k, steps, forced=(ft, fi)), "volley pull")
 
 
-@pytest.mark.parametrize("block,grid", [(256, 2), (512, 3)])
+@pytest.mark.parametrize("block,grid", [(256, 2), (512, 1)])
 def test_persistent_non_covering_grid_equals_graph(ctx, block, grid):
     """Persistent launches whose grid x block is smaller than the id count:
     each block walks its id blocks every step (state loaded and stored per
