@@ -14,7 +14,7 @@ Prints one JSON line (the driver's contract) with:
   value          synaptic events/s over the timed steps (natural dynamics)
   roofline       the pull gather kernel at the volley load (all neurons forced
                  every step: every synapse delivered each step), each launch
-                 preceded by a 2 GiB write sweep that evicts L2 and the
+                 preceded by a 2 GiB read sweep that evicts L2 and the
                  Infinity Cache (cold), HIP-event timed on the launch stream;
                  `traffic` = HBM bytes per launch from the rocprofv3
                  FETCH_SIZE / WRITE_SIZE passes of the same cold workload
@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--settle-steps", type=int, default=10000,
                     help="untimed-for-value steps run first (the full 1000 ms network run, reported as full_run)")
     ap.add_argument("--flush-mib", type=int, default=2048,
-                    help="cold-cache gather timing: MiB written between launches to evict L2 and the Infinity Cache")
+                    help="cold-cache gather timing: MiB read between launches to evict L2 and the Infinity Cache")
     ap.add_argument("--volley-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=1000, help="one-thread CPU baseline sample (steps of the same net)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -64,7 +64,11 @@ def parse():
                     help="profiler pass: a 1 GiB device copy (calibration), the volley's delay slots filled by "
                          "one persistent launch, then exactly --volley-steps gather_kernel launches")
     ap.add_argument("--cold", action="store_true",
-                    help="profile pass: a --flush-mib write sweep before every gather launch")
+                    help="profile pass: a --flush-mib read sweep before every gather launch")
+    ap.add_argument("--fill", choices=("persistent", "graph"), default="persistent",
+                    help="profile pass: fill the delay slots with one persistent (cooperative) launch, whose "
+                         "gather work is inside it, or with single graph-mode steps (more gather_kernel "
+                         "dispatches; no cooperative launch, after which rocprofv3 exits with SIGSEGV)")
     ap.add_argument("--natural-pass", action="store_true",
                     help="profiler pass: natural dynamics, --warmup steps then --steps steps as ONE persistent "
                          "launch (block 512, grid 256), then --volley-steps single-step gather+update launches")
@@ -122,16 +126,18 @@ def log(*a):
 
 class CacheFlush:
     """A device buffer several times the Infinity Cache (256 MB) and the L2s;
-    writing it evicts whatever a following kernel would otherwise find there."""
+    reading it evicts whatever a following kernel would otherwise find there.
+    The sweep reads rather than writes: a write sweep leaves the Infinity
+    Cache full of dirty lines, and the next kernel then pays for writing them
+    back (a 253 MB read measured 104 us after a 2 GiB write sweep against 75
+    us after a read sweep and 49 us warm)."""
 
     def __init__(self, torch, mib):
         self.torch = torch
-        self.buf = torch.empty(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
-        self.k = 0.0
+        self.buf = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
 
     def __call__(self):
-        self.k += 1.0
-        self.buf.fill_(self.k)
+        self.buf.sum()
         self.torch.cuda.synchronize()
 
     def close(self):
@@ -329,6 +335,8 @@ def batched_pass(args):
     ctx = Context(0)
     nets = batched_izhikevich(n_instances=args.batch_instances)
     bs = BatchSimulation(ctx, nets)
+    if not args.no_autotune:
+        log(f"[bench] batched tiling: {bs.autotune(100)}")
     bs.run(args.warmup * args.chunk)
     ctx.synchronize()
     c0 = bs.counters()
@@ -339,7 +347,7 @@ def batched_pass(args):
     ev = bs.counters()["events"] - c0["events"]
     print(json.dumps({"batched_pass": True, "instances": len(nets), "neurons": sum(n.n_neurons for n in nets),
                       "steps": args.batch_steps, "us_per_step": wall * 1e6 / args.batch_steps,
-                      "events_per_step": ev / args.batch_steps, "tiling": bs.tiling()}), flush=True)
+                      "events_per_step": ev / args.batch_steps, "tiling": bs.tiling_ex()}), flush=True)
 
 
 def profile_pass(args, torch, ctx, net, sim):
@@ -356,9 +364,10 @@ def profile_pass(args, torch, ctx, net, sim):
     del a, b
     flush = CacheFlush(torch, args.flush_mib) if args.cold else None
     sim.set_tiling(gather_block=args.gather_block, gather_grid=args.gather_grid)
-    sim.set_mode(1)
+    sim.set_mode(1 if args.fill == "persistent" else 0)
     sim.set_volley(net.n_neurons)
     sim.run(net.max_delay + 2)
+    sim.set_mode(0)
     ctx.synchronize()
     c0 = sim.counters()
     if flush:
@@ -586,7 +595,7 @@ def headline(args, torch, ctx, net, sim):
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
         "kernel": "gather_kernel pull, 2-byte count-format entries (volley: all 100k neurons forced each step), "
-                  "cold: a 2 GiB write sweep before each launch",
+                  "cold: a 2 GiB read sweep before each launch",
         "avg_launch_us": g_cold_s * 1e6, "traffic_detail": t_cold,
         "alg_bytes_per_launch": v_cold["alg_bytes_per_launch"], "alg_gbs": v_cold["alg_gbs"],
         "alg_unit": "8 B per delivered synapse + 8 B per delivered (source, delay) row",

@@ -178,6 +178,34 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
 int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_block, uint32_t gather_grid);
 int szsnn_get_tiling(szsnn_net* net, uint32_t* update_block, uint32_t* gather_block, uint32_t* gather_grid);
 
+/* The whole launch configuration at once.  The reference's only tiling knobs
+ * are compile-time constants (work-items per group, NEURONS_PER_WI neurons
+ * per work-item, szsnn1.1/src/locales.h:17-46); here they are per network
+ * and chosen by szsnn_autotune:
+ *   update_block, update_ids_per_lane: the update kernel's block and ids per
+ *     lane (1, 2 or 4: a block serves that many consecutive blocks of ids);
+ *   gather_block, gather_grid: the pull gather's block and grid (grid 0: a
+ *     default from the work items and CUs);
+ *   gather_stage: delays whose spike bitmaps one pull pass stages in LDS at
+ *     once (1 .. gather_stage_max; 0 on set: the maximum);
+ *   gather_batch: 16-byte entry vectors per lane in each batch of the pull's
+ *     two-batch load pipeline (2 or 4: deeper prefetch, fewer waves);
+ *   mode, persist_block, persist_grid: as szsnn_set_mode. */
+typedef struct szsnn_tiling {
+  uint32_t update_block;
+  uint32_t update_ids_per_lane;
+  uint32_t gather_block;
+  uint32_t gather_grid;
+  uint32_t gather_stage;
+  uint32_t gather_stage_max;  /* get only: the most delays one pass can stage */
+  uint32_t gather_batch;      /* 16-byte entry vectors per lane loaded ahead together: 2 or 4 */
+  int32_t mode;
+  uint32_t persist_block;
+  uint32_t persist_grid;
+} szsnn_tiling;
+int szsnn_set_tiling_ex(szsnn_net* net, const szsnn_tiling* tiling);
+int szsnn_get_tiling_ex(szsnn_net* net, szsnn_tiling* tiling);
+
 /* Gather policy: the spikes of a step are either pushed by the update that
  * produced them, each synapse into the accumulator slot of its delivery step
  * (cost ~ events, global atomics), or, with the pull layout, deferred: each
@@ -208,8 +236,10 @@ int szsnn_get_heavy_push(szsnn_net* net, uint32_t* min_entries);
 /* Execution mode of szsnn_run.  GRAPH: each step is a gather launch (a no-op
  * unless something is due for the pull) and an update launch, replayed from a
  * captured HIP graph.  PERSISTENT: one cooperative launch runs all steps, a
- * grid barrier after each update and one after each pull; block
- * 256/512/1024, grid 0 = the co-resident maximum (clamped to it). */
+ * grid barrier after each update and one after each pull; block 256 or 512
+ * (the step loop keeps an id's state in registers: 220-241 VGPRs, 2 waves
+ * per SIMD; a 1024-thread block would have to spill), grid 0 = the
+ * co-resident maximum (clamped to it). */
 #define SZSNN_MODE_GRAPH 0
 #define SZSNN_MODE_PERSISTENT 1
 int szsnn_set_mode(szsnn_net* net, int32_t mode, uint32_t block, uint32_t grid);

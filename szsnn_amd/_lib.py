@@ -89,6 +89,21 @@ class Counters(ctypes.Structure):
     ]
 
 
+class Tiling(ctypes.Structure):
+    _fields_ = [
+        ("update_block", ctypes.c_uint32),
+        ("update_ids_per_lane", ctypes.c_uint32),
+        ("gather_block", ctypes.c_uint32),
+        ("gather_grid", ctypes.c_uint32),
+        ("gather_stage", ctypes.c_uint32),
+        ("gather_stage_max", ctypes.c_uint32),
+        ("gather_batch", ctypes.c_uint32),
+        ("mode", ctypes.c_int32),
+        ("persist_block", ctypes.c_uint32),
+        ("persist_grid", ctypes.c_uint32),
+    ]
+
+
 # (name, restype, argtypes) of every entry point declared in include/szsnn_hip.h
 _vp = ctypes.c_void_p
 PROTOTYPES = [
@@ -122,6 +137,8 @@ PROTOTYPES = [
     ("szsnn_set_tiling", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("szsnn_get_tiling", ctypes.c_int, [_vp, c_u32p, c_u32p, c_u32p]),
     ("szsnn_autotune", ctypes.c_int, [_vp, ctypes.c_uint32]),
+    ("szsnn_set_tiling_ex", ctypes.c_int, [_vp, ctypes.POINTER(Tiling)]),
+    ("szsnn_get_tiling_ex", ctypes.c_int, [_vp, ctypes.POINTER(Tiling)]),
     ("szsnn_set_mode", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]),
     ("szsnn_set_gather_policy", ctypes.c_int, [_vp, ctypes.c_float]),
     ("szsnn_get_gather_policy", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]),

@@ -21,29 +21,40 @@ constexpr uint32_t kMaxPullGroups = 8;           // delay groups of the pull (wo
 //   wide:  8 bytes (rel, fixed-point weight);
 //   table: 4 bytes rel | i << kPullRelBits, weight i (1-based) of a table of
 //          at most kPullMaxWeights;
-//   count: 2 bytes delta | tag << kPullDeltaBits, for networks with at most
-//          kPullCountFields distinct weights.  A lane's entries come in
-//          ascending source order and rel is the running sum of the deltas
-//          (from 0 at the start of each column).  Tag i < 3 is weight i + 1:
-//          the lane counts selected entries in kPullCountBits-bit fields of
-//          one register, tag i's field at shift kPullCountBits * i.  Tag 3
-//          is a skip (a gap wider than a delta, or padding with delta 0): its
-//          shift lands above the fields.
+//   count: 2 bytes tag | word delta << 2 | bit << 11, for networks with at
+//          most kPullCountFields distinct weights.  A lane's entries come in
+//          ascending source order; the 32-bit word of an entry's source is
+//          the running sum of the word deltas (from word 0 at the start of
+//          each column) and `bit` its bit in that word (rel % 32).  Tag
+//          i < 3 is weight i + 1: the lane counts selected entries in
+//          kPullCountBits-bit fields of one register, tag i's field at
+//          shift kPullCountBits * i.  Tag 3 is a skip (a word gap wider than
+//          kPullMaxDelta, or padding with delta 0): its field is above the
+//          counted ones.
 // Wide and table padding entries point at a zero word staged after the
 // bitmap (rel = 64 * staged words), weight 0 or index 0.
 enum : uint32_t { kPullWide = 0, kPullCount = 1, kPullTable = 2 };
 constexpr uint32_t kPullRelBits = 21;
 constexpr uint32_t kPullRelMask = (1u << kPullRelBits) - 1;
 constexpr uint32_t kPullCountFields = 3;
-constexpr uint32_t kPullCountBits = 10;
-constexpr uint32_t kPullDeltaBits = 14;
+constexpr uint32_t kPullCountBits = 8;
+constexpr uint32_t kPullDeltaBits = 9;                       // word delta of a count entry
 constexpr uint32_t kPullMaxDelta = (1u << kPullDeltaBits) - 1;
 constexpr uint16_t kPullSkipTag = 3;
+// A count-format entry and its fields.
+constexpr uint16_t count_entry(uint32_t tag, uint32_t bit, uint32_t word_delta) {
+  return (uint16_t)(tag | word_delta << 2 | bit << 11);
+}
+constexpr uint32_t count_tag(uint16_t x) { return x & 3u; }
+constexpr uint32_t count_delta(uint16_t x) { return (x >> 2) & kPullMaxDelta; }
+constexpr uint32_t count_bit(uint16_t x) { return (uint32_t)x >> 11; }
 static_assert((uint64_t)(kPullMaxWords + 1) * 64 <= kPullRelMask, "rel of the zero word must fit");
 static_assert(kPullMaxWeights < (1u << (32 - kPullRelBits)), "weight index must fit");
 static_assert(kPullCountBits * kPullCountFields <= 32, "count fields must fit one register");
 static_assert(kPullCountBits * kPullSkipTag < 32 && kPullCountBits * kPullSkipTag >= kPullCountBits * kPullCountFields,
               "the skip tag's shift lands above the count fields");
+static_assert(kPullCountBits == 8, "the kernel forms a tag's field shift as tag * 8");
+static_assert(2 + 5 + kPullDeltaBits == 16, "count entries are 16 bits");
 
 // Push entries carry the API delay in the target word's top bits.
 constexpr uint32_t kSynDelayShift = 26;

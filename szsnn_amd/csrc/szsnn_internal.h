@@ -110,7 +110,10 @@ struct DevNet {
   uint32_t pull_wn;         // count, table: weight-table entries
   const uint32_t* pull_lo;  // [NB] first raster word of the block's sources
   uint32_t pull_nb;         // NB
+  uint32_t pull_null;       // the null vector (16 bytes of padding entries after the last slice), in vectors
   uint32_t pull_words;      // LDS words staged per block (max over blocks)
+  uint32_t pull_stage;      // gather launches: delays whose bitmaps are staged at once (>= 1)
+  uint32_t pull_batch;      // gather launches: 16-byte vectors per lane and batch (2 or 4)
 };
 
 // 64-bit words of a pull block's LDS bitmap: the staged words, the zero word
@@ -214,20 +217,24 @@ hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t g
 // a persistent launch and at threshold changes; the gather launch of a step
 // pushes its previous step's list).
 hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s);
-// update: block 64, 128, 256, 512 or 1024.
-hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s);
-// One cooperative launch running nsteps steps (block 256/512/1024; grid must
+// update: block 64, 128, 256, 512 or 1024; npl (ids per lane) 1, 2 or 4.
+hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, uint32_t npl, hipStream_t s);
+// One cooperative launch running nsteps steps (block 256 or 512; grid must
 // not exceed persistent_max_blocks_per_cu(model, block) * CUs).
 hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint32_t block, uint32_t grid,
                              hipStream_t s);
 int persistent_max_blocks_per_cu(const DevNet& n, uint32_t block);
-// dynamic LDS bytes of a gather launch (pull path)
-size_t pull_dyn_bytes(const DevNet& n);
+// dynamic LDS bytes of a pull with `stage` delays staged at once (the
+// persistent loop stages one)
+size_t pull_dyn_bytes(const DevNet& n, uint32_t stage);
 // The same for a layout of `words` staged bitmap words, entry format `fmt`
 // and a weight table of `wn` entries.
-size_t pull_dyn_bytes_for(uint32_t words, uint32_t fmt, uint32_t wn);
-// Resident gather_kernel<block> blocks per CU with `dyn` bytes of LDS (0 on error).
-int gather_max_blocks_per_cu(size_t dyn, uint32_t block);
+size_t pull_dyn_bytes_for(uint32_t words, uint32_t fmt, uint32_t wn, uint32_t stage);
+// Resident gather blocks of `block` threads per CU with `dyn` bytes of LDS,
+// the given staging depth and batch (vectors per lane: 2 or 4; 0 on error).
+int gather_max_blocks_per_cu(size_t dyn, uint32_t block, uint32_t stage, uint32_t batch = 2);
+// LDS the delays staged at once by one gather block may take (their bitmaps).
+constexpr size_t kPullStageBytes = 51200;
 hipError_t launch_advance(uint64_t* t_base, uint64_t k, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t value, uint64_t n, hipStream_t s);
 

@@ -409,10 +409,13 @@ __device__ inline void store_lane(const DevNet& n, uint32_t id, const LaneState<
 // with one atomic per (block, delay).  CARRY: `ls` already holds the lane's
 // state and keeps it (the caller stores it at the end); otherwise it is
 // loaded and stored here, and the drive table is staged in LDS here.
+// `first`: the block's first id block of the step (a block that serves
+// several id blocks folds the delivery-time counters once).
 template <int MODEL, int BLK, bool CARRY, bool AGENT>
 __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm,
                                                  LaneState<MODEL>& ls, const StepFacts& facts,
-                                                 unsigned long long& out_drive) {
+                                                 unsigned long long& out_drive, bool first = true) {
+  const bool folder = first && blockIdx.x == 0;
   const uint32_t lane = __lane_id();
   const uint32_t slot = (uint32_t)(t % n.R);
   // every load the lane needs, issued before the first barrier
@@ -436,7 +439,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       sm.spikes = sm.drive = 0;
       sm.nlist = 0;
     }
-    if (!CARRY) step_facts<BLK, AGENT>(n, t, slot, blockIdx.x == 0, sm, fold_e, fold_r);
+    if (!CARRY) step_facts<BLK, AGENT>(n, t, slot, folder, sm, fold_e, fold_r);
   }
   __syncthreads();
   const uint64_t due = CARRY ? facts.due : sm.due;
@@ -653,7 +656,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       if (drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), drive);
     }
   }
-  if (!CARRY && blockIdx.x == 0 && threadIdx.x < 64) fold_finish(n, t, slot, fold_e, fold_r);
+  if (!CARRY && folder && threadIdx.x < 64) fold_finish(n, t, slot, fold_e, fold_r);
   // the new state, stored last (nothing in the step waits for these stores)
   if (!CARRY && id < n.T) store_lane<MODEL>(n, id, ls);
   out_drive = drive;
@@ -661,18 +664,34 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
 }
 
 template <int MODEL, int BLK, bool AGENT>
-__device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm) {
+__device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm, bool first) {
   LaneState<MODEL> ls;
   unsigned long long drive = 0;
-  update_step<MODEL, BLK, false, AGENT>(n, t, id, sm, ls, StepFacts{0, 0, false}, drive);
+  update_step<MODEL, BLK, false, AGENT>(n, t, id, sm, ls, StepFacts{0, 0, false}, drive, first);
 }
 
-template <int MODEL, int BLK>
+// update(t): block b serves the NPL id blocks b * NPL .. b * NPL + NPL - 1 of
+// BLK ids each, one after the other (NPL ids per lane: the reference's
+// NEURONS_PER_WI tiling knob, locales.h:38-40).
+template <int MODEL, int BLK, int NPL>
 __global__ void __launch_bounds__(BLK) update_kernel(DevNet n, uint32_t j) {
   __shared__ UpdateSmem<BLK> sm;
   const uint64_t t = *n.t_base + j;
-  update_block<MODEL, BLK, false>(n, t, blockIdx.x * BLK + threadIdx.x, sm);
+#pragma unroll 1
+  for (uint32_t p = 0; p < (uint32_t)NPL; ++p) {
+    const uint32_t base = (blockIdx.x * NPL + p) * BLK;
+    if (p > 0 && base >= n.T) break;  // block-uniform
+    if (p > 0) __syncthreads();       // LDS reuse
+    update_block<MODEL, BLK, false>(n, t, base + threadIdx.x, sm, p == 0);
+  }
 }
+
+// The LDS byte address of a pointer into LDS, and a 32-bit LDS load from one.
+typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+__device__ inline uint32_t lds_address(const uint32_t* p) {
+  return (uint32_t)(uintptr_t)(lds_u32*)p;
+}
+__device__ inline uint32_t lds_load_u32(uint32_t addr) { return *(lds_u32*)(uintptr_t)addr; }
 
 // Component c (compile-time after unrolling) of a 16-byte load.
 __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
@@ -680,37 +699,48 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
 }
 
 // Dense gather (pull) for target block `b` and delay group `g` over the
-// sliced layout: for each due delay k of the group the spike bitmap of step
-// t-k-1 (the block's source range) is staged in LDS; each wave takes slices
-// of 64 targets, one target per lane, and walks its lane's entries with
-// 16-byte loads (the wave's j-th load is contiguous; the next batch is in
-// flight while the current one is summed).  Per entry one 32-bit LDS read
-// and one bit-field extract give the source's spike bit b.  Sums are exact:
-// FP64 sums of integers below 2^53 (wide: E += b * max(w, 0), I += b *
-// min(w, 0); table: the (E, I) pair at 2 * index + b, zero for b = 0), or,
-// in the count format (2-byte entries: the source as a running sum of
-// deltas), per-weight counts of selected entries in bit fields of one
-// register, multiplied out once per column.  The lane adds its sums
-// into the block's LDS sums of its target (one lane per target and delay,
-// delays one after the other: no atomics); the block stores its targets'
-// sums for the group into pull slot g: plain stores, no global atomics.
-// dyn: bitmap (pull_bitmap_u64 words; the two 32-bit words after the staged
-// ones stay zero for the padding entries) | 2*kPullTB f64 sums | narrow
-// formats: 2 * pull_wn (E, I) f64 pairs.
-template <int BLK, int FMT, bool AGENT>
-__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
+// sliced layout.  The group's due delays are taken in passes of up to
+// `stage` delays: a pass stages the spike bitmaps of steps t-k-1 (the
+// block's source range) of all its due delays k in LDS at once, behind one
+// block barrier, and the waves then walk their columns of every delay of the
+// pass with no barrier in between.  A column is one slice of 64 targets of
+// one delay, one target per lane; the item's columns are numbered q = (k -
+// k0) * kPullSlices + slice and wave w walks q = w, w + waves, ... as one
+// stream of 16-byte loads in batches of KB vectors per lane (the wave's j-th
+// load is contiguous), held in two buffers that take turns, so one batch is
+// in flight while the other is summed; the batch after a column's last is
+// the next column's first, loaded before the next pass is staged.  Per
+// entry one 32-bit LDS read and one bit-field extract give the source's
+// spike bit.  Sums are exact: FP64 sums of integers below 2^53 (wide: E +=
+// b * max(w, 0), I += b * min(w, 0); table: the (E, I) pair at 2 * index +
+// b, zero for b = 0), or, in the count format (2-byte entries: the source as
+// a running sum of deltas), per-weight counts of selected entries in bit
+// fields of one register, multiplied out once per column.  The lane adds its
+// column's sums (integers) into the block's LDS sums of its target with an
+// LDS atomic: a target sits in different slices for different delays, so
+// the columns of one pass that serve it may belong to different waves; the
+// block stores its targets' sums for the group into pull slot g: plain
+// stores, no global atomics.
+// dyn: `stage` bitmaps of pull_bitmap_u64(pull_words) u64 each (the u64 at
+// index pull_words of each is zero, for the padding entries) | 2*kPullTB
+// int64 sums | narrow formats: 2 * pull_wn (E, I) f64 pairs.
+template <int BLK, int FMT, bool AGENT, uint32_t KB, bool MULTI>
+__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint32_t stage,
+                                 uint64_t* dyn) {
+  if (!MULTI) stage = 1;
   constexpr bool kNarrow = FMT != kPullWide;
   constexpr uint32_t kV = FMT == kPullWide ? 2 : FMT == kPullTable ? 4 : 8;  // entries per 16-byte vector
-  constexpr uint32_t kB = 2;                // vectors per lane and batch
+  constexpr uint32_t kB = KB;               // vectors per lane and batch
   constexpr uint32_t kWaves = BLK / 64;
   // count format: flush the fields before any can pass 2^kPullCountBits - 1
   constexpr uint32_t kFlushVectors = ((1u << kPullCountBits) - 1) / kV / kB * kB;
   static_assert(kFlushVectors >= kB, "count fields too narrow for one batch");
-  const uint32_t* bits = reinterpret_cast<const uint32_t*>(dyn);
-  double* sums = reinterpret_cast<double*>(dyn + pull_bitmap_u64(n.pull_words));  // [2][kPullTB]: E, I
-  double2* wt = reinterpret_cast<double2*>(sums + 2 * kPullTB);                   // [2 * pull_wn]
-  for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) sums[i] = 0.0;
-  if (threadIdx.x == 0) dyn[n.pull_words] = 0ull;  // the padding entries' zero word
+  const uint32_t bstride = pull_bitmap_u64(n.pull_words);  // u64 per staged bitmap
+  const uint32_t* bits0 = reinterpret_cast<const uint32_t*>(dyn);
+  unsigned long long* sums = reinterpret_cast<unsigned long long*>(dyn + (size_t)stage * bstride);  // [2][kPullTB]
+  double2* wt = reinterpret_cast<double2*>(sums + 2 * kPullTB);      // [2 * pull_wn]
+  for (uint32_t i = threadIdx.x; i < 2 * kPullTB; i += BLK) sums[i] = 0ull;
+  if (threadIdx.x < stage) dyn[(size_t)threadIdx.x * bstride + n.pull_words] = 0ull;  // the padding's zero word
   if (kNarrow)
     for (uint32_t i = threadIdx.x; i < 2 * n.pull_wn; i += BLK) {
       const double w = (i & 1u) ? (double)n.pull_wtab[i >> 1] : 0.0;
@@ -720,100 +750,107 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   const uint32_t nw = min(n.pull_words, n.W - lo);
   const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
   const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
+  const uint32_t qend = (k1 - k0) * kPullSlices;
   const uint4* vec = FMT == kPullWide    ? reinterpret_cast<const uint4*>(n.pull_syn)
                      : FMT == kPullTable ? reinterpret_cast<const uint4*>(n.pull_syn32)
                                          : reinterpret_cast<const uint4*>(n.pull_syn16);
   const uint32_t lane = __lane_id();
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
   uint32_t events = 0;
-  // The first due delay at or after k with synapses into the block (k1: none).
-  auto due_from = [&](uint32_t k) {
-    for (; k < k1; ++k) {
+  // col_from moves q to the wave's next nonempty column of a due delay at or
+  // after it (q >= qend: none) and gives its first entry s0 and its vectors
+  // per lane nv (the slice's longest lane).
+  auto col_from = [&](uint32_t& q, uint32_t& s0, uint32_t& nv) {
+    for (; q < qend; q += kWaves) {
+      const uint32_t k = k0 + q / kPullSlices;
       if (!((due >> k) & 1ull)) continue;
-      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices;
-      if (ld_const(&so[0]) != ld_const(&so[kPullSlices])) break;
-    }
-    return k;
-  };
-  // The wave walks its columns (slices wave, wave + kWaves, ... of each due
-  // delay, one lane per target) as one stream.  col_from moves (k, sl) to
-  // the next nonempty column at or after it (k = k1: none) and gives its
-  // first entry s0 and its vectors per lane nv (the slice's longest lane).
-  auto col_from = [&](uint32_t& k, uint32_t& sl, uint32_t& s0, uint32_t& nv) {
-    while (k < k1) {
-      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices;
-      for (; sl < kPullSlices; sl += kWaves) {
-        s0 = ld_const(&so[sl]);
-        const uint32_t s1 = ld_const(&so[sl + 1]);
-        if (s1 != s0) {
-          nv = (s1 - s0) / (64 * kV);
-          return;
-        }
+      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices + q % kPullSlices;
+      s0 = ld_const(&so[0]);
+      const uint32_t s1 = ld_const(&so[1]);
+      if (s1 != s0) {
+        nv = (s1 - s0) / (64 * kV);
+        return;
       }
-      k = due_from(k + 1);
-      sl = wave;
     }
     nv = 0;
   };
-  // The lane's pull_lane word of column (k, sl), loaded unconditionally:
-  // past the end of the stream it reads a valid word and ignores it.
-  auto lane_word = [&](uint32_t k, uint32_t sl) {
-    return n.pull_lane[(size_t)(b * n.D + min(k, k1 - 1)) * kPullTB + min(sl, kPullSlices - 1) * 64 + lane];
+  // The lane's pull_lane word of column q, loaded unconditionally: past the
+  // end of the stream it reads a valid word and ignores it.
+  auto lane_word = [&](uint32_t q) {
+    const uint32_t qq = min(q, qend - 1);
+    return n.pull_lane[(size_t)(b * n.D + k0 + qq / kPullSlices) * kPullTB + (qq % kPullSlices) * 64 + lane];
   };
-  // Entry loads run two batches ahead of the sums in two buffers, A and B,
-  // that take turns (no register copies, so the compiler waits for the batch
-  // being summed alone).  Every load is unconditional: a lane's loads are
-  // clamped to its last own vector (its padding vectors are never read: the
-  // repeats hit the same line, and the sums skip them), a column is walked
-  // in an even number of batches, and the batch after a column's last is
-  // the next column's first (clamped to that slice), loaded into A before
-  // the bitmap of its delay is staged.  Past the end of the stream the loads
-  // re-read vector `lane`, which exists (a nonempty slice has 64 lanes).
-  uint32_t k = due_from(k0);
-  uint32_t ck = k, csl = wave, cs0 = 0, cnv = 0;
-  col_from(ck, csl, cs0, cnv);
-  uint32_t cln = lane_word(ck, csl);
+  // Entry loads run a batch ahead of the sums in two buffers, A and B, that
+  // take turns (no register copies, so the compiler waits for the batch
+  // being summed alone).  Every load is unconditional and every loaded
+  // vector is summed, with no branch per vector: a lane's loads past its own
+  // vectors read the null vector (padding entries, which add nothing) after
+  // the layout's last slice, whose line every lane shares, so the padding
+  // of a column is never fetched.  A column is walked in an even number of
+  // batches, and the batch after a column's last is the next column's first
+  // (its slice's vectors, before its lane words are known; the slice's
+  // padding vectors are stored, and beyond the slice the null vector),
+  // loaded into A before the next pass is staged.
+  const uint4* null_vec = vec + n.pull_null;
+  uint32_t cq = wave, cs0 = 0, cnv = 0;
+  col_from(cq, cs0, cnv);
+  uint32_t cln = lane_word(cq);
   uint4 qa[kB], qb[kB];
   {
-    // clamped to the lane's own vectors: these loads follow the lane word's
-    const uint4* p0 = vec + (cnv ? cs0 / kV : 0u) + lane;
+    // the lane's own vectors: these loads follow the lane word's
+    const uint4* p0 = vec + cs0 / kV + lane;
     const uint32_t m0 = cnv ? cln >> kPullShift : 0u;
-    const uint32_t l0 = m0 ? m0 - 1 : 0u;
 #pragma unroll
-    for (uint32_t u = 0; u < kB; ++u) qa[u] = p0[64 * min(u, l0)];
+    for (uint32_t u = 0; u < kB; ++u) qa[u] = *(u < m0 ? p0 + 64 * u : null_vec);
   }
-  while (k < k1) {  // block-uniform: every wave stages every due delay
-    const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
-    __syncthreads();  // the previous delay's readers are done with the bitmap
-    for (uint32_t w = threadIdx.x; w < nw; w += BLK) dyn[w] = ld_step<AGENT>(&src[w]);
+  uint32_t kk = k0;
+  for (;;) {  // block-uniform: every wave stages every pass
+    while (kk < k1 && !((due >> kk) & 1ull)) ++kk;
+    if (kk >= k1) break;
+    const uint32_t kend = min(kk + stage, k1);
+    __syncthreads();  // the previous pass's readers are done with the bitmaps
+    for (uint32_t k = kk; k < kend; ++k) {
+      if (!((due >> k) & 1ull)) continue;
+      const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
+      uint64_t* dst = dyn + (size_t)(k - kk) * bstride;
+      for (uint32_t w = threadIdx.x; w < nw; w += BLK) dst[w] = ld_step<AGENT>(&src[w]);
+    }
     __syncthreads();
-    while (ck == k) {  // the wave's columns of delay k
-      uint32_t nk = ck, nsl = csl + kWaves, ns0 = 0, nnv = 0;
-      col_from(nk, nsl, ns0, nnv);
-      const uint32_t nln = lane_word(nk, nsl);
+    while (cq < qend && k0 + cq / kPullSlices < kend) {  // the wave's columns of this pass
+      uint32_t nq = cq + kWaves, ns0 = 0, nnv = 0;
+      col_from(nq, ns0, nnv);
+      const uint32_t nln = lane_word(nq);
+      // one delay per pass: the bitmap at a fixed LDS address (folded into
+      // the reads' offsets)
+      const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq / kPullSlices - kk) * bstride * 2 : bits0;
       const uint32_t tl = cln & (kPullTB - 1), mine = cln >> kPullShift;  // target, the lane's own vectors
-      const uint32_t last = mine ? mine - 1 : 0u;
-      const uint4* p = vec + cs0 / kV + lane;                 // the lane's vector j at p[64 * j]
-      const uint4* pn = vec + (nnv ? ns0 / kV : 0u) + lane;   // the next column's
-      const uint32_t nlast = nnv ? nnv - 1 : 0u;
+      const uint4* p = vec + cs0 / kV + lane;    // the lane's vector j at p[64 * j]
+      const uint4* pn = vec + ns0 / kV + lane;   // the next column's
       const uint32_t nv = cnv;
       double ae = 0.0, ai = 0.0;
       uint32_t cnt = 0;  // count format: kPullCountBits-bit fields per weight
-      uint32_t rel = 0;  // count format: the running source offset
+      // count format: LDS byte address of the running 32-bit word
+      uint32_t ba = lds_address(bits);
       // sums of the batch of vectors j0 .. j0 + kB - 1 held in qq
       auto sum_batch = [&](const uint4 (&qq)[kB], uint32_t j0) {
 #pragma unroll
         for (uint32_t u = 0; u < kB; ++u) {
-          if (j0 + u >= mine) break;
+          if (j0 + u >= nv) break;  // uniform: past the slice's longest lane
 #pragma unroll
           for (uint32_t c = 0; c < kV; ++c) {
             if (FMT == kPullCount) {
-              // 16-bit entry c: delta, then the tag
-              const uint32_t x = comp(qq[u], c >> 1), h = 16 * (c & 1);
-              rel += __builtin_amdgcn_ubfe(x, h, kPullDeltaBits);
-              const uint32_t tag = __builtin_amdgcn_ubfe(x, h + kPullDeltaBits, 16 - kPullDeltaBits);
-              const uint32_t word = bits[(rel >> 5) & (kPullRelMask >> 5)];
-              cnt += __builtin_amdgcn_ubfe(word, rel, 1u) << (tag * kPullCountBits);  // bit rel % 32
+              // 16-bit entry c (tag | word delta << 2 | bit << 11), the low
+              // or the high half of a 32-bit component: the byte delta of
+              // the running word is the entry masked, the bit's offset its
+              // top five bits; v_bfe and the shift read only the low 5 bits
+              // of their offset operands
+              const uint32_t x = comp(qq[u], c >> 1);
+              const uint32_t e = (c & 1) == 0 ? x : x >> 16;
+              const uint32_t off = (c & 1) == 0 ? x >> 11 : x >> 27;
+              const uint32_t shift = (e << 3) & 31u;  // tag * 8
+              ba += e & (kPullMaxDelta << 2);
+              const uint32_t word = lds_load_u32(ba);
+              cnt += __builtin_amdgcn_ubfe(word, off, 1u) << shift;
             } else {
               const uint32_t e = comp(qq[u], FMT == kPullWide ? 2 * c : c);
               const uint32_t word = bits[(e >> 5) & (kPullRelMask >> 5)];
@@ -853,26 +890,27 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
       uint32_t j0 = 0;
       do {
 #pragma unroll
-        for (uint32_t u = 0; u < kB; ++u) qb[u] = p[64 * min(j0 + kB + u, last)];
+        for (uint32_t u = 0; u < kB; ++u) qb[u] = *(j0 + kB + u < mine ? p + 64 * (j0 + kB + u) : null_vec);
         sum_batch(qa, j0);
         const bool here = j0 + 2 * kB < nv;
 #pragma unroll
         for (uint32_t u = 0; u < kB; ++u) {
-          const uint4* a = here ? p + 64 * min(j0 + 2 * kB + u, last) : pn + 64 * min(u, nlast);
+          const uint32_t j = j0 + 2 * kB + u;
+          const uint4* a = here ? (j < mine ? p + 64 * j : null_vec) : (u < nnv ? pn + 64 * u : null_vec);
           qa[u] = *a;
         }
         sum_batch(qb, j0 + kB);
         j0 += 2 * kB;
       } while (j0 < nv);
-      sums[tl] += ae;
-      sums[kPullTB + tl] += ai;
-      ck = nk;
-      csl = nsl;
+      // exact integers: the order of the additions is free
+      if (ae != 0.0) atomicAdd(&sums[tl], (unsigned long long)(long long)ae);
+      if (ai != 0.0) atomicAdd(&sums[kPullTB + tl], (unsigned long long)(long long)ai);
+      cq = nq;
       cs0 = ns0;
       cnv = nnv;
       cln = nln;
     }
-    k = due_from(k + 1);
+    kk = kend;
   }
   __syncthreads();
   long long* out = n.acc_pull + (size_t)g * 2 * n.npad;
@@ -880,7 +918,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     const uint32_t tgt = b * kPullTB + i;
     if (tgt < n.N) {
       st_agent(&out[tgt], (long long)sums[i]);
-      st_agent(&out[n.npad + tgt], (long long)sums[kPullTB + i]);
+      st_agent(&out[n.npad + tgt], (long long)sums[kPullTB + i]);  // two's complement: negative sums
     }
   }
   // event count: one atomic per wave
@@ -891,23 +929,25 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
 }
 
 // The pull of work item (b, g) in the network's entry format.
-template <int BLK, bool AGENT>
-__device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t* dyn) {
+template <int BLK, bool AGENT, uint32_t KB, bool MULTI>
+__device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint32_t stage,
+                                 uint64_t* dyn) {
   if (n.pull_fmt == kPullWide)
-    pull_body<BLK, kPullWide, AGENT>(n, t, b, g, due, dyn);
+    pull_body<BLK, kPullWide, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
   else if (n.pull_fmt == kPullCount)
-    pull_body<BLK, kPullCount, AGENT>(n, t, b, g, due, dyn);
+    pull_body<BLK, kPullCount, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
   else
-    pull_body<BLK, kPullTable, AGENT>(n, t, b, g, due, dyn);
+    pull_body<BLK, kPullTable, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
 }
 
 // gather(t): the pull of the deferred steps whose synapses are due at t, one
 // (target block, delay group) work item per block iteration.  Exits at once
-// when nothing is due (every block reads the same stamps).
-// At most 72 VGPRs (7 waves per SIMD): as many 256-thread blocks as the LDS
-// of a typical pull block admits stay resident.
-template <int BLK>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(7))) gather_kernel(DevNet n, uint32_t j) {
+// when nothing is due (every block reads the same stamps).  Batches of KB
+// vectors per lane (2: at most 72 VGPRs, 7 waves per SIMD; 4: at most 96,
+// 5 waves) with one delay staged at a time, or n.pull_stage delays staged
+// at once (fewer blocks fit the LDS, so up to 128 VGPRs).
+template <int BLK, uint32_t KB, int WPE>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) gather_kernel(DevNet n, uint32_t j) {
   extern __shared__ __align__(16) uint64_t dyn[];
   __shared__ uint32_t hev[64];
   const uint64_t t = *n.t_base + j;
@@ -917,7 +957,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(7))) g
   const uint64_t due = due_mask<false>(n, t);
   if (!due) return;
   for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-    pull_item<BLK, false>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
+    pull_item<BLK, false, KB, WPE == 4>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, n.pull_stage, dyn);
     __syncthreads();  // LDS reuse by the next work item
   }
 }
@@ -1047,7 +1087,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
     const uint64_t due = carry ? f.due : due_mask<true>(n, t);
     if (due) {
       for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-        pull_item<BLK, true>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, dyn);
+        pull_item<BLK, true, 2, false>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, 1u, dyn);
         __syncthreads();
       }
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
@@ -1064,8 +1104,10 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       f.spk_prev = sum;
       f.defer = n.pull_off && sum >= n.defer_spikes;
     } else {
-      for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK)
-        update_block<MODEL, BLK, true>(n, t, base + threadIdx.x, usm);
+      for (uint32_t base = blockIdx.x * BLK; base < n.T; base += gridDim.x * BLK) {
+        if (base != blockIdx.x * BLK) __syncthreads();  // LDS reuse
+        update_block<MODEL, BLK, true>(n, t, base + threadIdx.x, usm, base == blockIdx.x * BLK);
+      }
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
     }
   }
@@ -1112,15 +1154,23 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t value, uint64_t count) {
 
 }  // namespace
 
-hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_t s) {
-#define SZSNN_UPD(M)                                                                          \
-  switch (block) {                                                                            \
-    case 64: update_kernel<M, 64><<<(n.T + 63) / 64, 64, 0, s>>>(n, j); break;                 \
-    case 128: update_kernel<M, 128><<<(n.T + 127) / 128, 128, 0, s>>>(n, j); break;            \
-    case 256: update_kernel<M, 256><<<(n.T + 255) / 256, 256, 0, s>>>(n, j); break;            \
-    case 512: update_kernel<M, 512><<<(n.T + 511) / 512, 512, 0, s>>>(n, j); break;            \
-    case 1024: update_kernel<M, 1024><<<(n.T + 1023) / 1024, 1024, 0, s>>>(n, j); break;       \
-    default: return hipErrorInvalidValue;                                                     \
+hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, uint32_t npl, hipStream_t s) {
+#define SZSNN_UPD_NPL(M, B, P) update_kernel<M, B, P><<<(n.T + B * P - 1) / (B * P), B, 0, s>>>(n, j)
+#define SZSNN_UPD_B(M, B)                           \
+  switch (npl) {                                    \
+    case 1: SZSNN_UPD_NPL(M, B, 1); break;          \
+    case 2: SZSNN_UPD_NPL(M, B, 2); break;          \
+    case 4: SZSNN_UPD_NPL(M, B, 4); break;          \
+    default: return hipErrorInvalidValue;           \
+  }
+#define SZSNN_UPD(M)                                \
+  switch (block) {                                  \
+    case 64: SZSNN_UPD_B(M, 64) break;              \
+    case 128: SZSNN_UPD_B(M, 128) break;            \
+    case 256: SZSNN_UPD_B(M, 256) break;            \
+    case 512: SZSNN_UPD_B(M, 512) break;            \
+    case 1024: SZSNN_UPD_B(M, 1024) break;          \
+    default: return hipErrorInvalidValue;           \
   }
   switch (n.model) {
     case kIfCurrExp: SZSNN_UPD(kIfCurrExp) break;
@@ -1129,24 +1179,42 @@ hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, hipStream_
     default: return hipErrorInvalidValue;
   }
 #undef SZSNN_UPD
+#undef SZSNN_UPD_B
+#undef SZSNN_UPD_NPL
   return hipGetLastError();
 }
 
-size_t pull_dyn_bytes_for(uint32_t words, uint32_t fmt, uint32_t wn) {
-  return (size_t)pull_bitmap_u64(words) * 8 + (size_t)2 * kPullTB * 8 + (fmt != kPullWide ? (size_t)32 * wn : 0);
+size_t pull_dyn_bytes_for(uint32_t words, uint32_t fmt, uint32_t wn, uint32_t stage) {
+  return (size_t)stage * pull_bitmap_u64(words) * 8 + (size_t)2 * kPullTB * 8 + (fmt != kPullWide ? (size_t)32 * wn : 0);
 }
 
-size_t pull_dyn_bytes(const DevNet& n) {
-  return n.pull_off ? pull_dyn_bytes_for(n.pull_words, n.pull_fmt, n.pull_wn) : 0;
+size_t pull_dyn_bytes(const DevNet& n, uint32_t stage) {
+  return n.pull_off ? pull_dyn_bytes_for(n.pull_words, n.pull_fmt, n.pull_wn, stage) : 0;
 }
 
-int gather_max_blocks_per_cu(size_t dyn, uint32_t block) {
+namespace {
+// The gather kernel of one block size, staging depth and batch.
+template <int BLK>
+const void* gather_fn(uint32_t stage, uint32_t batch) {
+  if (stage <= 1)
+    return batch == 4 ? reinterpret_cast<const void*>(&gather_kernel<BLK, 4, 5>)
+                      : reinterpret_cast<const void*>(&gather_kernel<BLK, 2, 7>);
+  return batch == 4 ? reinterpret_cast<const void*>(&gather_kernel<BLK, 4, 4>)
+                    : reinterpret_cast<const void*>(&gather_kernel<BLK, 2, 4>);
+}
+const void* gather_fn_for(uint32_t block, uint32_t stage, uint32_t batch) {
+  return block == 128    ? gather_fn<128>(stage, batch)
+         : block == 256  ? gather_fn<256>(stage, batch)
+         : block == 512  ? gather_fn<512>(stage, batch)
+         : block == 1024 ? gather_fn<1024>(stage, batch)
+                         : nullptr;
+}
+}  // namespace
+
+int gather_max_blocks_per_cu(size_t dyn, uint32_t block, uint32_t stage, uint32_t batch) {
   int nb = 0;
-  const void* f = block == 128   ? reinterpret_cast<const void*>(&gather_kernel<128>)
-                  : block == 256 ? reinterpret_cast<const void*>(&gather_kernel<256>)
-                  : block == 512 ? reinterpret_cast<const void*>(&gather_kernel<512>)
-                                 : reinterpret_cast<const void*>(&gather_kernel<1024>);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)block, dyn) != hipSuccess) return 0;
+  const void* f = gather_fn_for(block, stage, batch);
+  if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)block, dyn) != hipSuccess) return 0;
   return nb;
 }
 
@@ -1159,15 +1227,25 @@ hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s) {
 
 hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {
   if (!n.pull_off) return hipSuccess;  // no pull layout: every step pushes
-  if (grid == 0) return hipErrorInvalidValue;
-  const size_t dyn = pull_dyn_bytes(n);
+  if (grid == 0 || n.pull_stage == 0) return hipErrorInvalidValue;
+  const size_t dyn = pull_dyn_bytes(n, n.pull_stage);
+  const bool wide = n.pull_batch == 4;
+#define SZSNN_GATHER(B)                                                        \
+  if (n.pull_stage <= 1) {                                                     \
+    if (wide) gather_kernel<B, 4, 5><<<grid, B, dyn, s>>>(n, j);               \
+    else gather_kernel<B, 2, 7><<<grid, B, dyn, s>>>(n, j);                    \
+  } else {                                                                     \
+    if (wide) gather_kernel<B, 4, 4><<<grid, B, dyn, s>>>(n, j);               \
+    else gather_kernel<B, 2, 4><<<grid, B, dyn, s>>>(n, j);                    \
+  }
   switch (block) {
-    case 128: gather_kernel<128><<<grid, 128, dyn, s>>>(n, j); break;
-    case 256: gather_kernel<256><<<grid, 256, dyn, s>>>(n, j); break;
-    case 512: gather_kernel<512><<<grid, 512, dyn, s>>>(n, j); break;
-    case 1024: gather_kernel<1024><<<grid, 1024, dyn, s>>>(n, j); break;
+    case 128: SZSNN_GATHER(128) break;
+    case 256: SZSNN_GATHER(256) break;
+    case 512: SZSNN_GATHER(512) break;
+    case 1024: SZSNN_GATHER(1024) break;
     default: return hipErrorInvalidValue;
   }
+#undef SZSNN_GATHER
   return hipGetLastError();
 }
 
@@ -1179,7 +1257,7 @@ hipError_t coop_launch(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint3
   uint64_t* tb = t_base;
   void* args[] = {&nn, &ns, &tb};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&persistent_kernel<MODEL, BLK>), dim3(grid),
-                                    dim3(BLK), args, pull_dyn_bytes(n), s);
+                                    dim3(BLK), args, pull_dyn_bytes(n, 1), s);
 }
 template <int MODEL, int BLK>
 int coop_max_blocks(size_t dyn) {
@@ -1197,7 +1275,6 @@ hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base,
   switch (block) {                                                             \
     case 256: return coop_launch<M, 256>(n, nsteps, t_base, grid, s);          \
     case 512: return coop_launch<M, 512>(n, nsteps, t_base, grid, s);          \
-    case 1024: return coop_launch<M, 1024>(n, nsteps, t_base, grid, s);        \
     default: return hipErrorInvalidValue;                                      \
   }
   switch (n.model) {
@@ -1210,13 +1287,12 @@ hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base,
 }
 
 int persistent_max_blocks_per_cu(const DevNet& n, uint32_t block) {
-  const size_t dyn = pull_dyn_bytes(n);
+  const size_t dyn = pull_dyn_bytes(n, 1);
   const int model = n.model;
 #define SZSNN_OCC(M)                                       \
   switch (block) {                                         \
     case 256: return coop_max_blocks<M, 256>(dyn);            \
     case 512: return coop_max_blocks<M, 512>(dyn);            \
-    case 1024: return coop_max_blocks<M, 1024>(dyn);          \
     default: return 0;                                     \
   }
   switch (model) {

@@ -17,7 +17,9 @@
 //     lane, rounded up to whole 16-byte vectors), stored as vectors
 //     [L / V][64 lanes][V entries], so the wave's j-th 16-byte load is
 //     contiguous; each lane also records how many vectors hold its own
-//     entries (the rest of its column is padding and is not read).  Entries
+//     entries (the rest of its column is padding and is not read: loads past
+//     a lane's own vectors read one null vector of padding entries stored
+//     after the last slice).  Entries
 //     name the source relative to the block's first staged id, in one of
 //     three formats (kPullWide, kPullCount, kPullTable in szsnn_constants.h):
 //     8-byte (offset, weight); with at most kPullCountFields distinct weights
@@ -191,9 +193,10 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
   auto widx = [&](int32_t w) {
     return (uint32_t)(std::lower_bound(wtab.begin() + 1, wtab.end(), w) - wtab.begin());
   };
-  // count format: a gap wider than kPullMaxDelta takes skip entries first
+  // count format: a word gap wider than kPullMaxDelta takes skip entries
+  // first (each advancing kPullMaxDelta words)
   auto skips = [](uint32_t gap) { return gap > kPullMaxDelta ? (gap - 1) / kPullMaxDelta : 0u; };
-  std::vector<uint32_t> prev;  // count format: last rel of each (segment, target)
+  std::vector<uint32_t> prev;  // count format: 32-bit word of the last source of each (segment, target)
   if (fmt == kPullCount) {
     // entries per (segment, target) with the skips; sources arrive in
     // ascending order per target (id-major rows)
@@ -205,11 +208,11 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
           const uint32_t tg = L.syn[q].x;
           const uint64_t b = tg >> kPullShift;
           const uint64_t key = (b * D + k) * kPullTB + (tg & (kPullTB - 1));
-          const uint32_t rel = (uint32_t)id - lo[b] * 64;
-          const uint64_t c = (uint64_t)cnt[key] + skips(rel - prev[key]) + 1;
+          const uint32_t word = ((uint32_t)id - lo[b] * 64) >> 5;
+          const uint64_t c = (uint64_t)cnt[key] + skips(word - prev[key]) + 1;
           if (c > 0xffffffffull) return true;
           cnt[key] = (uint32_t)c;
-          prev[key] = rel;
+          prev[key] = word;
         }
   }
   // target order per segment and slice offsets; each slice lane records its
@@ -245,8 +248,8 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     return (uint64_t)pos_of[key] + (uint64_t)(j / V) * 64 * V + j % V;
   };
   if (fmt == kPullCount) {
-    const uint16_t pad = (uint16_t)(kPullSkipTag << kPullDeltaBits);  // skip by 0
-    std::vector<uint16_t> ps(pos, pad);
+    const uint16_t pad = count_entry(kPullSkipTag, 0, 0);  // skip by 0
+    std::vector<uint16_t> ps(pos + V, pad);  // and the null vector
     std::fill(prev.begin(), prev.end(), 0u);
     for (uint64_t id = 0; id < T; ++id)
       for (uint64_t k = 0; k < D; ++k)
@@ -255,19 +258,19 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
           const uint64_t b = tg >> kPullShift;
           const uint64_t key = (b * D + k) * kPullTB + (tg & (kPullTB - 1));
           const uint32_t rel = (uint32_t)id - lo[b] * 64;
-          uint32_t gap = rel - prev[key];
+          uint32_t gap = (rel >> 5) - prev[key];
           for (uint32_t i = skips(gap); i > 0; --i) {
-            ps[place(key)] = (uint16_t)(kPullSkipTag << kPullDeltaBits | kPullMaxDelta);
+            ps[place(key)] = count_entry(kPullSkipTag, 0, kPullMaxDelta);
             gap -= kPullMaxDelta;
           }
           const uint32_t tag = widx((int32_t)L.syn[q].y) - 1;
-          ps[place(key)] = (uint16_t)(tag << kPullDeltaBits | gap);
-          prev[key] = rel;
+          ps[place(key)] = count_entry(tag, rel & 31u, gap);
+          prev[key] = rel >> 5;
         }
     L.pull_syn16.swap(ps);
     L.pull_wtab.swap(wtab);
   } else if (fmt == kPullTable) {
-    std::vector<uint32_t> ps(pos, pad_rel);  // padding: the zero word, index 0
+    std::vector<uint32_t> ps(pos + V, pad_rel);  // padding and the null vector: the zero word, index 0
     for (uint64_t id = 0; id < T; ++id)
       for (uint64_t k = 0; k < D; ++k)
         for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {
@@ -279,7 +282,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     L.pull_syn32.swap(ps);
     L.pull_wtab.swap(wtab);
   } else {
-    std::vector<LayoutPair> ps(pos, LayoutPair{pad_rel, 0u});  // padding: the zero word, weight 0
+    std::vector<LayoutPair> ps(pos + V, LayoutPair{pad_rel, 0u});  // padding, null vector: the zero word, weight 0
     for (uint64_t id = 0; id < T; ++id)
       for (uint64_t k = 0; k < D; ++k)
         for (uint32_t q = L.row_off[id * D + k]; q < L.row_off[id * D + k + 1]; ++q) {

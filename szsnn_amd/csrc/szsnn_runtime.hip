@@ -43,8 +43,11 @@ struct szsnn_ctx {
 
 struct Tiling {
   uint32_t update_block = 256;
+  uint32_t update_npl = 1;       // ids per lane of the update kernel
   uint32_t gather_block = 256;
   uint32_t gather_grid = 0;
+  uint32_t gather_stage = 1;     // delays whose bitmaps a pull pass stages at once
+  uint32_t gather_batch = 2;     // 16-byte vectors per lane and batch of the pull
   int32_t mode = 0;              // SZSNN_MODE_GRAPH or SZSNN_MODE_PERSISTENT
   uint32_t persist_block = 256;
   uint32_t persist_grid = 0;     // 0: the co-resident maximum
@@ -70,6 +73,7 @@ struct szsnn_net {
   uint32_t last_forced = 0;       // ids forced by the last szsnn_step, at step last_forced_t
   uint64_t last_forced_t = 0;
   float dense_fraction = 0.04f;   // set per pull entry format at upload
+  uint32_t stage_max = 1;         // the most delays a pull pass can stage (LDS)
   // batched networks: per-instance id ranges (empty for a plain network)
   std::vector<uint32_t> inst_noff, inst_soff, inst_n, inst_m;
 };
@@ -132,6 +136,26 @@ void free_graph(szsnn_net* net) {
   net->graph_dirty = true;
 }
 
+// The launch configuration takes effect: the staging depth travels in the
+// device description, and the captured step graph is rebuilt.
+void apply_tiling(szsnn_net* net) {
+  net->dn.pull_stage = net->tiling.gather_stage;
+  net->dn.pull_batch = net->tiling.gather_batch;
+  net->graph_dirty = true;
+}
+
+// The most delays one pull pass can stage: their bitmaps with the sums and
+// weight table in at most 64 KiB of LDS, no more than a delay group holds,
+// and a size the gather kernel can be resident with.
+uint32_t stage_limit(const DevNet& n) {
+  if (!n.pull_off) return 1;
+  const uint32_t per_group = (n.D + n.pull_groups - 1) / n.pull_groups;
+  uint32_t s = 1;
+  while (s < per_group && pull_dyn_bytes_for(n.pull_words, n.pull_fmt, n.pull_wn, s + 1) <= 65536) ++s;
+  while (s > 1 && gather_max_blocks_per_cu(pull_dyn_bytes_for(n.pull_words, n.pull_fmt, n.pull_wn, s), 256, s) <= 0) --s;
+  return s;
+}
+
 uint32_t default_gather_grid(const szsnn_net* net, uint32_t block) {
   (void)block;
   const uint64_t items = (uint64_t)net->dn.pull_nb * net->dn.pull_groups;
@@ -158,7 +182,7 @@ uint32_t defer_threshold(const szsnn_net* net, float dense_fraction) {
 int launch_step(szsnn_net* net, const DevNet& dn, uint32_t j) {
   szsnn_ctx* ctx = net->ctx;
   HIP_TRY(ctx, launch_gather(dn, j, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream));
-  HIP_TRY(ctx, launch_update(dn, j, net->tiling.update_block, ctx->stream));
+  HIP_TRY(ctx, launch_update(dn, j, net->tiling.update_block, net->tiling.update_npl, ctx->stream));
   return SZSNN_OK;
 }
 
@@ -434,7 +458,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint32_t groups = 1;
   if (!pull_off.empty()) {
     const int per_cu =
-        gather_max_blocks_per_cu(pull_dyn_bytes_for(L.pull_words, L.pull_fmt, (uint32_t)L.pull_wtab.size()), 256);
+        gather_max_blocks_per_cu(pull_dyn_bytes_for(L.pull_words, L.pull_fmt, (uint32_t)L.pull_wtab.size(), 1), 256, 1);
     const uint64_t slots = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(ctx->cu_count, 1);
     const uint64_t want =
         std::max<uint64_t>(1, std::min<uint64_t>({slots / std::max<uint64_t>(NB, 1), (uint64_t)D, (uint64_t)kMaxPullGroups}));
@@ -583,6 +607,8 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.pull_wn = (uint32_t)L.pull_wtab.size();
   n.pull_lo = d_pull_lo;
   n.pull_nb = d_pull_off ? (uint32_t)NB : 0;
+  // the null vector follows the last slice (its entries per vector by format)
+  n.pull_null = d_pull_off ? pull_off.back() / (L.pull_fmt == kPullWide ? 2u : L.pull_fmt == kPullTable ? 4u : 8u) : 0;
   n.pull_words = pull_words;
   n.heavy_min = heavy_cnt ? kHeavyPushMin : 0;
   n.heavy_cnt = heavy_cnt;
@@ -603,6 +629,9 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   net->info.pull_entry_bytes = !d_pull_off ? 0 : L.pull_fmt == kPullWide ? 8 : L.pull_fmt == kPullTable ? 4 : 2;
   net->info.raster_words = n.W;
   net->tiling.gather_grid = default_gather_grid(net, net->tiling.gather_block);
+  net->stage_max = stage_limit(n);
+  n.pull_stage = 1;
+  n.pull_batch = 2;
   *out = net;
   return SZSNN_OK;
 }
@@ -884,7 +913,7 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
     hipError_t e = hipEventRecord(ev[3 * s], ctx->stream);
     if (e == hipSuccess) e = launch_gather(net->dn, 0, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3 * s + 1], ctx->stream);
-    if (e == hipSuccess) e = launch_update(net->dn, 0, net->tiling.update_block, ctx->stream);
+    if (e == hipSuccess) e = launch_update(net->dn, 0, net->tiling.update_block, net->tiling.update_npl, ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3 * s + 2], ctx->stream);
     if (e == hipSuccess) e = launch_advance(net->t_base, 1, ctx->stream);
     if (e != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_time_kernels: ") + hipGetErrorString(e));
@@ -919,7 +948,69 @@ int szsnn_set_tiling(szsnn_net* net, uint32_t update_block, uint32_t gather_bloc
   if (update_block) net->tiling.update_block = update_block;
   if (gather_block) net->tiling.gather_block = gather_block;
   net->tiling.gather_grid = gather_grid ? gather_grid : default_gather_grid(net, net->tiling.gather_block);
-  net->graph_dirty = true;
+  apply_tiling(net);
+  return SZSNN_OK;
+}
+
+int szsnn_set_tiling_ex(szsnn_net* net, const szsnn_tiling* t) {
+  if (!net || !t) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: bad argument");
+  szsnn_ctx* ctx = net->ctx;
+  auto pow2_in = [](uint32_t v, uint32_t lo, uint32_t hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; };
+  if (!pow2_in(t->update_block, 64, 1024))
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: update_block must be 64, 128, 256, 512 or 1024");
+  if (t->update_ids_per_lane != 1 && t->update_ids_per_lane != 2 && t->update_ids_per_lane != 4)
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: update_ids_per_lane must be 1, 2 or 4");
+  if (!pow2_in(t->gather_block, 128, 1024))
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: gather_block must be 128, 256, 512 or 1024");
+  const uint32_t stage = t->gather_stage ? t->gather_stage : net->stage_max;
+  if (stage > net->stage_max) {
+    std::ostringstream os;
+    os << "szsnn_set_tiling_ex: gather_stage " << stage << " exceeds " << net->stage_max
+       << " (the delays whose bitmaps fit the LDS of one gather block)";
+    return fail(ctx, SZSNN_ERR_INVALID, os.str());
+  }
+  if (t->gather_batch != 2 && t->gather_batch != 4)
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: gather_batch must be 2 or 4");
+  if (net->dn.pull_off &&
+      gather_max_blocks_per_cu(pull_dyn_bytes(net->dn, stage), t->gather_block, stage, t->gather_batch) <= 0)
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: that gather block and stage cannot be resident");
+  if (t->mode != SZSNN_MODE_GRAPH && t->mode != SZSNN_MODE_PERSISTENT)
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: unknown mode");
+  if (t->mode == SZSNN_MODE_PERSISTENT) {
+    if (t->persist_block != 256 && t->persist_block != 512)
+      return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: persist_block must be 256 or 512");
+    if (persistent_grid(net, t->persist_block, t->persist_grid) == 0)
+      return fail(ctx, SZSNN_ERR_INVALID, "szsnn_set_tiling_ex: no co-resident occupancy for that persistent block");
+  }
+  Tiling& tl = net->tiling;
+  tl.update_block = t->update_block;
+  tl.update_npl = t->update_ids_per_lane;
+  tl.gather_block = t->gather_block;
+  tl.gather_grid = t->gather_grid ? t->gather_grid : default_gather_grid(net, t->gather_block);
+  tl.gather_stage = stage;
+  tl.gather_batch = t->gather_batch;
+  tl.mode = t->mode;
+  if (t->mode == SZSNN_MODE_PERSISTENT) {
+    tl.persist_block = t->persist_block;
+    tl.persist_grid = t->persist_grid;
+  }
+  apply_tiling(net);
+  return SZSNN_OK;
+}
+
+int szsnn_get_tiling_ex(szsnn_net* net, szsnn_tiling* t) {
+  if (!net || !t) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_get_tiling_ex: bad argument");
+  const Tiling& tl = net->tiling;
+  t->update_block = tl.update_block;
+  t->update_ids_per_lane = tl.update_npl;
+  t->gather_block = tl.gather_block;
+  t->gather_grid = tl.gather_grid;
+  t->gather_stage = tl.gather_stage;
+  t->gather_stage_max = net->stage_max;
+  t->gather_batch = tl.gather_batch;
+  t->mode = tl.mode;
+  t->persist_block = tl.persist_block;
+  t->persist_grid = persistent_grid(net, tl.persist_block, tl.persist_grid);
   return SZSNN_OK;
 }
 
@@ -966,8 +1057,8 @@ int szsnn_set_mode(szsnn_net* net, int32_t mode, uint32_t block, uint32_t grid) 
     return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_mode: unknown mode");
   if (mode == SZSNN_MODE_PERSISTENT) {
     if (block == 0) block = net->tiling.persist_block;
-    if (block != 256 && block != 512 && block != 1024)
-      return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_mode: persistent block must be 256, 512 or 1024");
+    if (block != 256 && block != 512)
+      return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_mode: persistent block must be 256 or 512");
     if (persistent_grid(net, block, grid) == 0)
       return fail(net->ctx, SZSNN_ERR_INVALID, "szsnn_set_mode: no co-resident occupancy for that block");
     net->tiling.persist_block = block;
@@ -1056,6 +1147,7 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
   if (rc) return rc;
   const int cus = std::max(ctx->cu_count, 1);
   const uint32_t ublocks[] = {64, 128, 256, 512, 1024};
+  const uint32_t npls[] = {1, 2, 4};
   const uint32_t gblocks[] = {128, 256, 512, 1024};
   const uint32_t gmult[] = {1, 2, 4, 8, 16};
   Tiling best = net->tiling;
@@ -1065,6 +1157,7 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
   HIP_TRY(ctx, hipEventCreate(&e1));
   auto trial = [&](const Tiling& tl, float* ms) -> int {
     net->tiling = tl;
+    apply_tiling(net);
     free_graph(net);
     int r = snapshot(net, true);
     net->t_host = t0;
@@ -1077,6 +1170,14 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
     HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
     HIP_TRY(ctx, hipEventSynchronize(e1));
     HIP_TRY(ctx, hipEventElapsedTime(ms, e0, e1));
+    // a persistent trial whose grid barrier timed out is not a candidate
+    unsigned long long c[kCntSlots];
+    r = read_counters(net, c);
+    if (r) return r;
+    if (c[kCntOverflow] >= kBarTimeout) {
+      HIP_TRY(ctx, hipMemsetAsync(net->dn.barrier, 0, kBarWords * 4, ctx->stream));
+      *ms = 1e30f;
+    }
     return SZSNN_OK;
   };
   auto consider = [&](const Tiling& tl) -> int {
@@ -1088,26 +1189,42 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
     }
     return r;
   };
-  // stage 1: graph mode, update block at the current gather tiling
+  // stage 1: graph mode, update block x ids per lane at the current gather tiling
   for (uint32_t ub : ublocks) {
-    Tiling tl = best;
-    tl.mode = SZSNN_MODE_GRAPH;
-    tl.update_block = ub;
-    if ((rc = consider(tl))) break;
-  }
-  // stage 2: graph mode, gather block x grid (grid in CU multiples)
-  for (uint32_t gb : gblocks) {
-    for (uint32_t gm : gmult) {
+    for (uint32_t np : npls) {
       if (rc) break;
       Tiling tl = best;
       tl.mode = SZSNN_MODE_GRAPH;
-      tl.gather_block = gb;
-      tl.gather_grid = (uint32_t)cus * gm;
+      tl.update_block = ub;
+      tl.update_npl = np;
       rc = consider(tl);
     }
   }
+  // stage 2: graph mode, gather block x grid (grid in CU multiples) x
+  // staging depth (one delay, or as many as fit)
+  const uint32_t stages[] = {1, net->stage_max};
+  const uint32_t batches[] = {2, 4};
+  for (uint32_t st : stages) {
+    for (uint32_t ba : batches) {
+      for (uint32_t gb : gblocks) {
+        if (net->dn.pull_off && gather_max_blocks_per_cu(pull_dyn_bytes(net->dn, st), gb, st, ba) <= 0) continue;
+        for (uint32_t gm : gmult) {
+          if (rc) break;
+          Tiling tl = best;
+          tl.mode = SZSNN_MODE_GRAPH;
+          tl.gather_block = gb;
+          tl.gather_grid = (uint32_t)cus * gm;
+          tl.gather_stage = st;
+          tl.gather_batch = ba;
+          rc = consider(tl);
+        }
+      }
+      if (!net->dn.pull_off) break;
+    }
+    if (net->stage_max == 1) break;
+  }
   // stage 3: persistent mode, block x grid (CUs/4, CUs/2, CUs, co-resident max)
-  const uint32_t pblocks[] = {256, 512, 1024};
+  const uint32_t pblocks[] = {256, 512};
   const uint32_t pdiv[] = {4, 2, 1};
   for (uint32_t pb : pblocks) {
     if (persistent_max_blocks_per_cu(net->dn, pb) <= 0) continue;
@@ -1123,6 +1240,7 @@ extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   net->tiling = best;
+  apply_tiling(net);
   free_graph(net);
   int r2 = snapshot(net, true);
   net->t_host = t0;

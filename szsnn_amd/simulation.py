@@ -223,6 +223,23 @@ class Simulation:
               self.ctx.handle)
         return {"update_block": a.value, "gather_block": b.value, "gather_grid": c.value}
 
+    def tiling_ex(self):
+        """The whole launch configuration (szsnn_get_tiling_ex) as a dict."""
+        t = _lib.Tiling()
+        check(self.lib.szsnn_get_tiling_ex(self.handle, ctypes.byref(t)), self.ctx.handle)
+        return {name: getattr(t, name) for name, _ in _lib.Tiling._fields_}
+
+    def set_tiling_ex(self, **changes):
+        """Change fields of the launch configuration (the others are kept);
+        gather_stage 0 stages as many delays as the LDS admits."""
+        cur = self.tiling_ex()
+        unknown = set(changes) - set(cur) | ({"gather_stage_max"} & set(changes))
+        if unknown:
+            raise ValueError(f"unknown or read-only tiling fields: {sorted(unknown)}")
+        cur.update(changes)
+        t = _lib.Tiling(**{k: v for k, v in cur.items()})
+        check(self.lib.szsnn_set_tiling_ex(self.handle, ctypes.byref(t)), self.ctx.handle)
+
     def set_gather_policy(self, dense_fraction):
         """Defer a step's spikes to the pull gather when the spikes predicted
         for it (the previous step's less the ids forced in it, plus the ids
@@ -259,7 +276,7 @@ class Simulation:
 
     def autotune(self, steps=100):
         check(self.lib.szsnn_autotune(self.handle, int(steps)), self.ctx.handle)
-        return dict(self.tiling(), **self.mode())
+        return self.tiling_ex()
 
     def close(self):
         if getattr(self, "handle", None):

@@ -146,11 +146,19 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
   const uint32_t fmt = L.pull_fmt;
   const uint32_t V = fmt == kPullWide ? 2 : fmt == kPullTable ? 4 : 8;
   const uint32_t pad_rel = words * 64;
-  const uint16_t skip0 = (uint16_t)(kPullSkipTag << kPullDeltaBits);
+  const uint16_t skip0 = count_entry(kPullSkipTag, 0, 0);
   const size_t nent = fmt == kPullWide ? L.pull_syn.size() : fmt == kPullTable ? L.pull_syn32.size() : L.pull_syn16.size();
   CHECK(L.pull_off.size() == (size_t)L.NB * D * kPullSlices + 1, "pull_off size");
   CHECK(L.pull_lane.size() == (size_t)L.NB * D * kPullTB, "pull_lane size");
-  CHECK(L.pull_off.back() == nent, "entries %zu vs offsets %u", nent, L.pull_off.back());
+  // the slices, then one null vector of padding entries (what loads past a
+  // lane's own vectors read)
+  CHECK(L.pull_off.back() + V == nent, "entries %zu vs offsets %u", nent, L.pull_off.back());
+  for (size_t e = L.pull_off.back(); e < nent; ++e) {
+    const bool pad = fmt == kPullWide    ? (L.pull_syn[e].x == pad_rel && L.pull_syn[e].y == 0)
+                     : fmt == kPullTable ? L.pull_syn32[e] == pad_rel
+                                         : L.pull_syn16[e] == skip0;
+    CHECK(pad, "null vector entry %zu is not padding", e);
+  }
   for (uint32_t bg = 0; bg < L.NB * G; ++bg) {
     const uint32_t b = bg / G, g = bg % G;
     std::fill(sums.begin(), sums.end(), 0.0);
@@ -207,12 +215,15 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
               int32_t w;
               if (fmt == kPullCount) {
                 const uint16_t x = L.pull_syn16[e];
-                run += x & kPullMaxDelta;
-                const uint32_t tag = x >> kPullDeltaBits;
-                CHECK((run >> 6) < nw, "LDS bit read rel=%u nw=%u", run, nw);
-                if (tag == kPullSkipTag) continue;  // gap or padding: no synapse
+                run += count_delta(x);  // the running 32-bit word
+                const uint32_t tag = count_tag(x);
+                CHECK((run >> 1) < nw, "LDS word read word=%u nw=%u", run, nw);
+                if (tag == kPullSkipTag) {  // gap or padding: no synapse
+                  CHECK(count_bit(x) == 0, "skip entry with a bit");
+                  continue;
+                }
                 CHECK(tag < kPullCountFields && tag + 1 < L.pull_wtab.size(), "count tag %u", tag);
-                rel = run;
+                rel = run * 32 + count_bit(x);
                 w = L.pull_wtab[tag + 1];
               } else if (fmt == kPullTable) {
                 const uint32_t x = L.pull_syn32[e];
@@ -364,21 +375,24 @@ int main() {
     check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 1);   // wide entries asked for
   }
   // count format at the delta boundaries: one target whose sources sit at
-  // gaps 0 (duplicate), 16383 (one delta), 16384, 32766 (one skip), 32767,
-  // 32768 (two skips); the lane's decoded sources and weights must come back
+  // word gaps 0 (same word, another bit; then a duplicate), 511 (one delta),
+  // 512, 1022 (one skip), 1023, 1024 (two skips), with bits 0..31 in their
+  // words; the lane's decoded sources and weights must come back
   {
-    const uint32_t gaps[] = {0, 0, 16383, 16384, 32766, 32767, 32768};
+    const uint32_t wgaps[] = {0, 0, 0, 511, 512, 1022, 1023, 1024};
+    const uint32_t bits[] = {5, 31, 31, 0, 17, 1, 30, 9};
     Net n{1, 0, 1, {}, {}, {}, {}};
-    uint32_t id = 5;
+    uint32_t word = 0;
     uint64_t want_entries = 0;
-    for (uint32_t i = 0; i < sizeof(gaps) / sizeof(gaps[0]); ++i) {
-      id += gaps[i];
+    uint32_t id = 0;
+    for (uint32_t i = 0; i < sizeof(wgaps) / sizeof(wgaps[0]); ++i) {
+      word += wgaps[i];
+      id = word * 32 + bits[i];
       n.src.push_back(id);
       n.tgt.push_back(0);
       n.w.push_back(i % 2 ? 0.25f : -0.5f);
       n.dl.push_back(0);
-      const uint32_t gap = i == 0 ? id : gaps[i];  // the first delta counts from rel 0 (block base 0)
-      want_entries += 1 + (gap > kPullMaxDelta ? (gap - 1) / kPullMaxDelta : 0);
+      want_entries += 1 + (wgaps[i] > kPullMaxDelta ? (wgaps[i] - 1) / kPullMaxDelta : 0);
     }
     n.M = id + 1;
     const szsnn_net_desc d = desc_of(n);
@@ -400,9 +414,9 @@ int main() {
     for (uint32_t j = 0; j < mine; ++j)
       for (uint32_t c = 0; c < 8; ++c) {
         const uint16_t x = L.pull_syn16[(size_t)(L.pull_off[sl] / 8 + 64 * j + lane) * 8 + c];
-        run += x & kPullMaxDelta;
+        run += count_delta(x);
         ++seen;
-        if ((x >> kPullDeltaBits) != kPullSkipTag) got.emplace_back(run, L.pull_wtab[(x >> kPullDeltaBits) + 1]);
+        if (count_tag(x) != kPullSkipTag) got.emplace_back(run * 32 + count_bit(x), L.pull_wtab[count_tag(x) + 1]);
       }
     CHECK(got.size() == n.src.size(), "decoded %zu of %zu sources", got.size(), n.src.size());
     for (size_t i = 0; i < got.size(); ++i) {

@@ -201,6 +201,12 @@ def test_errors_are_reported(ctx):
         Simulation(ctx, bad)
     with pytest.raises(SzsnnError, match="update_block"):
         sim.set_tiling(update_block=192)
+    with pytest.raises(SzsnnError, match="persistent block must be 256 or 512"):
+        sim.set_mode(1, 1024, 0)
+    with pytest.raises(SzsnnError, match="update_ids_per_lane"):
+        sim.set_tiling_ex(update_ids_per_lane=3)
+    with pytest.raises(SzsnnError, match="gather_batch"):
+        sim.set_tiling_ex(gather_batch=3)
     # the reference's queues could overflow (reported, reference D4); delivery
     # through the ring of accumulator slots has no queue: a tiny
     # queue_capacity is ignored and a full volley still delivers every event
@@ -272,7 +278,7 @@ def test_full_size_if_cond_exp_event_checksum(ctx):
     assert cnt["overflow"] == 0
 
 
-@pytest.mark.parametrize("block,grid", [(256, 0), (512, 16), (1024, 4)])
+@pytest.mark.parametrize("block,grid", [(256, 0), (512, 16), (512, 4)])
 def test_persistent_mode_equals_graph_mode(ctx, block, grid):
     """The cooperative persistent step loop gives the same rasters and state
     as the graph-captured per-kernel loop."""
@@ -413,3 +419,63 @@ def test_narrow_and_wide_pull_entries_agree(ctx, which):
         o, ro, _ = _oracle_run(net, steps, 32)
         np.testing.assert_array_equal(rn, ro)
         assert cn["events"] == o.events
+
+
+@pytest.mark.parametrize("tiling", [
+    {"gather_stage": 0, "gather_batch": 2, "gather_block": 256},
+    {"gather_stage": 0, "gather_batch": 4, "gather_block": 512},
+    {"gather_stage": 0, "gather_batch": 2, "gather_block": 1024},
+    {"gather_stage": 1, "gather_batch": 4, "gather_block": 256},
+    {"gather_stage": 1, "gather_batch": 2, "gather_block": 1024},
+])
+@pytest.mark.parametrize("which", ["izh", "cond_volley"])
+def test_pull_tilings_equal_push(ctx, which, tiling):
+    """The pull's launch shapes (several delays staged per pass, batches of
+    4 vectors, more waves than slices) give the push's rasters, state and
+    event counts, also when only some steps are deferred."""
+    from szsnn_amd import configs
+    from szsnn_amd.simulation import Simulation
+    volley = 0
+    if which == "izh":
+        net, steps = configs.izhikevich2003(), 300
+    else:
+        net, steps, volley = configs.if_cond_exp_100k(n_exc=4000, n_inh=1000, k_exc=200, k_inh=50), 60, 1000
+    print(f"case: pull tiling {tiling} on {which}", flush=True)
+    rp, vp, cp = _run_policy(ctx, net, steps, 2.0, volley)   # push only
+    frac, n_def = _mixed_fraction(net, rp, volley)
+    for fraction in (0.0, frac):
+        s = Simulation(ctx, net)
+        s.set_tiling_ex(**tiling)
+        t = s.tiling_ex()
+        assert t["gather_stage"] == (t["gather_stage_max"] if tiling["gather_stage"] == 0 else 1)
+        s.set_gather_policy(fraction)
+        s.set_volley(volley)
+        s.set_recording(0, steps)
+        s.run(steps)
+        np.testing.assert_array_equal(s.raster(), rp)
+        np.testing.assert_array_equal(s.get_vars(), vp)
+        c = s.counters()
+        assert c["events"] == cp["events"] and c["rows"] == cp["rows"]
+        s.close()
+
+
+def test_update_ids_per_lane_equal_one(ctx):
+    """Update blocks serving 2 or 4 id blocks (ids per lane) give the same
+    run as one id per lane, in graph mode with pushes and with pulls."""
+    from szsnn_amd.configs import if_curr_exp_10k, izhikevich2003
+    from szsnn_amd.simulation import Simulation
+    for net, steps in ((izhikevich2003(n_exc=1600, n_inh=400), 300),
+                       (if_curr_exp_10k(n_exc=2000, n_inh=500, n_poisson=300, poisson_hz=100.0, w_poisson=1.5), 400)):
+        runs = []
+        for npl, block in ((1, 256), (2, 256), (4, 128), (4, 1024)):
+            s = Simulation(ctx, net)
+            s.set_tiling_ex(update_block=block, update_ids_per_lane=npl)
+            s.set_recording(0, steps)
+            s.run(steps)
+            runs.append((s.raster(), s.get_vars(), s.counters()))
+            s.close()
+        for r in runs[1:]:
+            np.testing.assert_array_equal(r[0], runs[0][0])
+            np.testing.assert_array_equal(r[1], runs[0][1])
+            for k in ("events", "spikes", "rows", "drive_events"):
+                assert r[2][k] == runs[0][2][k], k
