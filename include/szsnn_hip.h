@@ -262,6 +262,11 @@ int szsnn_autotune(szsnn_net* net, uint32_t steps);
  *                     same with min(W1[i][j], 0), for rows r < rows of X
  *                     [rows][n_in] and W1 [n_in][H], over the nonzero X in
  *                     ascending i (spiking inputs are sparse).
+ *   input_grad      : dW1[i][j] = sum_r X[r][i] * (W1[i][j] >= 0 ? LE[r][j] : LI[r][j])
+ *                     over the rows r with X[r][i] != 0, ascending (the weight
+ *                     gradient of input_currents' sign-split currents); needs a
+ *                     device workspace of szsnn_adj_input_grad_workspace(rows,
+ *                     n_in) bytes (the input spikes as a bit matrix).
  *   readout_forward : y_t = alpha y_{t-1} + drive_t over [T][BR]; y0 may be NULL.
  *   readout_backward: lam_t = scale (y_t - target_t) + alpha lam_{t+1}. */
 int szsnn_adj_forward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params, float* state,
@@ -271,6 +276,9 @@ int szsnn_adj_backward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K,
                                float* LE, float* LI);
 int szsnn_adj_input_currents(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
                              float* AE, float* AI);
+size_t szsnn_adj_input_grad_workspace(uint32_t rows, uint32_t n_in);
+int szsnn_adj_input_grad(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
+                         const float* LE, const float* LI, float* dW1, void* workspace, size_t workspace_bytes);
 int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,
                           float* y);
 int szsnn_readout_backward(void* stream, uint32_t BR, uint32_t T, float alpha, float scale, const float* y,

@@ -18,9 +18,10 @@
 //   lse_t  = lv'_t * (r_t ? 0 : 1/cm) + Ce,  lsi_t likewise with Ci
 //   Cv = lv'_t * (r_t ? 1 : 1 - 1/tau_m);  Ce = (1-1/tau_e) lse_t;  Ci = (1-1/tau_i) lsi_t
 // The carries (Cv, Ce, Ci) cross segment boundaries through `carry`.
-// dL/dW1 = sum_t x_{t-1} (lse_t or lsi_t by the weight's sign) is formed by
-// the caller with library GEMMs; the input currents AE, AI of all steps come
-// from input_currents_kernel, which adds only the nonzero inputs.
+// dL/dW1[i][j] = sum_t x_{t-1}[i] (lse_t[j] or lsi_t[j] by the sign of
+// W1[i][j]) comes from input_grad_kernel, which walks only the steps where
+// input i spiked; the input currents AE, AI of all steps come from
+// input_currents_kernel, which adds only the nonzero inputs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -226,6 +227,115 @@ __global__ void __launch_bounds__(kInBlock) input_currents_kernel(uint32_t n_in,
   }
 }
 
+// Input spikes as a bit matrix: bit k of XT[i][w] is X[64 w + k][i] != 0
+// (rows x n_in row-major X; XT [n_in][ceil(rows / 64)]).  Thread (i, w)
+// reads its 64 rows of column i: each of its loads is one contiguous 256-byte
+// piece of a row across the wave.  Every word of XT is written.
+__global__ void __launch_bounds__(64) input_bits_kernel(uint32_t rows, uint32_t n_in, uint32_t words,
+                                                        const float* __restrict__ X, uint64_t* __restrict__ XT) {
+  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+  const uint32_t w = blockIdx.y;
+  if (i >= n_in) return;  // no barrier in this kernel
+  const uint32_t r0 = w * 64;
+  const uint32_t nr = min(64u, rows - r0);
+  uint64_t m = 0;
+  for (uint32_t k = 0; k < nr; ++k) m |= (uint64_t)(X[(size_t)(r0 + k) * n_in + i] != 0.0f) << k;
+  XT[(size_t)i * words + w] = m;
+}
+
+// dW1[i][j] = sum over rows r with X[r][i] != 0, ascending, of X[r][i] *
+// (W1[i][j] >= 0 ? LE[r][j] : LI[r][j]): the gradient of the sign-split
+// input currents (the E current takes the nonnegative weights).  One block
+// per input i and column chunk; each thread keeps kGradCols columns
+// (strided by the block size, so each LE/LI row is read coalesced), and the
+// block takes the rows kGradRows at a time so their loads are in flight
+// together.  Both adjoints of a column are loaded and one is selected in
+// registers: the signs of neighbouring weights are independent, so a branch
+// per column would split every load into two half-filled ones that fetch
+// the same lines.  The rows a spiking input's sum reads (about 2 % of the
+// rows per input) come from the Infinity Cache, where LE and LI stay
+// resident: about 4 GB per call for B = 64, T = 200, 1000 inputs and 2000
+// hidden neurons.
+constexpr int kGradBlock = 256;
+constexpr int kGradCols = 8;
+constexpr int kGradRows = 8;
+
+__global__ void __launch_bounds__(kGradBlock) input_grad_kernel(uint32_t n_in, uint32_t H, uint32_t words,
+                                                                uint32_t col0, const float* __restrict__ X,
+                                                                const float* __restrict__ W1,
+                                                                const float* __restrict__ LE,
+                                                                const float* __restrict__ LI,
+                                                                const uint64_t* __restrict__ XT,
+                                                                float* __restrict__ dW1) {
+  const uint32_t i = blockIdx.x, tid = threadIdx.x;
+  bool pos[kGradCols];
+  float acc[kGradCols];
+#pragma unroll
+  for (int c = 0; c < kGradCols; ++c) {
+    const uint32_t j = col0 + c * kGradBlock + tid;
+    pos[c] = j < H ? W1[(size_t)i * H + j] >= 0.0f : true;
+    acc[c] = 0.0f;
+  }
+  const uint64_t* xt = XT + (size_t)i * words;
+  uint32_t w = 0;
+  uint64_t m = words ? xt[0] : 0ull;
+  for (;;) {
+    // the next kGradRows rows of input i (r = 0xffffffff: none)
+    uint32_t r[kGradRows];
+#pragma unroll
+    for (int q = 0; q < kGradRows; ++q) {
+      while (!m && w + 1 < words) m = xt[++w];
+      if (m) {
+        r[q] = w * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+      } else {
+        r[q] = 0xffffffffu;
+      }
+    }
+    if (r[0] == 0xffffffffu) break;
+    float x[kGradRows], v[kGradRows][kGradCols];
+#pragma unroll
+    for (int q = 0; q < kGradRows; ++q) {
+      const bool ok = r[q] != 0xffffffffu;
+      const uint32_t rr = ok ? r[q] : r[0];
+      x[q] = ok ? X[(size_t)rr * n_in + i] : 0.0f;
+      const float* le = LE + (size_t)rr * H;
+      const float* li = LI + (size_t)rr * H;
+#pragma unroll
+      for (int c = 0; c < kGradCols; ++c) {
+        const uint32_t j = min(col0 + c * kGradBlock + tid, H - 1);  // past H: a valid column, never stored
+        const float e = le[j], in = li[j];
+        v[q][c] = pos[c] ? e : in;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kGradRows; ++q)
+#pragma unroll
+      for (int c = 0; c < kGradCols; ++c) acc[c] += x[q] * v[q][c];
+  }
+#pragma unroll
+  for (int c = 0; c < kGradCols; ++c) {
+    const uint32_t j = col0 + c * kGradBlock + tid;
+    if (j < H) dW1[(size_t)i * H + j] = acc[c];
+  }
+}
+
+// Bytes of the input gradient's scratch for `rows` rows of n_in inputs:
+// the input bit matrix.
+size_t input_grad_bytes(uint32_t rows, uint32_t n_in) { return (size_t)n_in * ((rows + 63) / 64) * 8; }
+
+int launch_input_grad(hipStream_t st, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
+                      const float* LE, const float* LI, float* dW1, uint64_t* XT) {
+  const uint32_t words = (rows + 63) / 64;
+  input_bits_kernel<<<dim3((n_in + 63) / 64, words), 64, 0, st>>>(rows, n_in, words, X, XT);
+  hipError_t e = hipGetLastError();
+  for (uint32_t col0 = 0; e == hipSuccess && col0 < H; col0 += kGradCols * kGradBlock) {
+    input_grad_kernel<<<n_in, kGradBlock, 0, st>>>(n_in, H, words, col0, X, W1, LE, LI, XT, dW1);
+    e = hipGetLastError();
+  }
+  return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP;
+}
+
 int launch_check(hipError_t e) { return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP; }
 
 }  // namespace
@@ -261,6 +371,15 @@ int szsnn_adj_input_currents(void* stream, uint32_t rows, uint32_t n_in, uint32_
     if (rc != SZSNN_OK) return rc;
   }
   return SZSNN_OK;
+}
+
+size_t szsnn_adj_input_grad_workspace(uint32_t rows, uint32_t n_in) { return input_grad_bytes(rows, n_in); }
+
+int szsnn_adj_input_grad(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
+                         const float* LE, const float* LI, float* dW1, void* workspace, size_t workspace_bytes) {
+  if (!rows || !n_in || !H || !X || !W1 || !LE || !LI || !dW1 || !workspace) return SZSNN_ERR_INVALID;
+  if (workspace_bytes < input_grad_bytes(rows, n_in)) return SZSNN_ERR_INVALID;
+  return launch_input_grad((hipStream_t)stream, rows, n_in, H, X, W1, LE, LI, dW1, (uint64_t*)workspace);
 }
 
 int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,

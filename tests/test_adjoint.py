@@ -210,3 +210,44 @@ def test_input_currents_equal_sign_split_products(cuda, rows, n_in, H):
     assert torch.allclose(AE.cpu().double(), re, rtol=1e-5, atol=1e-6)
     assert torch.allclose(AI.cpu().double(), ri, rtol=1e-5, atol=1e-6)
     assert float(AE[rows // 2].abs().max()) == 0.0 and float(AI[rows // 2].abs().max()) == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,n_in,H", [(12800, 1000, 2000), (300, 77, 2100), (65, 513, 4500), (1, 5, 3)])
+def test_input_grad_equals_sign_selected_products(cuda, rows, n_in, H):
+    """szsnn_adj_input_grad (rows where the input spiked only, ascending)
+    against the FP64 products X^T LE and X^T LI selected by the weight's
+    sign; inputs with values other than 0/1, an input that never spikes,
+    row counts not multiples of 64, widths beyond one column pass, zero
+    weights (sign >= 0: the E adjoint)."""
+    import ctypes
+    import torch
+    from szsnn_amd import _lib
+    lib = _lib.load()
+    g = torch.Generator().manual_seed(rows * 3 + H)
+    X = (torch.rand(rows, n_in, generator=g) < 0.03).float() * (1.0 + (torch.rand(rows, n_in, generator=g) < 0.2))
+    X[:, n_in // 2] = 0.0
+    W = torch.randn(n_in, H, generator=g) * 0.1
+    W[0, : min(H, 7)] = 0.0
+    LE = torch.randn(rows, H, generator=g)
+    LI = torch.randn(rows, H, generator=g)
+    d = {k: v.to(cuda).contiguous() for k, v in (("X", X), ("W", W), ("LE", LE), ("LI", LI))}
+    nbytes = lib.szsnn_adj_input_grad_workspace(rows, n_in)
+    assert nbytes == n_in * ((rows + 63) // 64) * 8
+    ws = torch.empty(nbytes // 8, dtype=torch.int64, device=cuda)
+    dW = torch.full((n_in, H), float("nan"), device=cuda)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.szsnn_adj_input_grad(stream, rows, n_in, H, ptr(d["X"]), ptr(d["W"]), ptr(d["LE"]), ptr(d["LI"]),
+                                    ptr(dW), ptr(ws), nbytes - 1) == -1   # workspace too small
+    rc = lib.szsnn_adj_input_grad(stream, rows, n_in, H, ptr(d["X"]), ptr(d["W"]), ptr(d["LE"]), ptr(d["LI"]),
+                                  ptr(dW), ptr(ws), nbytes)
+    assert rc == 0
+    torch.cuda.synchronize()
+    X64 = X.double()
+    ref = torch.where(W >= 0, X64.t() @ LE.double(), X64.t() @ LI.double())
+    # FP32 sums over the ~3 % spiking rows: relative to the scale of the sum
+    scale = (X64.t() @ LE.double().abs()).clamp(min=1.0)
+    err = ((dW.cpu().double() - ref).abs() / scale).max()
+    assert float(err) <= 1e-5, float(err)
+    assert float(dW[n_in // 2].abs().max()) == 0.0
