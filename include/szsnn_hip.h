@@ -279,6 +279,19 @@ int szsnn_adj_input_currents(void* stream, uint32_t rows, uint32_t n_in, uint32_
 size_t szsnn_adj_input_grad_workspace(uint32_t rows, uint32_t n_in);
 int szsnn_adj_input_grad(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
                          const float* LE, const float* LI, float* dW1, void* workspace, size_t workspace_bytes);
+/* szsnn_adj_input_grad without a workspace argument and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * it).  The input bit matrix lives in a 16 MiB device array of the library
+ * (one per GPU, held until the process ends): n_in * ceil(rows / 64) must not
+ * exceed 2^21, else SZSNN_ERR_INVALID with a message in szsnn_last_error(NULL).
+ * Every word of the scratch a call uses is written before it is read, so
+ * nothing is cleared.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams or threads) race.  Results equal szsnn_adj_input_grad's
+ * bit for bit. */
+int szsnn_adj_input_grad_noalloc(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X,
+                                 const float* W1, const float* LE, const float* LI, float* dW1);
 int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,
                           float* y);
 int szsnn_readout_backward(void* stream, uint32_t BR, uint32_t T, float alpha, float scale, const float* y,

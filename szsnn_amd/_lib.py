@@ -153,6 +153,8 @@ PROTOTYPES = [
     ("szsnn_adj_input_grad_workspace", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32]),
     ("szsnn_adj_input_grad", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
                                              _vp, _vp, _vp, ctypes.c_size_t]),
+    ("szsnn_adj_input_grad_noalloc", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp,
+                                                     _vp, _vp, _vp, _vp]),
     ("szsnn_readout_forward", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, _vp, _vp,
                                              _vp]),
     ("szsnn_readout_backward", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,

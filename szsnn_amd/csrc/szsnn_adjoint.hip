@@ -25,7 +25,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/szsnn_hip.h"
+
+namespace szsnn {
+void set_global_error(const char* msg);  // szsnn_runtime.hip: the message szsnn_last_error(NULL) returns
+}
 
 namespace {
 
@@ -336,6 +342,17 @@ int launch_input_grad(hipStream_t st, uint32_t rows, uint32_t n_in, uint32_t H, 
   return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP;
 }
 
+// Scratch of szsnn_adj_input_grad_noalloc: the input bit matrix, in a
+// file-scope device array (each device holds its own copy at its own
+// address, looked up once per device with hipGetSymbolAddress).
+constexpr size_t kScratchWords = (size_t)2 << 20;  // 16 MiB
+static __device__ __align__(256) uint64_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+uint64_t* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
 int launch_check(hipError_t e) { return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP; }
 
 }  // namespace
@@ -380,6 +397,25 @@ int szsnn_adj_input_grad(void* stream, uint32_t rows, uint32_t n_in, uint32_t H,
   if (!rows || !n_in || !H || !X || !W1 || !LE || !LI || !dW1 || !workspace) return SZSNN_ERR_INVALID;
   if (workspace_bytes < input_grad_bytes(rows, n_in)) return SZSNN_ERR_INVALID;
   return launch_input_grad((hipStream_t)stream, rows, n_in, H, X, W1, LE, LI, dW1, (uint64_t*)workspace);
+}
+
+int szsnn_adj_input_grad_noalloc(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X,
+                                 const float* W1, const float* LE, const float* LI, float* dW1) {
+  if (!rows || !n_in || !H || !X || !W1 || !LE || !LI || !dW1) return SZSNN_ERR_INVALID;
+  if (input_grad_bytes(rows, n_in) > kScratchWords * 8) {
+    szsnn::set_global_error(
+        "szsnn_adj_input_grad_noalloc: n_in * ceil(rows / 64) exceeds the 2^21 words (16 MiB) of its scratch; "
+        "use szsnn_adj_input_grad with a workspace");
+    return SZSNN_ERR_INVALID;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SZSNN_ERR_HIP;
+  // the device's copy of the scratch array, looked up once with the device current
+  std::call_once(g_scratch_once[dev], [dev] {
+    g_scratch_err[dev] = hipGetSymbolAddress(reinterpret_cast<void**>(&g_scratch_ptr[dev]), HIP_SYMBOL(scratch));
+  });
+  if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev]) return SZSNN_ERR_HIP;
+  return launch_input_grad((hipStream_t)stream, rows, n_in, H, X, W1, LE, LI, dW1, g_scratch_ptr[dev]);  // the device copy
 }
 
 int szsnn_readout_forward(void* stream, uint32_t BR, uint32_t T, float alpha, const float* y0, const float* drive,

@@ -33,6 +33,10 @@ void set_global_error(const std::string& m) {
 }
 }  // namespace
 
+namespace szsnn {
+void set_global_error(const char* msg) { ::set_global_error(std::string(msg)); }
+}  // namespace szsnn
+
 struct szsnn_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
