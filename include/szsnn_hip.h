@@ -40,7 +40,9 @@ extern "C" {
 #define SZSNN_ERR_HIP (-2)      /* HIP runtime failure */
 #define SZSNN_ERR_OVERFLOW (-3) /* a device queue overflowed (spikes would be lost) */
 #define SZSNN_ERR_NOMEM (-4)    /* allocation failed */
-#define SZSNN_ERR_STATE (-5)    /* call not valid in the current state */
+#define SZSNN_ERR_STATE (-5)    /* call not valid in the current state (also: after a persistent launch's
+                                   grid barrier timed out, the network's state is lost and stepping calls
+                                   fail until it is uploaded again) */
 
 #define SZSNN_MODEL_IF_CURR_EXP 0 /* params: v_rest cm tau_m tau_refrac tau_syn_E tau_syn_I i_offset v_reset v_thresh; vars: v ie ii fired */
 #define SZSNN_MODEL_IZHIKEVICH 1  /* params: a b c d; vars: v u */
@@ -170,6 +172,20 @@ int szsnn_reset_counters(szsnn_net* net);
  * launched one kernel at a time (no graph) and returns the summed durations of
  * the gather and update kernels in milliseconds. */
 int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, double* update_ms);
+
+/* One kernel of the current step, launched alone and waited for: the
+ * reference's single-kernel harnesses Simulation::testUpdate / testScatter /
+ * testGather (szsnn1.1/src/Simulation.ipp:142-210).  SZSNN_KERNEL_UPDATE is
+ * the neuron (or source) update of the step and the scatter of its spikes,
+ * which this library runs as one kernel; SZSNN_KERNEL_GATHER is the pull of
+ * the synapses due at the step (it does nothing unless a deferred step has
+ * synapses due).  The step counter does not advance, as in the reference's
+ * harnesses: a following szsnn_step runs the same step again, so these are
+ * for inspecting one kernel's effect (szsnn_get_vars) and time (ms), not
+ * for stepping. */
+#define SZSNN_KERNEL_GATHER 0
+#define SZSNN_KERNEL_UPDATE 1
+int szsnn_test_kernel(szsnn_net* net, int32_t kernel, double* ms);
 
 /* Launch configuration of the graph mode (update block, gather block and
  * grid); 0 keeps the current value.  szsnn_autotune times candidates of both

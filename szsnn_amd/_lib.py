@@ -27,6 +27,9 @@ SOURCE_POISSON = 2
 MODE_GRAPH = 0
 MODE_PERSISTENT = 1
 
+KERNEL_GATHER = 0
+KERNEL_UPDATE = 1
+
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 c_u64p = ctypes.POINTER(ctypes.c_uint64)
 c_u16p = ctypes.POINTER(ctypes.c_uint16)
@@ -137,6 +140,7 @@ PROTOTYPES = [
     ("szsnn_set_tiling", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("szsnn_get_tiling", ctypes.c_int, [_vp, c_u32p, c_u32p, c_u32p]),
     ("szsnn_autotune", ctypes.c_int, [_vp, ctypes.c_uint32]),
+    ("szsnn_test_kernel", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     ("szsnn_set_tiling_ex", ctypes.c_int, [_vp, ctypes.POINTER(Tiling)]),
     ("szsnn_get_tiling_ex", ctypes.c_int, [_vp, ctypes.POINTER(Tiling)]),
     ("szsnn_set_mode", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]),

@@ -78,6 +78,9 @@ struct szsnn_net {
   uint64_t last_forced_t = 0;
   float dense_fraction = 0.04f;   // set per pull entry format at upload
   uint32_t stage_max = 1;         // the most delays a pull pass can stage (LDS)
+  // set when a persistent launch's grid barrier timed out: its blocks left
+  // the step loop at different steps, so the state is no longer one step's
+  bool lost = false;
   // batched networks: per-instance id ranges (empty for a plain network)
   std::vector<uint32_t> inst_noff, inst_soff, inst_n, inst_m;
 };
@@ -271,9 +274,24 @@ int check_overflow(szsnn_net* net) {
   int rc = read_counters(net, c);
   if (rc) return rc;
   if (c[kCntOverflow] >= kBarTimeout) {
+    // the blocks left the step loop where their waits ended: the step
+    // counter is the device's (the launch did not advance it), and the
+    // state is not one step's, so the network must be uploaded again
     (void)hipMemset(net->dn.barrier, 0, kBarWords * 4);
-    return fail(net->ctx, SZSNN_ERR_STATE, "persistent loop: a grid barrier wait timed out (grid not co-resident)");
+    uint64_t tb = net->t_host;
+    if (hipMemcpy(&tb, net->t_base, 8, hipMemcpyDeviceToHost) == hipSuccess) net->t_host = tb;
+    net->lost = true;
+    return fail(net->ctx, SZSNN_ERR_STATE,
+                "persistent loop: a grid barrier wait timed out (grid not co-resident); the network state is lost, "
+                "upload it again");
   }
+  return SZSNN_OK;
+}
+
+int check_usable(szsnn_net* net, const char* what) {
+  if (net->lost)
+    return fail(net->ctx, SZSNN_ERR_STATE,
+                std::string(what) + ": the network state was lost when a grid barrier timed out; upload it again");
   return SZSNN_OK;
 }
 
@@ -765,6 +783,7 @@ int szsnn_net_get_info(szsnn_net* net, szsnn_net_info* out) {
 
 int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, uint64_t* out_bitmap) {
   if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_step: net is NULL");
+  if (int rc = check_usable(net, "szsnn_step")) return rc;
   szsnn_ctx* ctx = net->ctx;
   DevNet dn = net->dn;
   if (n_forced) {
@@ -811,6 +830,7 @@ int szsnn_step(szsnn_net* net, const uint32_t* forced_ids, uint32_t n_forced, ui
 
 int szsnn_run_async(szsnn_net* net, uint64_t n_steps) {
   if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_run_async: net is NULL");
+  if (int rc = check_usable(net, "szsnn_run")) return rc;
   return run_steps(net, n_steps);
 }
 
@@ -908,6 +928,7 @@ int szsnn_reset_counters(szsnn_net* net) {
 
 int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, double* update_ms) {
   if (!net || !gather_ms || !update_ms) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_time_kernels: bad argument");
+  if (int rc = check_usable(net, "szsnn_time_kernels")) return rc;
   szsnn_ctx* ctx = net->ctx;
   // three events per step: before gather, between gather and update, after update
   std::vector<hipEvent_t> ev(3 * n_steps);
@@ -939,6 +960,35 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
   if (e != hipSuccess) return fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_time_kernels: ") + hipGetErrorString(e));
   *gather_ms = g;
   *update_ms = u;
+  return check_overflow(net);
+}
+
+int szsnn_test_kernel(szsnn_net* net, int32_t kernel, double* ms) {
+  if (!net) return fail(nullptr, SZSNN_ERR_INVALID, "szsnn_test_kernel: net is NULL");
+  if (int rc = check_usable(net, "szsnn_test_kernel")) return rc;
+  szsnn_ctx* ctx = net->ctx;
+  if (kernel != SZSNN_KERNEL_GATHER && kernel != SZSNN_KERNEL_UPDATE)
+    return fail(ctx, SZSNN_ERR_INVALID, "szsnn_test_kernel: kernel must be SZSNN_KERNEL_GATHER or SZSNN_KERNEL_UPDATE");
+  hipEvent_t e0, e1;
+  HIP_TRY(ctx, hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_test_kernel: ") + hipGetErrorString(e));
+  }
+  e = hipEventRecord(e0, ctx->stream);
+  if (e == hipSuccess)
+    e = kernel == SZSNN_KERNEL_GATHER
+            ? launch_gather(net->dn, 0, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream)
+            : launch_update(net->dn, 0, net->tiling.update_block, net->tiling.update_npl, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_test_kernel: ") + hipGetErrorString(e));
+  if (ms) *ms = t;
   return check_overflow(net);
 }
 
@@ -1137,6 +1187,9 @@ int snapshot(szsnn_net* net, bool restore) {
     else
       HIP_TRY(ctx, hipMemcpyAsync(net->snap[i].first, regs[i].p, regs[i].bytes, hipMemcpyDeviceToDevice, ctx->stream));
   }
+  // a restored state starts with clean grid-barrier counters (a trial whose
+  // barrier timed out may have left leaf counts behind)
+  if (restore) HIP_TRY(ctx, hipMemsetAsync(net->dn.barrier, 0, kBarWords * 4, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return SZSNN_OK;
 }
@@ -1145,6 +1198,7 @@ int snapshot(szsnn_net* net, bool restore) {
 
 extern "C" int szsnn_autotune(szsnn_net* net, uint32_t steps) {
   if (!net || steps == 0) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_autotune: bad argument");
+  if (int rc = check_usable(net, "szsnn_autotune")) return rc;
   szsnn_ctx* ctx = net->ctx;
   const uint64_t t0 = net->t_host;
   int rc = snapshot(net, false);

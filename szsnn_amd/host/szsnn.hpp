@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <fstream>
 #include <iomanip>
+#include <iostream>
 #include <iterator>
 #include <map>
 #include <set>
@@ -365,6 +366,7 @@ class Simulation {
     szsnn_net_info info{};
     check(szsnn_net_get_info(h, &info), ctx_.handle(), "Simulation::Simulation()");
     n_neurons_ = info.n_neurons;
+    n_vars_ = info.n_vars;
     bitmap_.assign(info.raster_words, 0);
   }
 
@@ -388,6 +390,38 @@ class Simulation {
   size_t getTime() const { return (size_t)wall_ms_; }
   szsnn_net* handle() const { return net_.get(); }
 
+  // Single-kernel harnesses (Simulation.ipp:142-210): one kernel of the
+  // current step, launched alone and waited for; the cycle does not advance.
+  // The scatter is part of the update kernel here, so testScatter launches
+  // the update.  Each returns the kernel's time in ms.
+  double testUpdate() { return test_kernel(SZSNN_KERNEL_UPDATE, "UPDATE"); }
+  double testScatter() { return test_kernel(SZSNN_KERNEL_UPDATE, "SCATTER"); }
+  double testGather() { return test_kernel(SZSNN_KERNEL_GATHER, "GATHER"); }
+
+  // State readback (the reference keeps host mirrors of the device state in
+  // its public NeuronData / SpikeBuffer members, Simulation.hpp:100-105):
+  // neuron variables [var][neuron], the counters since upload, the spike
+  // ids of the last stepAndFired.
+  unsigned getNeuronCount() const { return n_neurons_; }
+  unsigned getVarCount() const { return n_vars_; }
+  std::vector<float> getVars() const {
+    std::vector<float> v((size_t)n_vars_ * n_neurons_);
+    check(szsnn_get_vars(net_.get(), v.data()), ctx_.handle(), "Simulation::getVars()");
+    return v;
+  }
+  void setVars(const std::vector<float>& v) {
+    if (v.size() != (size_t)n_vars_ * n_neurons_)
+      throw SzsnnException("SZSNN::Simulation::setVars() error: expected var_count x neuron_count values");
+    check(szsnn_set_vars(net_.get(), v.data()), ctx_.handle(), "Simulation::setVars()");
+  }
+  szsnn_counters getCounters() const {
+    szsnn_counters c{};
+    check(szsnn_get_counters(net_.get(), &c), ctx_.handle(), "Simulation::getCounters()");
+    return c;
+  }
+  uint64_t getCycle() const { return getCounters().cycle; }
+  const firing_output& spikes() const { return fired_; }
+
  private:
   void do_step(const firing_input* in, uint64_t* out) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -395,6 +429,12 @@ class Simulation {
     const uint32_t n = ids ? (uint32_t)in->size() : 0;
     check(szsnn_step(net_.get(), ids, n, out), ctx_.handle(), "Simulation::step()");
     wall_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  double test_kernel(int32_t which, const char* name) {
+    std::cout << "Simulation test -> " << name << " KERNEL" << std::endl;
+    double ms = 0;
+    check(szsnn_test_kernel(net_.get(), which, &ms), ctx_.handle(), "Simulation::test()");
+    return ms;
   }
   // SpikeBuffer::bitvectorToSparse (SpikeBuffer.cpp:146-165): ascending neuron ids
   const firing_output& decode() {
@@ -413,7 +453,7 @@ class Simulation {
 
   Context ctx_;
   std::shared_ptr<szsnn_net> net_;
-  unsigned n_neurons_ = 0;
+  unsigned n_neurons_ = 0, n_vars_ = 0;
   std::vector<uint64_t> bitmap_;
   firing_output fired_;
   double wall_ms_ = 0;

@@ -165,6 +165,27 @@ class Simulation:
     def getTime(self):
         return self._wall_ms
 
+    # single-kernel harnesses (Simulation.ipp:142-210): one kernel of the
+    # current step, launched alone and waited for; the cycle does not advance
+    def testUpdate(self):
+        """The update kernel of the current step (with the scatter of its
+        spikes); returns its time in ms."""
+        return self._test_kernel(_lib.KERNEL_UPDATE)
+
+    def testScatter(self):
+        """The scatter runs inside the update kernel here: the same launch as
+        testUpdate."""
+        return self._test_kernel(_lib.KERNEL_UPDATE)
+
+    def testGather(self):
+        """The gather kernel of the current step; returns its time in ms."""
+        return self._test_kernel(_lib.KERNEL_GATHER)
+
+    def _test_kernel(self, which):
+        ms = ctypes.c_double()
+        check(self.lib.szsnn_test_kernel(self.handle, int(which), ctypes.byref(ms)), self.ctx.handle)
+        return ms.value
+
     # -- batched API ----------------------------------------------------
     def run(self, n_steps, sync=True):
         t0 = time.perf_counter()

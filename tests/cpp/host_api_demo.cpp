@@ -1,10 +1,15 @@
 // Driver in the style of the reference's main.cpp (szsnn1.1/src/main.cpp:312-391):
 // build a NetworkGraph from a .conn file, add Const_rate inputs, step with
 // stepAndFired (plus forced input at some steps) and record the spikes.
-//   host_api_demo <conn file> <steps> <spike file>
+//   host_api_demo <conn file> <steps> <spike file> [vars file]
+// With a vars file: after the run, the neuron variables (getVars), then a
+// gather and an update of the next step launched alone (testGather,
+// testUpdate: the cycle does not advance) and the variables again.
 // Exit codes: 0 ok, 2 usage, 3 SzsnnException (message on stderr).
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
 
 #include "../../szsnn_amd/host/szsnn.hpp"
@@ -12,8 +17,8 @@
 using namespace szsnn;
 
 int main(int argc, char** argv) {
-  if (argc != 4) {
-    std::cerr << "usage: host_api_demo <conn file> <steps> <spike file>\n";
+  if (argc != 4 && argc != 5) {
+    std::cerr << "usage: host_api_demo <conn file> <steps> <spike file> [vars file]\n";
     return 2;
   }
   try {
@@ -52,6 +57,20 @@ int main(int argc, char** argv) {
     for (unsigned t = 0; t < steps; ++t) {
       const auto& fired = (t % 25 == 3) ? sim.stepAndFired({1u, 50u, 99u}) : sim.stepAndFired();
       rec.recordCycleSpikes(t, fired);
+    }
+    if (argc == 5) {
+      std::ofstream vf(argv[4]);
+      vf << std::setprecision(9);
+      auto dump = [&](const std::vector<float>& v) {
+        for (size_t i = 0; i < v.size(); ++i) vf << (i ? " " : "") << v[i];
+        vf << "\n";
+      };
+      dump(sim.getVars());
+      const double g = sim.testGather(), u = sim.testUpdate();
+      dump(sim.getVars());
+      const szsnn_counters c = sim.getCounters();
+      vf << sim.getNeuronCount() << " " << sim.getVarCount() << " " << c.cycle << " " << c.spikes << "\n";
+      std::cout << "kernels gather " << g << " ms update " << u << " ms" << std::endl;
     }
     std::cout << "ok " << ctx.getDeviceName() << " " << sim.getTime() << " ms" << std::endl;
   } catch (const SzsnnException& e) {

@@ -102,6 +102,35 @@ def test_cpp_driver_matches_oracle(tmp_path):
     assert got.sum() > 20  # the network is active (forced volleys plus Const_rate drive)
 
 
+@pytest.mark.gpu
+def test_cpp_state_readback_and_single_kernel_harness(tmp_path):
+    """Simulation::getVars after a run equals the FP32 oracle's state; the
+    single-kernel harness (testGather, then testUpdate: the next step's
+    kernels launched alone) leaves the state of one more oracle step and the
+    cycle where it was (Simulation.ipp:142-210)."""
+    from oracle.pyoracle import Oracle
+    steps = 300
+    vf = tmp_path / "vars.txt"
+    r = subprocess.run([_binary(), CONN, str(steps), str(tmp_path / "spikes.txt"), str(vf)], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Simulation test -> GATHER KERNEL" in r.stdout and "Simulation test -> UPDATE KERNEL" in r.stdout
+    lines = vf.read_text().splitlines()
+    v0 = np.array(lines[0].split(), np.float32).reshape(2, 100)
+    v1 = np.array(lines[1].split(), np.float32).reshape(2, 100)
+    n, nv, cycle, spikes = (int(x) for x in lines[2].split())
+    assert (n, nv, cycle) == (100, 2, steps) and spikes > 0
+    net = demo_network()
+    forced = [(t, i) for t in range(steps) if t % 25 == 3 for i in (1, 50, 99)]
+    assert steps % 25 != 3   # step `steps` (the harness's) is not forced
+    o = Oracle(net, 32)
+    o.run(steps, forced=forced)
+    np.testing.assert_array_equal(v0, o.vars().astype(np.float32))
+    o.run(1)
+    np.testing.assert_array_equal(v1, o.vars().astype(np.float32))
+    assert not np.array_equal(v0, v1)
+
+
 def test_cpp_construction_and_record_classes(tmp_path):
     """Population files, NetworkCreator -> NetworkGraph and record::SpikeRecorder
     of the C++ layer (tests/cpp/construction_check.cpp), compiled on the host
