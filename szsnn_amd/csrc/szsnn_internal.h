@@ -176,14 +176,19 @@ __host__ __device__ inline unsigned long long* counter(unsigned long long* base,
   return base + ((size_t)slot * kCntStripes + (stripe % kCntStripes)) * kCntStride;
 }
 
-// Delivery-time counter rings: which = 0 events, 1 rows; each entry on its own
-// 128-byte line.
+// Delivery-time counter ring: per slot and stripe one word, the rows due at
+// the slot's step in the low 32 bits and the events above (each at most the
+// stored synapse count, < 2^32, per step: a synapse delivers at most once per
+// step and a row holds at least one synapse), so one atomic adds both; each
+// entry on its own 128-byte line.
 constexpr uint32_t kRingStripes = 64;
-constexpr size_t kRingCntWords(uint32_t R) { return (size_t)R * 2 * kRingStripes * kCntStride; }
+constexpr size_t kRingCntWords(uint32_t R) { return (size_t)R * kRingStripes * kCntStride; }
 constexpr size_t kSpkRingWords = (size_t)4 * kRingStripes * kCntStride;
-__host__ __device__ inline unsigned long long* ring_counter(unsigned long long* base, uint32_t slot, uint32_t which,
-                                                            uint32_t stripe) {
-  return base + (((size_t)slot * 2 + which) * kRingStripes + (stripe % kRingStripes)) * kCntStride;
+__host__ __device__ inline unsigned long long ring_pack(uint32_t rows, uint32_t events) {
+  return (unsigned long long)rows | ((unsigned long long)events << 32);
+}
+__host__ __device__ inline unsigned long long* ring_counter(unsigned long long* base, uint32_t slot, uint32_t stripe) {
+  return base + ((size_t)slot * kRingStripes + (stripe % kRingStripes)) * kCntStride;
 }
 __host__ __device__ inline unsigned long long* spk_counter(unsigned long long* base, uint64_t step, uint32_t stripe) {
   return base + ((size_t)(step & 3) * kRingStripes + (stripe % kRingStripes)) * kCntStride;

@@ -280,7 +280,7 @@ __device__ inline void heavy_push(const DevNet& n, uint64_t s, uint32_t hc, uint
   if (threadIdx.x < n.D) {
     uint32_t ds = first_slot + threadIdx.x;
     if (ds >= n.R) ds -= n.R;
-    if (ev[threadIdx.x]) atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)ev[threadIdx.x]);
+    if (ev[threadIdx.x]) atomicAdd(ring_counter(n.ring_cnt, ds, blockIdx.x), ring_pack(0u, ev[threadIdx.x]));
   }
 }
 
@@ -314,11 +314,11 @@ struct StepFacts {
 // with all its loads issued together: the pull's due mask (as due_mask)
 // into sm.due, and the spikes of step t - 1 into sm.spk_prev (LDS atomics of
 // the stripes, no wave reduction).  With `fold` (block 0), lane s also reads
-// stripe s of the events and of the rows due at t into fold_e / fold_r;
+// stripe s of the rows and events due at t (packed) into fold;
 // fold_finish moves them into the counters at the end of the step.
 template <int BLK, bool AGENT>
-__device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bool fold, UpdateSmem<BLK>& sm,
-                                  unsigned long long& fold_e, unsigned long long& fold_r) {
+__device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bool do_fold, UpdateSmem<BLK>& sm,
+                                  unsigned long long& fold) {
   static_assert(kRingStripes == 64, "one stripe per lane");
   const uint32_t lane = __lane_id();
   if (lane == 0) sm.spk_prev = 0;
@@ -327,28 +327,29 @@ __device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bo
   const uint32_t k = lane < n.D ? lane : 0u;
   const uint64_t stamp = n.pull_off ? ld_step<AGENT>(&n.dstamp[(t - k - 1) % n.R]) : 0ull;
   const unsigned long long spk = ld_step<AGENT>(spk_counter(n.spk_ring, t - 1, lane));
-  fold_e = fold ? ld_step<AGENT>(ring_counter(n.ring_cnt, slot, 0, lane)) : 0ull;
-  fold_r = fold ? ld_step<AGENT>(ring_counter(n.ring_cnt, slot, 1, lane)) : 0ull;
+  fold = do_fold ? ld_step<AGENT>(ring_counter(n.ring_cnt, slot, lane)) : 0ull;
   const bool stamp_ok = n.pull_off && lane < n.D && t >= (uint64_t)lane + 1 && stamp == t - lane;
   const uint64_t due = __ballot(stamp_ok);
   if (lane == 0) sm.due = due;
   if (t > 0 && spk) atomicAdd(&sm.spk_prev, spk);
 }
 
+// Adds a packed delivery-ring word (rows low, events high) to stripe q of the
+// rows and events counters.
+__device__ inline void fold_counters(const DevNet& n, uint32_t q, unsigned long long v) {
+  if (v >> 32) atomicAdd(counter(n.counters, kCntEvents, q), v >> 32);
+  if (v & 0xffffffffull) atomicAdd(counter(n.counters, kCntRows, q), v & 0xffffffffull);
+}
+
 // Block 0, one wave, end of step t: add the events and rows due at t (read
 // by step_facts) to the counters, stripe by stripe, clear their ring
 // stripes, and clear the spike-count slot that step t + 1 will fill.  Done
 // last so that these writes do not hold up the step's own loads.
-__device__ inline void fold_finish(const DevNet& n, uint64_t t, uint32_t slot, unsigned long long fold_e,
-                                   unsigned long long fold_r) {
+__device__ inline void fold_finish(const DevNet& n, uint64_t t, uint32_t slot, unsigned long long fold) {
   const uint32_t lane = __lane_id();
-  if (fold_e) {
-    st_agent(ring_counter(n.ring_cnt, slot, 0, lane), 0ull);
-    atomicAdd(counter(n.counters, kCntEvents, lane), fold_e);
-  }
-  if (fold_r) {
-    st_agent(ring_counter(n.ring_cnt, slot, 1, lane), 0ull);
-    atomicAdd(counter(n.counters, kCntRows, lane), fold_r);
+  if (fold) {
+    st_agent(ring_counter(n.ring_cnt, slot, lane), 0ull);
+    fold_counters(n, lane, fold);
   }
   st_agent(spk_counter(n.spk_ring, t + 1, lane), 0ull);
 }
@@ -431,7 +432,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     if (threadIdx.x < n.drive_k) sm.cdf[threadIdx.x] = n.drive_cdf[threadIdx.x];
   }
   if (id < n.T && n.forced_bits) fbits = n.forced_bits[id >> 6];
-  unsigned long long fold_e = 0, fold_r = 0;
+  unsigned long long fold = 0;
   if (threadIdx.x < 64) {
     sm.cnt[threadIdx.x] = 0;
     sm.ev[threadIdx.x] = 0;
@@ -439,7 +440,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       sm.spikes = sm.drive = 0;
       sm.nlist = 0;
     }
-    if (!CARRY) step_facts<BLK, AGENT>(n, t, slot, folder, sm, fold_e, fold_r);
+    if (!CARRY) step_facts<BLK, AGENT>(n, t, slot, folder, sm, fold);
   }
   __syncthreads();
   const uint64_t due = CARRY ? facts.due : sm.due;
@@ -631,14 +632,13 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     __syncthreads();
   }
   // rows and events per delay: one atomic per (block, delay) into the
-  // delivery ring
+  // delivery ring, both counts in one word
   if (threadIdx.x < n.D) {
     const uint32_t k = threadIdx.x;
     uint32_t ds = slot + 1 + k;
     if (ds >= n.R) ds -= n.R;
     const uint32_t c = sm.cnt[k], e = sm.ev[k];
-    if (c) atomicAdd(ring_counter(n.ring_cnt, ds, 1, blockIdx.x), (unsigned long long)c);
-    if (e) atomicAdd(ring_counter(n.ring_cnt, ds, 0, blockIdx.x), (unsigned long long)e);
+    if (c | e) atomicAdd(ring_counter(n.ring_cnt, ds, blockIdx.x), ring_pack(c, e));
   }
   const unsigned long long spikes = sm.spikes, drive = sm.drive;
   if (threadIdx.x == 0) {
@@ -656,7 +656,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       if (drive) atomicAdd(counter(n.counters, kCntDrive, blockIdx.x), drive);
     }
   }
-  if (!CARRY && folder && threadIdx.x < 64) fold_finish(n, t, slot, fold_e, fold_r);
+  if (!CARRY && folder && threadIdx.x < 64) fold_finish(n, t, slot, fold);
   // the new state, stored last (nothing in the step waits for these stores)
   if (!CARRY && id < n.T) store_lane<MODEL>(n, id, ls);
   out_drive = drive;
@@ -1074,13 +1074,13 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
   const uint64_t dmask = n.D >= 64 ? ~0ull : (1ull << n.D) - 1ull;
   // counter fold of the carry path: lane L of wave 0 owns stripe q
   const uint32_t q = blockIdx.x + __lane_id() * gridDim.x;
-  const bool folds = carry && threadIdx.x < 64 && q < 2 * kRingStripes;
+  const bool folds = carry && threadIdx.x < 64 && q < kRingStripes;
   unsigned long long fold_v = 0;
   unsigned long long spikes_acc = 0, drive_acc = 0;
   for (uint64_t s = 0; s < nsteps; ++s) {
     const uint64_t t = t0 + s;
     if (fold_v) {
-      atomicAdd(counter(n.counters, q < kRingStripes ? kCntEvents : kCntRows, q % kRingStripes), fold_v);
+      fold_counters(n, q, fold_v);
       fold_v = 0;
     }
     unsigned long long sum = 0;
@@ -1098,7 +1098,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       spikes_acc += spikes;
       drive_acc += drive;
       const uint32_t slot = (uint32_t)(t % n.R);
-      if (folds) fold_v = atomicExch(ring_counter(n.ring_cnt, slot, q / kRingStripes, q % kRingStripes), 0ull);
+      if (folds) fold_v = atomicExch(ring_counter(n.ring_cnt, slot, q), 0ull);
       if (!grid_barrier(n.barrier, bar_sh, err, threadIdx.x == 0 ? spikes : 0ull, &sum)) return;
       f.due = ((f.due << 1) | (f.defer ? 1ull : 0ull)) & dmask;
       f.spk_prev = sum;
@@ -1111,7 +1111,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
     }
   }
-  if (fold_v) atomicAdd(counter(n.counters, q < kRingStripes ? kCntEvents : kCntRows, q % kRingStripes), fold_v);
+  if (fold_v) fold_counters(n, q, fold_v);
   if (carry) {
     store_lane<MODEL>(n, id, ls);
     if (threadIdx.x == 0) {

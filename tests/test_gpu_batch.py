@@ -26,6 +26,10 @@ def test_batched_izhikevich_equals_plain_runs(ctx):
     steps = 300
     print("case: batch of", [n.n_neurons for n in nets], flush=True)
     bs = BatchSimulation(ctx, nets)
+    # a batch pushes every step from its update blocks (no pull, no heavy
+    # pushes: graph steps without gather launches); the plain runs keep the
+    # default policy, which pulls and posts heavy pushes
+    assert bs.heavy_push() == 0 and bs.gather_policy()["dense_fraction"] > 1.0
     bs.set_recording(0, steps)
     bs.run(steps)
     r = bs.raster()

@@ -112,3 +112,47 @@ def test_persistent_then_single_steps(ctx):
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
     assert runs[0][2]["events"] == runs[1][2]["events"]
     assert runs[0][2]["spikes"] == runs[1][2]["spikes"]
+
+
+@pytest.mark.parametrize("last", ["heavy", "policy"])
+def test_gather_left_out_only_after_the_last_due_pull(ctx, last):
+    """With no heavy pushes and a policy that never defers, steps leave the
+    gather launch out, but only once no deferred step can still be due.
+    20 steps pulled every step (each deferred; delays over all 64 slots),
+    then 20 steps with the last switch still pending: heavy pushes from
+    every block (then heavy pushes off) or still pulling (then a policy
+    that never defers).  The following graph and single steps, the first R
+    of them still pulling the deferred steps' late delays, equal the FP32
+    oracle bit for bit."""
+    from oracle.pyoracle import Oracle
+    from szsnn_amd.simulation import Simulation
+    net = _ragged_net(seed=13)
+    s = Simulation(ctx, net)
+    s.set_volley(200)
+    steps = 40 + 150 + 7
+    s.set_recording(0, steps)
+    print(f"case: gather left out after the last due pull (last switch: {last})", flush=True)
+    s.set_heavy_push(0)
+    s.set_gather_policy(0.0)
+    s.run(20)
+    if last == "heavy":
+        s.set_gather_policy(2.0)
+        s.set_heavy_push(1)
+        s.run(20)
+        s.set_heavy_push(0)
+    else:
+        s.run(20)
+        s.set_gather_policy(2.0)
+    s.run(150)       # three 50-step graphs: gathers in the first two, none in the last
+    for _ in range(7):
+        s.step()
+    r, v, c = s.raster(), s.get_vars(), s.counters()
+    s.close()
+    o = Oracle(net, 32)
+    ft = np.repeat(np.arange(steps, dtype=np.uint64), 200)
+    fi = np.tile(np.arange(200, dtype=np.uint32), steps)
+    ro, _ = o.run(steps, forced=(ft, fi))
+    bad = np.flatnonzero((r != ro).any(axis=1))
+    assert bad.size == 0, f"raster differs first at step {bad[0]}"
+    np.testing.assert_array_equal(v, o.vars().astype(np.float32))
+    assert c["events"] == o.events and c["events"] > 0
