@@ -133,8 +133,10 @@ int szsnn_net_get_info(szsnn_net* net, szsnn_net_info* out);
  * instance order, then their input sources.  Each instance keeps the random
  * streams of its own seed and local ids, so it evolves exactly as its plain
  * upload would.  Instances must share model, frac_bits, drive table and
- * source model.  No reference counterpart: the reference runs one network
- * per Simulation (Simulation.hpp:109-112). */
+ * source model.  A batch starts with heavy pushes off and a gather policy
+ * that never defers (every step pushed by the update blocks).  No reference
+ * counterpart: the reference runs one network per Simulation
+ * (Simulation.hpp:109-112). */
 int szsnn_batch_upload(szsnn_ctx* ctx, const szsnn_net_desc* instances, uint32_t n_instances,
                        szsnn_net** out);
 int szsnn_batch_instance(szsnn_net* net, uint32_t instance, uint32_t* neuron_offset, uint32_t* n_neurons,
@@ -233,7 +235,10 @@ int szsnn_get_tiling_ex(szsnn_net* net, szsnn_tiling* tiling);
  * dense_fraction <= 0 always defers, > 1 never does.  Both give identical
  * sums.  The default follows the pull entry width (where pushing a step
  * costs as much as pulling): 0.012 for 2-byte entries, 0.02 for 4-byte,
- * 0.04 for 8-byte.  has_pull reports whether the layout was built. */
+ * 0.04 for 8-byte (a batch: never).  With no heavy pushes and a policy
+ * that never defers, steps from R = max_delay + 1 steps after the change
+ * on launch no gather kernel.  has_pull reports whether the layout was
+ * built. */
 int szsnn_set_gather_policy(szsnn_net* net, float dense_fraction);
 int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_pull);
 
@@ -243,8 +248,8 @@ int szsnn_get_gather_policy(szsnn_net* net, float* dense_fraction, int32_t* has_
  * entries posts their synapse ranges instead of adding them itself, and the
  * next step's gather launch adds them with its whole grid, so a burst
  * concentrated in a few blocks is not limited by one CU's atomic rate.  The
- * sums are identical either way.  Default 1024; 0 = every block pushes its
- * own ids.  Setting it on a network without a pull layout is an error
+ * sums are identical either way.  Default 1024 (a batch: 0); 0 = every
+ * block pushes its own ids.  Setting it on a network without a pull layout is an error
  * unless min_entries is 0. */
 int szsnn_set_heavy_push(szsnn_net* net, uint32_t min_entries);
 int szsnn_get_heavy_push(szsnn_net* net, uint32_t* min_entries);
