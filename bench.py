@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-single", action="store_true",
+                    help="batched pass: every step a separate launch, no step graphs (rocprofv3 --pmc "
+                         "collection did not survive graph launches on this part)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tiling")
     ap.add_argument("--profile-pass", action="store_true",
                     help="profiler pass: a 1 GiB device copy (calibration), the volley's delay slots filled by "
@@ -337,11 +340,17 @@ def batched_pass(args):
     bs = BatchSimulation(ctx, nets)
     if not args.no_autotune:
         log(f"[bench] batched tiling: {bs.autotune(100)}")
-    bs.run(args.warmup * args.chunk)
+    def run(n):
+        if args.batch_single:
+            for _ in range(n):
+                bs.run(1)
+        else:
+            bs.run(n)
+    run(args.warmup * args.chunk)
     ctx.synchronize()
     c0 = bs.counters()
     t1 = time.perf_counter()
-    bs.run(args.batch_steps)
+    run(args.batch_steps)
     ctx.synchronize()
     wall = time.perf_counter() - t1
     ev = bs.counters()["events"] - c0["events"]
