@@ -26,13 +26,15 @@ constexpr uint32_t kMaxPullGroups = 8;           // delay groups of the pull (wo
 //          ascending source order; the 32-bit word of an entry's source is
 //          the running sum of the word deltas (from word 0 at the start of
 //          each column) and `bit` its bit in that word (rel % 32).  Tag
-//          i < 3 is weight i + 1: the lane counts selected entries in
-//          kPullCountBits-bit fields of one register, tag i's field at
-//          shift kPullCountBits * i.  Tag 3 is a skip (a word gap wider than
-//          kPullMaxDelta, or padding with delta 0): its field is above the
-//          counted ones.
+//          i >= 1 is weight i of the table: the lane counts selected entries
+//          in kPullCountBits-bit fields of one register, tag i's field at
+//          shift kPullCountBits * i.  Tag 0 (the table's inert weight 0) is
+//          a skip: a word gap wider than kPullMaxDelta, or padding, which is
+//          all zero bits (delta 0), so a zero vector is inert; its field,
+//          the lowest, is never read.
 // Wide and table padding entries point at a zero word staged after the
-// bitmap (rel = 64 * staged words), weight 0 or index 0.
+// bitmap (rel = 64 * staged words), weight 0 or index 0; an all-zero wide or
+// table entry (word 0, weight 0 or index 0) is inert too.
 enum : uint32_t { kPullWide = 0, kPullCount = 1, kPullTable = 2 };
 constexpr uint32_t kPullRelBits = 21;
 constexpr uint32_t kPullRelMask = (1u << kPullRelBits) - 1;
@@ -40,7 +42,12 @@ constexpr uint32_t kPullCountFields = 3;
 constexpr uint32_t kPullCountBits = 8;
 constexpr uint32_t kPullDeltaBits = 9;                       // word delta of a count entry
 constexpr uint32_t kPullMaxDelta = (1u << kPullDeltaBits) - 1;
-constexpr uint16_t kPullSkipTag = 3;
+constexpr uint16_t kPullSkipTag = 0;
+// Count-format entries are read through a buffer resource with 32-bit byte
+// offsets; a lane past its own vectors reads at offset kPullOutOfRange and
+// gets a zero vector (no memory access), so the array stays below it.
+constexpr uint32_t kPullOutOfRange = 0x80000000u;
+constexpr uint64_t kPullMaxBufferBytes = 0x7ff00000ull;
 // A count-format entry and its fields.
 constexpr uint16_t count_entry(uint32_t tag, uint32_t bit, uint32_t word_delta) {
   return (uint16_t)(tag | word_delta << 2 | bit << 11);
@@ -50,9 +57,9 @@ constexpr uint32_t count_delta(uint16_t x) { return (x >> 2) & kPullMaxDelta; }
 constexpr uint32_t count_bit(uint16_t x) { return (uint32_t)x >> 11; }
 static_assert((uint64_t)(kPullMaxWords + 1) * 64 <= kPullRelMask, "rel of the zero word must fit");
 static_assert(kPullMaxWeights < (1u << (32 - kPullRelBits)), "weight index must fit");
-static_assert(kPullCountBits * kPullCountFields <= 32, "count fields must fit one register");
-static_assert(kPullCountBits * kPullSkipTag < 32 && kPullCountBits * kPullSkipTag >= kPullCountBits * kPullCountFields,
-              "the skip tag's shift lands above the count fields");
+static_assert(kPullCountBits * (kPullCountFields + 1) <= 32, "the skip field and the count fields fit one register");
+static_assert(kPullSkipTag == 0 && count_entry(kPullSkipTag, 0, 0) == 0, "a zero entry is a skip by 0");
+static_assert(kPullCountFields <= 3, "tags are 2 bits");
 static_assert(kPullCountBits == 8, "the kernel forms a tag's field shift as tag * 8");
 static_assert(2 + 5 + kPullDeltaBits == 16, "count entries are 16 bits");
 

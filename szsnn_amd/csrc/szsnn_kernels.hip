@@ -698,6 +698,25 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
   return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
 }
 
+// Copies nw words of a step's spike bitmap into LDS for a pull pass, every
+// load issued before the first store: one round trip per pass (a loop that
+// stored each word before loading the next would wait once per word, and
+// each wait also drains the entry loads already in flight).
+template <int BLK, bool AGENT>
+__device__ inline void stage_words(uint64_t* dst, const uint64_t* src, uint32_t nw) {
+  constexpr uint32_t kU = 8;  // words per thread and round: 2048 words per round at 256 threads
+  for (uint32_t w0 = threadIdx.x; w0 < nw; w0 += BLK * kU) {
+    uint64_t v[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) v[u] = ld_step<AGENT>(&src[min(w0 + u * BLK, nw - 1u)]);  // no branch per load
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t w = w0 + u * BLK;
+      if (w < nw) dst[w] = v[u];
+    }
+  }
+}
+
 // Dense gather (pull) for target block `b` and delay group `g` over the
 // sliced layout.  The group's due delays are taken in passes of up to
 // `stage` delays: a pass stages the spike bitmaps of steps t-k-1 (the
@@ -713,9 +732,8 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
 // entry one 32-bit LDS read and one bit-field extract give the source's
 // spike bit.  Sums are exact: FP64 sums of integers below 2^53 (wide: E +=
 // b * max(w, 0), I += b * min(w, 0); table: the (E, I) pair at 2 * index +
-// b, zero for b = 0), or, in the count format (2-byte entries: the source as
-// a running sum of deltas), per-weight counts of selected entries in bit
-// fields of one register, multiplied out once per column.  The lane adds its
+// b, zero for b = 0).  (The count format has its own body, pull_count_body,
+// with the same work items, passes and column walk.)  The lane adds its
 // column's sums (integers) into the block's LDS sums of its target with an
 // LDS atomic: a target sits in different slices for different delays, so
 // the columns of one pass that serve it may belong to different waves; the
@@ -727,14 +745,12 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
 template <int BLK, int FMT, bool AGENT, uint32_t KB, bool MULTI>
 __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint32_t stage,
                                  uint64_t* dyn) {
+  static_assert(FMT == kPullWide || FMT == kPullTable, "the count format has pull_count_body");
   if (!MULTI) stage = 1;
   constexpr bool kNarrow = FMT != kPullWide;
-  constexpr uint32_t kV = FMT == kPullWide ? 2 : FMT == kPullTable ? 4 : 8;  // entries per 16-byte vector
+  constexpr uint32_t kV = FMT == kPullWide ? 2 : 4;  // entries per 16-byte vector
   constexpr uint32_t kB = KB;               // vectors per lane and batch
   constexpr uint32_t kWaves = BLK / 64;
-  // count format: flush the fields before any can pass 2^kPullCountBits - 1
-  constexpr uint32_t kFlushVectors = ((1u << kPullCountBits) - 1) / kV / kB * kB;
-  static_assert(kFlushVectors >= kB, "count fields too narrow for one batch");
   const uint32_t bstride = pull_bitmap_u64(n.pull_words);  // u64 per staged bitmap
   const uint32_t* bits0 = reinterpret_cast<const uint32_t*>(dyn);
   unsigned long long* sums = reinterpret_cast<unsigned long long*>(dyn + (size_t)stage * bstride);  // [2][kPullTB]
@@ -751,9 +767,8 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
   const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
   const uint32_t qend = (k1 - k0) * kPullSlices;
-  const uint4* vec = FMT == kPullWide    ? reinterpret_cast<const uint4*>(n.pull_syn)
-                     : FMT == kPullTable ? reinterpret_cast<const uint4*>(n.pull_syn32)
-                                         : reinterpret_cast<const uint4*>(n.pull_syn16);
+  const uint4* vec = FMT == kPullWide ? reinterpret_cast<const uint4*>(n.pull_syn)
+                                      : reinterpret_cast<const uint4*>(n.pull_syn32);
   const uint32_t lane = __lane_id();
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
   uint32_t events = 0;
@@ -811,9 +826,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     __syncthreads();  // the previous pass's readers are done with the bitmaps
     for (uint32_t k = kk; k < kend; ++k) {
       if (!((due >> k) & 1ull)) continue;
-      const uint64_t* src = n.hist + ((t - k - 1) % n.R) * n.W + lo;
-      uint64_t* dst = dyn + (size_t)(k - kk) * bstride;
-      for (uint32_t w = threadIdx.x; w < nw; w += BLK) dst[w] = ld_step<AGENT>(&src[w]);
+      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, n.hist + ((t - k - 1) % n.R) * n.W + lo, nw);
     }
     __syncthreads();
     while (cq < qend && k0 + cq / kPullSlices < kend) {  // the wave's columns of this pass
@@ -828,9 +841,6 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
       const uint4* pn = vec + ns0 / kV + lane;   // the next column's
       const uint32_t nv = cnv;
       double ae = 0.0, ai = 0.0;
-      uint32_t cnt = 0;  // count format: kPullCountBits-bit fields per weight
-      // count format: LDS byte address of the running 32-bit word
-      uint32_t ba = lds_address(bits);
       // sums of the batch of vectors j0 .. j0 + kB - 1 held in qq
       auto sum_batch = [&](const uint4 (&qq)[kB], uint32_t j0) {
 #pragma unroll
@@ -838,51 +848,22 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
           if (j0 + u >= nv) break;  // uniform: past the slice's longest lane
 #pragma unroll
           for (uint32_t c = 0; c < kV; ++c) {
-            if (FMT == kPullCount) {
-              // 16-bit entry c (tag | word delta << 2 | bit << 11), the low
-              // or the high half of a 32-bit component: the byte delta of
-              // the running word is the entry masked, the bit's offset its
-              // top five bits; v_bfe and the shift read only the low 5 bits
-              // of their offset operands
-              const uint32_t x = comp(qq[u], c >> 1);
-              const uint32_t e = (c & 1) == 0 ? x : x >> 16;
-              const uint32_t off = (c & 1) == 0 ? x >> 11 : x >> 27;
-              const uint32_t shift = (e << 3) & 31u;  // tag * 8
-              ba += e & (kPullMaxDelta << 2);
-              const uint32_t word = lds_load_u32(ba);
-              cnt += __builtin_amdgcn_ubfe(word, off, 1u) << shift;
+            const uint32_t e = comp(qq[u], FMT == kPullWide ? 2 * c : c);
+            const uint32_t word = bits[(e >> 5) & (kPullRelMask >> 5)];
+            const uint32_t bit = __builtin_amdgcn_ubfe(word, e, 1u);  // bit e % 32
+            if (FMT == kPullWide) {
+              const int32_t w = (int32_t)comp(qq[u], 2 * c + 1);
+              const double bw = (double)(bit ? w : 0);
+              ae += bw > 0.0 ? bw : 0.0;
+              ai += bw < 0.0 ? bw : 0.0;
+              events += bit;
             } else {
-              const uint32_t e = comp(qq[u], FMT == kPullWide ? 2 * c : c);
-              const uint32_t word = bits[(e >> 5) & (kPullRelMask >> 5)];
-              const uint32_t bit = __builtin_amdgcn_ubfe(word, e, 1u);  // bit e % 32
-              if (FMT == kPullWide) {
-                const int32_t w = (int32_t)comp(qq[u], 2 * c + 1);
-                const double bw = (double)(bit ? w : 0);
-                ae += bw > 0.0 ? bw : 0.0;
-                ai += bw < 0.0 ? bw : 0.0;
-                events += bit;
-              } else {
-                const double2 wv = wt[((e >> kPullRelBits) << 1) | bit];
-                ae += wv.x;
-                ai += wv.y;
-                events += bit;
-              }
+              const double2 wv = wt[((e >> kPullRelBits) << 1) | bit];
+              ae += wv.x;
+              ai += wv.y;
+              events += bit;
             }
           }
-        }
-        if (FMT == kPullCount && ((j0 / kB + 1) % (kFlushVectors / kB) == 0 || j0 + kB >= nv)) {
-          // fields -> sums: field i counts the selected entries of weight i + 1
-#pragma unroll
-          for (uint32_t i = 0; i < kPullCountFields; ++i) {
-            const uint32_t c = (cnt >> (kPullCountBits * i)) & ((1u << kPullCountBits) - 1u);
-            if (i + 1 < n.pull_wn) {
-              const double2 wv = wt[2 * (i + 1) + 1];
-              ae += (double)c * wv.x;
-              ai += (double)c * wv.y;
-              events += c;
-            }
-          }
-          cnt = 0;
         }
       };
       // a column has at least one vector: the loop body runs, so every path
@@ -928,6 +909,230 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   if (lane == 0 && ev) atomicAdd(counter(n.counters, kCntEvents, blockIdx.x * 16 + wave), ev);
 }
 
+typedef uint32_t vec4u __attribute__((ext_vector_type(4)));
+
+// Buffer-resource word 3 of a raw (untyped, unswizzled) buffer on gfx9 parts.
+constexpr int kRawBufferFlags = 0x00020000;
+
+// The field shift of the count entry in the high half of x: the entry's tag
+// (bits 16-17 of x) times 8, in the low five bits of the result (one SDWA
+// shift of the selected half word; a shift instruction reads only those five
+// bits of its amount).
+__device__ inline uint32_t high_field_shift(uint32_t x) {
+  uint32_t r;
+  asm("v_lshlrev_b32_sdwa %0, 3, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1"
+      : "=v"(r)
+      : "v"(x));
+  return r;
+}
+
+// The pull of work item (b, g) in the count format (2-byte entries, at most
+// kPullCountFields distinct weights): the passes and the column walk of
+// pull_body, with the loads, the decode and the sums shaped for the format.
+//  * Entries are read by buffer loads with 32-bit byte offsets into one
+//    buffer resource; the wave's j-th vector of a column takes j * 1024 as
+//    the scalar offset.  A lane past its own vectors loads at
+//    kPullOutOfRange and gets a zero vector (skips by 0, no memory access).
+//  * Per 16-byte vector the eight running word addresses are formed first
+//    and the eight LDS reads issued together.  Per entry: the delta (and,
+//    add), the bit's offset (one shift; the low entry of a 32-bit component
+//    at bits 11-15, the high one at 27-31), the tag's field shift (one
+//    shift), and one bit-field extract added, shifted, into the count
+//    register: kPullCountBits-bit fields, tag i's at shift 8 * i, tag 0
+//    (skips) in the lowest, which is never read.
+//  * The fields are added into per-(tag, target) 32-bit LDS counts at the end
+//    of a column, and every kFlushVectors vectors before a field can fill;
+//    the weights multiply the counts once per target, in 64-bit integers,
+//    when the item stores its sums (exact, so the order is free).
+// dyn: `stage` bitmaps of pull_bitmap_u64(pull_words) u64 each |
+// kPullCountFields * kPullTB u32 counts.
+template <int BLK, bool AGENT, uint32_t KB, bool MULTI>
+__device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due,
+                                       uint32_t stage, uint64_t* dyn) {
+  if (!MULTI) stage = 1;
+  constexpr uint32_t kV = 8;   // entries per 16-byte vector
+  constexpr uint32_t kB = KB;  // vectors per lane and batch
+  constexpr uint32_t kWaves = BLK / 64;
+  constexpr uint32_t kFlushVectors = ((1u << kPullCountBits) - 1) / kV / kB * kB;
+  constexpr uint32_t kDeltaMask = kPullMaxDelta << 2;  // the byte delta of the running word
+  static_assert(kFlushVectors >= kB, "count fields too narrow for one batch");
+  const uint32_t bstride = pull_bitmap_u64(n.pull_words);
+  const uint32_t* bits0 = reinterpret_cast<const uint32_t*>(dyn);
+  uint32_t* cnts = reinterpret_cast<uint32_t*>(dyn + (size_t)stage * bstride);  // [kPullCountFields][kPullTB]
+  for (uint32_t i = threadIdx.x; i < kPullCountFields * kPullTB; i += BLK) cnts[i] = 0u;
+  const uint32_t lo = ld_const(&n.pull_lo[b]);
+  const uint32_t nw = min(n.pull_words, n.W - lo);
+  const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
+  const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
+  const uint32_t qend = (k1 - k0) * kPullSlices;
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint16_t*>(n.pull_syn16), (short)0, (int)((n.pull_null + 1u) * 16u), kRawBufferFlags);
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
+  // col_from moves q to the wave's next nonempty column of a due delay at or
+  // after it (q >= qend: none) and gives its first entry s0 and its vectors
+  // per lane nv (the slice's longest lane).
+  auto col_from = [&](uint32_t& q, uint32_t& s0, uint32_t& nv) {
+    for (; q < qend; q += kWaves) {
+      const uint32_t k = k0 + q / kPullSlices;
+      if (!((due >> k) & 1ull)) continue;
+      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices + q % kPullSlices;
+      s0 = ld_const(&so[0]);
+      const uint32_t s1 = ld_const(&so[1]);
+      if (s1 != s0) {
+        nv = (s1 - s0) / (64 * kV);
+        return;
+      }
+    }
+    nv = 0;
+  };
+  // The lane's pull_lane word of column q, loaded unconditionally: past the
+  // end of the stream it reads a valid word and ignores it.
+  auto lane_word = [&](uint32_t q) {
+    const uint32_t qq = min(q, qend - 1);
+    return n.pull_lane[(size_t)(b * n.D + k0 + qq / kPullSlices) * kPullTB + (qq % kPullSlices) * 64 + lane];
+  };
+  // byte offset of the lane's vector 0 in the slice that starts at entry s0
+  // (slices start at whole vectors of all 64 lanes)
+  auto voff_of = [&](uint32_t s0) { return s0 * 2u + lane * 16u; };
+  auto vload = [&](uint32_t voff, uint32_t j) {
+    return __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, j * 1024u, 0);
+  };
+  // Entry loads run a batch ahead of the sums in two buffers, A and B, that
+  // take turns, as in pull_body; the batch after a column's last is the next
+  // column's first (its slice's vectors, before its lane words are known;
+  // a slice's stored padding is zero skips).
+  uint32_t cq = wave, cs0 = 0, cnv = 0;
+  col_from(cq, cs0, cnv);
+  uint32_t cln = lane_word(cq);
+  vec4u qa[kB], qb[kB];
+  {
+    const uint32_t v0 = voff_of(cs0);
+    const uint32_t m0 = cnv ? cln >> kPullShift : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kB; ++u) qa[u] = vload(u < m0 ? v0 : kPullOutOfRange, u);
+  }
+  uint32_t kk = k0;
+  for (;;) {  // block-uniform: every wave stages every pass
+    while (kk < k1 && !((due >> kk) & 1ull)) ++kk;
+    if (kk >= k1) break;
+    const uint32_t kend = min(kk + stage, k1);
+    __syncthreads();  // the previous pass's readers are done with the bitmaps
+    for (uint32_t k = kk; k < kend; ++k) {
+      if (!((due >> k) & 1ull)) continue;
+      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, n.hist + ((t - k - 1) % n.R) * n.W + lo, nw);
+    }
+    __syncthreads();
+    while (cq < qend && k0 + cq / kPullSlices < kend) {  // the wave's columns of this pass
+      const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq / kPullSlices - kk) * bstride * 2 : bits0;
+      const uint32_t tl = cln & (kPullTB - 1), mine = cln >> kPullShift;  // target, the lane's own vectors
+      const uint32_t vcur = voff_of(cs0);
+      const uint32_t nv = cnv;
+      // the column's second batch goes out before the next column's facts
+      // are looked up (their scalar loads and branch stall the wave)
+      auto load_b = [&](uint32_t j0) {
+#pragma unroll
+        for (uint32_t u = 0; u < kB; ++u) {
+          const uint32_t j = j0 + kB + u;
+          qb[u] = vload(j < mine ? vcur : kPullOutOfRange, j);
+        }
+      };
+      load_b(0);
+      uint32_t nq = cq + kWaves, ns0 = 0, nnv = 0;
+      col_from(nq, ns0, nnv);
+      const uint32_t nln = lane_word(nq);
+      const uint32_t vnext = voff_of(ns0);
+      uint32_t cnt = 0;                  // kPullCountBits-bit fields per tag
+      uint32_t ba = lds_address(bits);   // LDS byte address of the running 32-bit word
+      auto sum_vec = [&](const vec4u& q) {
+        uint32_t a[2 * 4];
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+          ba += q[c] & kDeltaMask;
+          a[2 * c] = ba;
+          ba += (q[c] >> 16) & kDeltaMask;
+          a[2 * c + 1] = ba;
+        }
+        uint32_t w[2 * 4];
+#pragma unroll
+        for (uint32_t i = 0; i < 2 * 4; ++i) w[i] = lds_load_u32(a[i]);
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+          const uint32_t x = q[c];
+          cnt += __builtin_amdgcn_ubfe(w[2 * c], x >> 11, 1u) << ((x << 3) & 31u);
+          cnt += __builtin_amdgcn_ubfe(w[2 * c + 1], x >> 27, 1u) << (high_field_shift(x) & 31u);
+        }
+      };
+      // sums of the batch of vectors j0 .. j0 + kB - 1 held in qq
+      auto sum_batch = [&](const vec4u (&qq)[kB], uint32_t j0) {
+        if (j0 >= nv) return;  // uniform: past the slice's longest lane
+#pragma unroll
+        for (uint32_t u = 0; u < kB; ++u) {
+          if (j0 + u >= nv) break;
+          sum_vec(qq[u]);
+        }
+        if ((j0 / kB + 1) % (kFlushVectors / kB) == 0 || j0 + kB >= nv) {
+#pragma unroll
+          for (uint32_t i = 1; i <= kPullCountFields; ++i)
+            if (i < n.pull_wn) atomicAdd(&cnts[(i - 1) * kPullTB + tl], __builtin_amdgcn_ubfe(cnt, 8u * i, 8u));
+          cnt = 0;
+        }
+      };
+      // a column has at least one vector; batch A of vectors j0.., batch B
+      // of j0 + kB.., then A again from j0 + 2 kB or, after the column's
+      // last, the next column's first
+      uint32_t j0 = 0;
+      for (;;) {
+        sum_batch(qa, j0);
+        const bool here = j0 + 2 * kB < nv;
+#pragma unroll
+        for (uint32_t u = 0; u < kB; ++u) {
+          // one load either way: its offsets are selected
+          const uint32_t j = j0 + 2 * kB + u;
+          const uint32_t vo = here ? (j < mine ? vcur : kPullOutOfRange) : (u < nnv ? vnext : kPullOutOfRange);
+          qa[u] = vload(vo, here ? j : u);
+        }
+        sum_batch(qb, j0 + kB);
+        j0 += 2 * kB;
+        if (j0 >= nv) break;
+        load_b(j0);
+      }
+      cq = nq;
+      cs0 = ns0;
+      cnv = nnv;
+      cln = nln;
+    }
+    kk = kend;
+  }
+  __syncthreads();
+  // sums = counts times weights, per target
+  long long* out = n.acc_pull + (size_t)g * 2 * n.npad;
+  int32_t wv[kPullCountFields];
+#pragma unroll
+  for (uint32_t i = 0; i < kPullCountFields; ++i) wv[i] = i + 1 < n.pull_wn ? ld_const(&n.pull_wtab[i + 1]) : 0;
+  uint32_t events = 0;
+  for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
+    const uint32_t tgt = b * kPullTB + i;
+    if (tgt < n.N) {
+      long long e = 0, in = 0;
+#pragma unroll
+      for (uint32_t f = 0; f < kPullCountFields; ++f) {
+        const uint32_t c = cnts[f * kPullTB + i];
+        const long long p = (long long)c * wv[f];
+        events += c;
+        if (wv[f] > 0) e += p; else in += p;
+      }
+      st_agent(&out[tgt], e);
+      st_agent(&out[n.npad + tgt], in);  // two's complement: negative sums
+    }
+  }
+  // event count: one atomic per wave
+  unsigned long long ev = events;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) ev += __shfl_xor(ev, off);
+  if (lane == 0 && ev) atomicAdd(counter(n.counters, kCntEvents, blockIdx.x * 16 + wave), ev);
+}
+
 // The pull of work item (b, g) in the network's entry format.
 template <int BLK, bool AGENT, uint32_t KB, bool MULTI>
 __device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint32_t stage,
@@ -935,7 +1140,7 @@ __device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32
   if (n.pull_fmt == kPullWide)
     pull_body<BLK, kPullWide, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
   else if (n.pull_fmt == kPullCount)
-    pull_body<BLK, kPullCount, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
+    pull_count_body<BLK, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
   else
     pull_body<BLK, kPullTable, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
 }
@@ -1185,7 +1390,9 @@ hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, uint32_t n
 }
 
 size_t pull_dyn_bytes_for(uint32_t words, uint32_t fmt, uint32_t wn, uint32_t stage) {
-  return (size_t)stage * pull_bitmap_u64(words) * 8 + (size_t)2 * kPullTB * 8 + (fmt != kPullWide ? (size_t)32 * wn : 0);
+  const size_t bitmaps = (size_t)stage * pull_bitmap_u64(words) * 8;
+  if (fmt == kPullCount) return bitmaps + (size_t)kPullCountFields * kPullTB * 4;  // per-(tag, target) counts
+  return bitmaps + (size_t)2 * kPullTB * 8 + (fmt != kPullWide ? (size_t)32 * wn : 0);
 }
 
 size_t pull_dyn_bytes(const DevNet& n, uint32_t stage) {

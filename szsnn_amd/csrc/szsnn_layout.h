@@ -241,6 +241,9 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     }
   }
   off[nseg * kPullSlices] = (uint32_t)pos;
+  // count format: the device reads the entries through a buffer resource
+  // whose out-of-range offsets (2^31 and up) read zero vectors
+  if (fmt == kPullCount && (pos + V) * sizeof(uint16_t) > kPullMaxBufferBytes) return true;
   // entry j of a lane sits at lane_first + (j / V) * 64 * V + j % V
   std::vector<uint32_t> fill(nseg * kPullTB, 0);
   auto place = [&](uint64_t key) {
@@ -248,8 +251,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
     return (uint64_t)pos_of[key] + (uint64_t)(j / V) * 64 * V + j % V;
   };
   if (fmt == kPullCount) {
-    const uint16_t pad = count_entry(kPullSkipTag, 0, 0);  // skip by 0
-    std::vector<uint16_t> ps(pos + V, pad);  // and the null vector
+    std::vector<uint16_t> ps(pos + V, count_entry(kPullSkipTag, 0, 0));  // padding, null vector: zero skips
     std::fill(prev.begin(), prev.end(), 0u);
     for (uint64_t id = 0; id < T; ++id)
       for (uint64_t k = 0; k < D; ++k)
@@ -263,8 +265,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L) {
             ps[place(key)] = count_entry(kPullSkipTag, 0, kPullMaxDelta);
             gap -= kPullMaxDelta;
           }
-          const uint32_t tag = widx((int32_t)L.syn[q].y) - 1;
-          ps[place(key)] = count_entry(tag, rel & 31u, gap);
+          ps[place(key)] = count_entry(widx((int32_t)L.syn[q].y), rel & 31u, gap);
           prev[key] = rel >> 5;
         }
     L.pull_syn16.swap(ps);

@@ -222,9 +222,9 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
                   CHECK(count_bit(x) == 0, "skip entry with a bit");
                   continue;
                 }
-                CHECK(tag < kPullCountFields && tag + 1 < L.pull_wtab.size(), "count tag %u", tag);
+                CHECK(tag <= kPullCountFields && tag < L.pull_wtab.size(), "count tag %u", tag);
                 rel = run * 32 + count_bit(x);
-                w = L.pull_wtab[tag + 1];
+                w = L.pull_wtab[tag];
               } else if (fmt == kPullTable) {
                 const uint32_t x = L.pull_syn32[e];
                 rel = x & kPullRelMask;
@@ -416,7 +416,7 @@ int main() {
         const uint16_t x = L.pull_syn16[(size_t)(L.pull_off[sl] / 8 + 64 * j + lane) * 8 + c];
         run += count_delta(x);
         ++seen;
-        if (count_tag(x) != kPullSkipTag) got.emplace_back(run * 32 + count_bit(x), L.pull_wtab[count_tag(x) + 1]);
+        if (count_tag(x) != kPullSkipTag) got.emplace_back(run * 32 + count_bit(x), L.pull_wtab[count_tag(x)]);
       }
     CHECK(got.size() == n.src.size(), "decoded %zu of %zu sources", got.size(), n.src.size());
     for (size_t i = 0; i < got.size(); ++i) {
