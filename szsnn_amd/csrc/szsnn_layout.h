@@ -19,7 +19,8 @@
 //     contiguous; each lane also records how many vectors hold its own
 //     entries (the rest of its column is padding and is not read: loads past
 //     a lane's own vectors read one null vector of padding entries stored
-//     after the last slice).  Entries
+//     after the last slice, or, in the count format, an out-of-range offset
+//     of a buffer load, which reads zeros).  Entries
 //     name the source relative to the block's first staged id, in one of
 //     three formats (kPullWide, kPullCount, kPullTable in szsnn_constants.h):
 //     8-byte (offset, weight); with at most kPullCountFields distinct weights

@@ -67,6 +67,11 @@ class Context:
     def synchronize(self):
         check(self.lib.szsnn_synchronize(self.handle), self.handle)
 
+    @property
+    def stream(self):
+        """The context's hipStream_t, as an integer (for torch.cuda.ExternalStream)."""
+        return int(self.lib.szsnn_stream(self.handle) or 0)
+
     def close(self):
         """Destroy the stream; networks still open on this context are closed first."""
         if self.handle:
