@@ -527,9 +527,9 @@ def headline(args, torch, ctx, net, sim):
     pull_entry_bytes = int(sim.info.pull_entry_bytes)
     flush = CacheFlush(torch, args.flush_mib, ctx.stream)
 
-    def volley(fraction, cold=False):
+    def volley(fraction, cold=False, ids=None):
         sim.set_gather_policy(fraction)
-        sim.set_volley(net.n_neurons)
+        sim.set_volley(net.n_neurons if ids is None else ids)
         sim.run(net.max_delay + 2)  # fill all delay slots
         if not args.no_autotune:
             log(f"[bench] autotuned for the volley (policy {fraction}): {sim.autotune(10)}")
@@ -562,6 +562,25 @@ def headline(args, torch, ctx, net, sim):
     log(f"[bench] volley pull, warm (policy {default_policy}): {v_warm}")
     v_cold = volley(default_policy, cold=True)
     log(f"[bench] volley pull, cold: {v_cold}")
+    # the volley benchmark at 1 %, 10 % and 100 % of the neurons forced every
+    # step (ids 0 .. k - 1; the network's own spikes come on top), warm, one
+    # launch of each kernel at a time: under the default policy (a step is
+    # pulled when its predicted spikes, the previous step's, reach the
+    # policy's fraction of the ids with synapses, else pushed) and with every
+    # step pulled (policy 0)
+    fractions = []
+    for f in (0.01, 0.1, 1.0):
+        k = int(round(f * net.n_neurons))
+        row = {"fraction": f, "forced_ids": k}
+        for name, pol in (("default_policy", default_policy), ("pull_every_step", 0.0)):
+            v = volley(pol, ids=k)
+            step_us = v["gather_us"] + v["update_us"]
+            row[name] = {"events_per_step": v["events_per_launch"], "rows_per_step": v["rows_per_launch"],
+                         "gather_us": v["gather_us"], "update_us": v["update_us"], "us_per_step": step_us,
+                         "events_per_s": v["events_per_launch"] / (step_us * 1e-6),
+                         "alg_gbs_per_step": v["alg_bytes_per_launch"] / (step_us * 1e-6) / 1e9}
+        fractions.append(row)
+        log(f"[bench] volley {f:.0%}: {row}")
     sim.set_gather_policy(default_policy)
     flush.close()
     sim.close()
@@ -654,6 +673,7 @@ def headline(args, torch, ctx, net, sim):
         "volley_push_only": v_push,
         "volley_pull_warm": v_warm,
         "volley_pull_cold": v_cold,
+        "volley_fractions": fractions,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

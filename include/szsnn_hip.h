@@ -108,6 +108,8 @@ typedef struct szsnn_counters {
   uint64_t drive_events;  /* binomial drive events (not synaptic) */
   uint64_t overflow;      /* >= 2^32 after a persistent-loop barrier timed out; else 0 */
   uint64_t rows;          /* (source, delay) rows delivered */
+  uint64_t deferred;      /* steps deferred to the pull gather (predicted dense, or late) */
+  uint64_t late_deferred; /* of them, deferred late: posted as heavy pushes, then pulled */
 } szsnn_counters;
 
 /* ---- context / device runtime (replaces Context.hpp:77-303) ---- */

@@ -89,6 +89,8 @@ class Counters(ctypes.Structure):
         ("drive_events", ctypes.c_uint64),
         ("overflow", ctypes.c_uint64),
         ("rows", ctypes.c_uint64),
+        ("deferred", ctypes.c_uint64),
+        ("late_deferred", ctypes.c_uint64),
     ]
 
 

@@ -67,7 +67,7 @@ struct DevNet {
   const uint32_t* syn_off;  // [T + 1] synapses of id at syn[syn_off[id] .. syn_off[id + 1])
   const uint2* syn;         // [S] (target | k << kSynDelayShift, fixed-point weight as int32 bits)
   const uint64_t* delay_bits;  // [T] bit k set iff id has a synapse of delay k
-  uint64_t* dstamp;         // [R] s + 1 in slot s % R when step s was deferred
+  uint64_t* dstamp;         // [R] s + 1 in slot s % R when step s was deferred (| kStampLate: deferred late)
   unsigned long long* ring_cnt;  // [R][2][kRingStripes] events and rows due at the slot's step
   unsigned long long* spk_ring;  // [4][kRingStripes] spikes of step s in slot s % 4
   uint32_t defer_spikes;    // defer step t when its predicted spikes >= this (0xffffffff: never)
@@ -90,6 +90,13 @@ struct DevNet {
   uint32_t heavy_min;       // 0: every block pushes its own ids (default kHeavyPushMin)
   uint32_t* heavy_cnt;      // [2]
   uint2* heavy_list;        // [2][T] (first synapse, synapses)
+  // Late deferral: when the ids that step s posted for heavy pushes reach the
+  // deferral threshold, the gather launch of s + 1 pulls step s instead of
+  // pushing them (a burst after quiet steps, which the prediction missed).
+  // hvy slot s % R holds the spike words of the blocks that posted at s (zero
+  // for the others, whose spikes were pushed), and the pull of a step stamped
+  // late stages those words instead of the full history.
+  uint64_t* hvy;            // [R][W] or null (no heavy pushes)
   const uint64_t* forced_bits;   // [W] forced ids this step, or null
   uint64_t* hist;           // [R][W] spike bitmaps of the last R steps (slot t % R)
   // pull layout (dense gather), a sliced ELLPACK: target blocks of kPullTB
@@ -122,7 +129,12 @@ struct DevNet {
 __host__ __device__ inline uint32_t pull_bitmap_u64(uint32_t words) { return (words + 2) & ~1u; }
 
 // Counter slots.
-enum : int { kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntRows = 4, kCntSlots = 5 };
+enum : int {
+  kCntEvents = 0, kCntSpikes = 1, kCntDrive = 2, kCntOverflow = 3, kCntRows = 4,
+  kCntDeferred = 5,  // steps deferred to the pull (predicted or late)
+  kCntLate = 6,      // of them, deferred late
+  kCntSlots = 7
+};
 
 // Load of data that no kernel writes (the network's layout tables) at a
 // wave-uniform address, through the scalar cache (constant address space), so
@@ -200,6 +212,11 @@ __host__ __device__ inline unsigned long long* spk_counter(unsigned long long* b
 // ids forced in this step.  Inside a persistent launch (no host-forced ids,
 // the volley forced in every step) it is the previous step's spikes from
 // the launch's second step on, which the loop uses directly.
+// Deferral stamps: slot s % R holds s + 1 when step s was deferred, with
+// kStampLate set when the gather launch of s + 1 deferred it (late deferral).
+constexpr uint64_t kStampLate = 1ull << 63;
+__host__ __device__ inline uint64_t stamp_step(uint64_t v) { return v & ~kStampLate; }
+
 __device__ inline bool defer_step(const DevNet& n, uint64_t t, unsigned long long spk_prev) {
   if (!n.pull_off) return false;
   const unsigned long long fprev =

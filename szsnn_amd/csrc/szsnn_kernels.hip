@@ -183,15 +183,21 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_pref, ui
 
 // Bit k set iff step t - k - 1 was deferred, i.e. its synapses of delay k
 // are pulled at t.  Every lane of the calling wave gets the mask.
+// `late` gets bit k set when that step was deferred late (its pull stages
+// the posted spike words, hvy, instead of the history).
 template <bool AGENT>
-__device__ inline uint64_t due_mask(const DevNet& n, uint64_t t) {
+__device__ inline uint64_t due_mask(const DevNet& n, uint64_t t, uint64_t& late) {
+  late = 0;
   if (!n.pull_off) return 0;
   const uint32_t k = __lane_id();
-  bool due = false;
+  bool due = false, lt = false;
   if (k < n.D && t >= (uint64_t)k + 1) {
     const uint64_t s = t - k - 1;
-    due = ld_step<AGENT>(&n.dstamp[s % n.R]) == s + 1;
+    const uint64_t v = ld_step<AGENT>(&n.dstamp[s % n.R]);
+    due = stamp_step(v) == s + 1;
+    lt = due && (v & kStampLate);
   }
+  late = __ballot(lt);
   return __ballot(due);
 }
 
@@ -306,6 +312,7 @@ struct UpdateSmem {
 // step's spike total); a single-step launch reads them with step_facts.
 struct StepFacts {
   uint64_t due;
+  uint64_t late;  // the due steps deferred late (due_mask)
   unsigned long long spk_prev;
   bool defer;  // the step's deferral decision (carried: taken by the loop)
 };
@@ -328,7 +335,7 @@ __device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bo
   const uint64_t stamp = n.pull_off ? ld_step<AGENT>(&n.dstamp[(t - k - 1) % n.R]) : 0ull;
   const unsigned long long spk = ld_step<AGENT>(spk_counter(n.spk_ring, t - 1, lane));
   fold = do_fold ? ld_step<AGENT>(ring_counter(n.ring_cnt, slot, lane)) : 0ull;
-  const bool stamp_ok = n.pull_off && lane < n.D && t >= (uint64_t)lane + 1 && stamp == t - lane;
+  const bool stamp_ok = n.pull_off && lane < n.D && t >= (uint64_t)lane + 1 && stamp_step(stamp) == t - lane;
   const uint64_t due = __ballot(stamp_ok);
   if (lane == 0) sm.due = due;
   if (t > 0 && spk) atomicAdd(&sm.spk_prev, spk);
@@ -564,6 +571,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
   if (ring_i) st_agent(&ring[n.npad + id], 0ll);
   __syncthreads();
   const uint32_t nlist = sm.nlist;
+  bool posted = false;  // the block's spiking ids went to the heavy list (block-uniform)
   if (nlist) {
     // push: the synapses of the block's spiking ids, each adding its weight
     // into the slot of step t + k + 1 (scatter_kernel.cl:194-355 and
@@ -584,6 +592,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       if (heavy_block(n, total, AGENT)) {
         if (threadIdx.x < 64) post_heavy(n, t, threadIdx.x < nlist, sm.list[threadIdx.x < nlist ? threadIdx.x : 0]);
         total = 0;
+        posted = true;
       }
       for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
         uint2 sw[kPushBatch];
@@ -610,6 +619,7 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       if (heavy_block(n, total, AGENT)) {
         post_heavy(n, t, has, make_uint2(r0, r1 - r0));
         total = 0;
+        posted = true;
       }
       for (uint32_t e0 = 0; e0 < total; e0 += BLK * kPushBatch) {
         uint2 sw[kPushBatch];
@@ -631,6 +641,9 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     }
     __syncthreads();
   }
+  // the spike words of a posting block for late deferral (zero for a block
+  // that pushed or had no spikes with synapses)
+  if (!AGENT && n.hvy && n.heavy_min && lane == 0 && word < n.W) n.hvy[(size_t)slot * n.W + word] = posted ? mask : 0ull;
   // rows and events per delay: one atomic per (block, delay) into the
   // delivery ring, both counts in one word
   if (threadIdx.x < n.D) {
@@ -643,7 +656,10 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
   const unsigned long long spikes = sm.spikes, drive = sm.drive;
   if (threadIdx.x == 0) {
     // the deferral stamp, by one block (every block takes the same decision)
-    if (defer && blockIdx.x == 0) st_agent(&n.dstamp[slot], t + 1);
+    if (defer && folder) {
+      st_agent(&n.dstamp[slot], t + 1);
+      atomicAdd(counter(n.counters, kCntDeferred, 0), 1ull);
+    }
     // the heavy list step t + 1 posts into was drained by this step's gather
     if (!AGENT && n.heavy_min && blockIdx.x == 0) st_agent(&n.heavy_cnt[(t + 1) & 1], 0u);
     // CARRY: the caller sums the block's spikes and drive events over the
@@ -667,7 +683,7 @@ template <int MODEL, int BLK, bool AGENT>
 __device__ inline void update_block(const DevNet& n, uint64_t t, uint32_t id, UpdateSmem<BLK>& sm, bool first) {
   LaneState<MODEL> ls;
   unsigned long long drive = 0;
-  update_step<MODEL, BLK, false, AGENT>(n, t, id, sm, ls, StepFacts{0, 0, false}, drive, first);
+  update_step<MODEL, BLK, false, AGENT>(n, t, id, sm, ls, StepFacts{0, 0, 0, false}, drive, first);
 }
 
 // update(t): block b serves the NPL id blocks b * NPL .. b * NPL + NPL - 1 of
@@ -743,8 +759,8 @@ __device__ inline void stage_words(uint64_t* dst, const uint64_t* src, uint32_t 
 // index pull_words of each is zero, for the padding entries) | 2*kPullTB
 // int64 sums | narrow formats: 2 * pull_wn (E, I) f64 pairs.
 template <int BLK, int FMT, bool AGENT, uint32_t KB, bool MULTI>
-__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint32_t stage,
-                                 uint64_t* dyn) {
+__device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t late,
+                                 uint32_t stage, uint64_t* dyn) {
   static_assert(FMT == kPullWide || FMT == kPullTable, "the count format has pull_count_body");
   if (!MULTI) stage = 1;
   constexpr bool kNarrow = FMT != kPullWide;
@@ -826,7 +842,8 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     __syncthreads();  // the previous pass's readers are done with the bitmaps
     for (uint32_t k = kk; k < kend; ++k) {
       if (!((due >> k) & 1ull)) continue;
-      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, n.hist + ((t - k - 1) % n.R) * n.W + lo, nw);
+      const uint64_t* ring = ((late >> k) & 1ull) ? n.hvy : n.hist;  // a late-deferred step: its posted spikes
+      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring + ((t - k - 1) % n.R) * n.W + lo, nw);
     }
     __syncthreads();
     while (cq < qend && k0 + cq / kPullSlices < kend) {  // the wave's columns of this pass
@@ -948,7 +965,7 @@ __device__ inline uint32_t high_field_shift(uint32_t x) {
 // kPullCountFields * kPullTB u32 counts.
 template <int BLK, bool AGENT, uint32_t KB, bool MULTI>
 __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due,
-                                       uint32_t stage, uint64_t* dyn) {
+                                       uint64_t late, uint32_t stage, uint64_t* dyn) {
   if (!MULTI) stage = 1;
   constexpr uint32_t kV = 8;   // entries per 16-byte vector
   constexpr uint32_t kB = KB;  // vectors per lane and batch
@@ -1020,7 +1037,8 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
     __syncthreads();  // the previous pass's readers are done with the bitmaps
     for (uint32_t k = kk; k < kend; ++k) {
       if (!((due >> k) & 1ull)) continue;
-      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, n.hist + ((t - k - 1) % n.R) * n.W + lo, nw);
+      const uint64_t* ring = ((late >> k) & 1ull) ? n.hvy : n.hist;  // a late-deferred step: its posted spikes
+      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring + ((t - k - 1) % n.R) * n.W + lo, nw);
     }
     __syncthreads();
     while (cq < qend && k0 + cq / kPullSlices < kend) {  // the wave's columns of this pass
@@ -1135,14 +1153,14 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
 
 // The pull of work item (b, g) in the network's entry format.
 template <int BLK, bool AGENT, uint32_t KB, bool MULTI>
-__device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint32_t stage,
-                                 uint64_t* dyn) {
+__device__ inline void pull_item(const DevNet& n, uint64_t t, uint32_t b, uint32_t g, uint64_t due, uint64_t late,
+                                 uint32_t stage, uint64_t* dyn) {
   if (n.pull_fmt == kPullWide)
-    pull_body<BLK, kPullWide, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
+    pull_body<BLK, kPullWide, AGENT, KB, MULTI>(n, t, b, g, due, late, stage, dyn);
   else if (n.pull_fmt == kPullCount)
-    pull_count_body<BLK, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
+    pull_count_body<BLK, AGENT, KB, MULTI>(n, t, b, g, due, late, stage, dyn);
   else
-    pull_body<BLK, kPullTable, AGENT, KB, MULTI>(n, t, b, g, due, stage, dyn);
+    pull_body<BLK, kPullTable, AGENT, KB, MULTI>(n, t, b, g, due, late, stage, dyn);
 }
 
 // gather(t): the pull of the deferred steps whose synapses are due at t, one
@@ -1156,13 +1174,28 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
   extern __shared__ __align__(16) uint64_t dyn[];
   __shared__ uint32_t hev[64];
   const uint64_t t = *n.t_base + j;
-  // the heavy pushes of step t - 1 (their delay-0 synapses are due at t)
+  // the heavy pushes of step t - 1 (their delay-0 synapses are due at t), or,
+  // when its posted ids reach the deferral threshold, late deferral: step
+  // t - 1 is pulled instead, from its posted spike words (every block takes
+  // the same decision from the same count; block 0 stamps the step for the
+  // launches after this one)
   const uint32_t hc = n.heavy_min && t > 0 ? n.heavy_cnt[(t - 1) & 1] : 0u;
-  if (hc) heavy_push<BLK>(n, t - 1, hc, hev);
-  const uint64_t due = due_mask<false>(n, t);
+  const bool late0 = hc && n.hvy && hc >= n.defer_spikes;
+  if (hc && !late0) heavy_push<BLK>(n, t - 1, hc, hev);
+  uint64_t late;
+  uint64_t due = due_mask<false>(n, t, late);
+  if (late0) {
+    due |= 1ull;
+    late |= 1ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st_agent(&n.dstamp[(t - 1) % n.R], t | kStampLate);
+      atomicAdd(counter(n.counters, kCntDeferred, 0), 1ull);
+      atomicAdd(counter(n.counters, kCntLate, 0), 1ull);
+    }
+  }
   if (!due) return;
   for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-    pull_item<BLK, false, KB, WPE == 4>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, n.pull_stage, dyn);
+    pull_item<BLK, false, KB, WPE == 4>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, late, n.pull_stage, dyn);
     __syncthreads();  // LDS reuse by the next work item
   }
 }
@@ -1252,14 +1285,21 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       bar_sh[0] = (uint32_t)ld_agent(reinterpret_cast<unsigned long long*>(n.barrier + (kBarLeaves + 1) * kBarStride));
       bar_sh[1] = 0;
       f0.spk_prev = 0;
+      f0.due = 0;
+      f0.late = 0;
     }
     if (carry) {
       // the facts of step t0, as step_facts reads them
       const uint32_t k = lane < n.D ? lane : 0u;
       const uint64_t stamp = n.pull_off ? ld_agent(&n.dstamp[(t0 - k - 1) % n.R]) : 0ull;
       const unsigned long long spk = ld_agent(spk_counter(n.spk_ring, t0 - 1, lane));
-      const uint64_t due = __ballot(n.pull_off && lane < n.D && t0 >= (uint64_t)lane + 1 && stamp == t0 - lane);
-      if (lane == 0) f0.due = due;
+      const bool ok = n.pull_off && lane < n.D && t0 >= (uint64_t)lane + 1 && stamp_step(stamp) == t0 - lane;
+      const uint64_t due = __ballot(ok);
+      const uint64_t late = __ballot(ok && (stamp & kStampLate));
+      if (lane == 0) {
+        f0.due = due;
+        f0.late = late;
+      }
       if (t0 > 0 && spk) atomicAdd(&f0.spk_prev, spk);
     }
   }
@@ -1289,10 +1329,11 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       fold_v = 0;
     }
     unsigned long long sum = 0;
-    const uint64_t due = carry ? f.due : due_mask<true>(n, t);
+    uint64_t late = f.late;
+    const uint64_t due = carry ? f.due : due_mask<true>(n, t, late);
     if (due) {
       for (uint32_t bg = blockIdx.x; bg < n.pull_nb * n.pull_groups; bg += gridDim.x) {
-        pull_item<BLK, true, 2, false>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, 1u, dyn);
+        pull_item<BLK, true, 2, false>(n, t, bg / n.pull_groups, bg % n.pull_groups, due, late, 1u, dyn);
         __syncthreads();
       }
       if (!grid_barrier(n.barrier, bar_sh, err, 0, &sum)) return;
@@ -1306,6 +1347,7 @@ __global__ void __launch_bounds__(BLK) persistent_kernel(DevNet n, uint64_t nste
       if (folds) fold_v = atomicExch(ring_counter(n.ring_cnt, slot, q), 0ull);
       if (!grid_barrier(n.barrier, bar_sh, err, threadIdx.x == 0 ? spikes : 0ull, &sum)) return;
       f.due = ((f.due << 1) | (f.defer ? 1ull : 0ull)) & dmask;
+      f.late = (f.late << 1) & dmask;  // the loop's own deferrals stage the history
       f.spk_prev = sum;
       f.defer = n.pull_off && sum >= n.defer_spikes;
     } else {

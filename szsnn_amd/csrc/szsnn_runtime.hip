@@ -540,11 +540,13 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint32_t* d_pull_lane = nullptr;
   uint32_t* heavy_cnt = nullptr;
   uint2* heavy_list = nullptr;
+  uint64_t* hvy = nullptr;
   if (rc == SZSNN_OK) guard(dev_alloc(net, &hist, (size_t)n.R * n.W));
   // heavy pushes are drained by the gather launch of the next step, so they
   // exist with a pull layout only
   if (rc == SZSNN_OK && !pull_off.empty())
-    guard(dev_alloc(net, &heavy_cnt, 2)) && guard(dev_alloc(net, &heavy_list, (size_t)2 * T));
+    guard(dev_alloc(net, &heavy_cnt, 2)) && guard(dev_alloc(net, &heavy_list, (size_t)2 * T)) &&
+        guard(dev_alloc(net, &hvy, (size_t)n.R * n.W));
   if (rc == SZSNN_OK && !pull_off.empty())
     guard(dev_alloc(net, &d_pull_off, pull_off.size())) &&
         (L.pull_fmt == kPullWide    ? guard(dev_alloc(net, &d_pull_syn, pull_syn_size))
@@ -609,6 +611,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
     if (he == hipSuccess) he = hipMemsetAsync(net->forced_bits, 0, (size_t)n.W * 8, s);
     if (he == hipSuccess) he = hipMemsetAsync(hist, 0, (size_t)n.R * n.W * 8, s);
     if (he == hipSuccess && heavy_cnt) he = hipMemsetAsync(heavy_cnt, 0, 2 * 4, s);
+    if (he == hipSuccess && hvy) he = hipMemsetAsync(hvy, 0, (size_t)n.R * n.W * 8, s);
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     if (he != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_net_upload: ") + hipGetErrorString(he));
   }
@@ -662,6 +665,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.heavy_min = heavy_cnt ? kHeavyPushMin : 0;
   n.heavy_cnt = heavy_cnt;
   n.heavy_list = heavy_list;
+  n.hvy = hvy;
   net->active_ids = active;
   net->dense_fraction = default_dense_fraction(d_pull_off ? L.pull_fmt : kPullWide);
   n.defer_spikes = defer_threshold(net, net->dense_fraction);
@@ -953,6 +957,8 @@ int szsnn_get_counters(szsnn_net* net, szsnn_counters* out) {
   out->drive_events = c[kCntDrive];
   out->overflow = c[kCntOverflow];
   out->rows = c[kCntRows];
+  out->deferred = c[kCntDeferred];
+  out->late_deferred = c[kCntLate];
   return SZSNN_OK;
 }
 
@@ -1203,6 +1209,7 @@ std::vector<Region> state_regions(szsnn_net* net) {
   if (n.heavy_cnt) {
     r.push_back({n.heavy_cnt, 2 * 4});
     r.push_back({n.heavy_list, (size_t)2 * n.T * 8});
+    r.push_back({n.hvy, (size_t)n.R * n.W * 8});
   }
   return r;
 }

@@ -228,7 +228,8 @@ class Simulation:
         c = _lib.Counters()
         check(self.lib.szsnn_get_counters(self.handle, ctypes.byref(c)), self.ctx.handle)
         return {"cycle": c.cycle, "events": c.events, "spikes": c.spikes,
-                "drive_events": c.drive_events, "overflow": c.overflow, "rows": c.rows}
+                "drive_events": c.drive_events, "overflow": c.overflow, "rows": c.rows,
+                "deferred": c.deferred, "late_deferred": c.late_deferred}
 
     def reset_counters(self):
         check(self.lib.szsnn_reset_counters(self.handle), self.ctx.handle)

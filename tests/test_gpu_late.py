@@ -1,0 +1,91 @@
+"""Late deferral: a step the prediction pushed, whose spiking ids were posted
+as heavy pushes in numbers that reach the deferral threshold, is pulled by
+the next gather launch instead (from the posted blocks' spike words).  The
+sums are the same integers added in another order, so rasters, state and
+event counts stay bit-identical to the FP32 oracle and to pushing.
+
+The case: the 10k-neuron IF_cond_exp network after a volley of every neuron,
+then under a volley of 100 ids; it answers with a burst of about 3100 spikes
+every 22 steps, right after a step with the 100 forced spikes alone, so the
+prediction (the previous step's spikes) pushes the burst step."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SETTLE, FULL, SMALL = 500, 60, 80
+STEPS = SETTLE + FULL + SMALL
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from szsnn_amd.simulation import Context
+    return Context(0)
+
+
+@pytest.fixture(scope="module")
+def net():
+    from szsnn_amd.configs import if_cond_exp_100k
+    return if_cond_exp_100k(n_exc=8000, n_inh=2000, k_exc=400, k_inh=100)
+
+
+@pytest.fixture(scope="module")
+def oracle_run(net):
+    from oracle.pyoracle import Oracle
+    o = Oracle(net, 32)
+    N = net.n_neurons
+    st = np.concatenate([np.repeat(np.arange(SETTLE, SETTLE + FULL), N),
+                         np.repeat(np.arange(SETTLE + FULL, STEPS), 100)])
+    ids = np.concatenate([np.tile(np.arange(N), FULL), np.tile(np.arange(100), SMALL)])
+    r, _ = o.run(STEPS, forced=(st, ids), threads=8)
+    return r, o.vars().astype(np.float32), o.events, o.drive_events
+
+
+def _run(ctx, net, policy, heavy=None):
+    from szsnn_amd.simulation import Simulation
+    s = Simulation(ctx, net)
+    if heavy is not None:
+        s.set_heavy_push(heavy)
+    s.set_gather_policy(policy)
+    s.set_recording(0, STEPS)
+    s.run(SETTLE)
+    s.set_volley(net.n_neurons)
+    s.run(FULL)
+    s.set_volley(100)
+    s.run(SMALL)
+    out = (s.raster(), s.get_vars(), s.counters())
+    s.close()
+    return out
+
+
+def _check(got, ref):
+    r, v, c = got
+    ro, vo, events, drive = ref
+    bad = np.flatnonzero((r != ro).any(axis=1))
+    assert bad.size == 0, f"raster differs first at step {bad[0]}"
+    np.testing.assert_array_equal(v, vo)
+    assert c["events"] == events > 0
+    assert c["drive_events"] == drive
+
+
+def test_burst_after_quiet_steps_is_deferred_late(ctx, net, oracle_run):
+    """Default policy and heavy threshold: every burst of the 100-id volley is
+    deferred late (3 bursts in its 80 steps), and the run equals the oracle."""
+    print("case: 10k IF_cond_exp, full volley then 100 ids, default policy", flush=True)
+    got = _run(ctx, net, 0.012)
+    c = got[2]
+    assert c["late_deferred"] >= 3, c
+    assert c["deferred"] > c["late_deferred"], c
+    _check(got, oracle_run)
+
+
+def test_late_deferral_equals_push_only(ctx, net, oracle_run):
+    """The same run pushing every step (no deferral, so no late one) and with
+    heavy pushes switched off: both equal the oracle too."""
+    print("case: 10k IF_cond_exp, full volley then 100 ids, push only / no heavy pushes", flush=True)
+    push = _run(ctx, net, 2.0)
+    assert push[2]["deferred"] == 0 and push[2]["late_deferred"] == 0
+    _check(push, oracle_run)
+    noheavy = _run(ctx, net, 0.012, heavy=0)
+    assert noheavy[2]["late_deferred"] == 0
+    _check(noheavy, oracle_run)
