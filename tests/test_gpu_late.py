@@ -41,18 +41,28 @@ def oracle_run(net):
     return r, o.vars().astype(np.float32), o.events, o.drive_events
 
 
-def _run(ctx, net, policy, heavy=None):
+def _run(ctx, net, policy=None, heavy=None, persistent_after=None):
+    """policy None: the layout's default; persistent_after: steps of the
+    100-id volley run as single graph steps before the rest runs in the
+    persistent mode."""
+    from szsnn_amd._lib import MODE_PERSISTENT
     from szsnn_amd.simulation import Simulation
     s = Simulation(ctx, net)
     if heavy is not None:
         s.set_heavy_push(heavy)
-    s.set_gather_policy(policy)
+    if policy is not None:
+        s.set_gather_policy(policy)
     s.set_recording(0, STEPS)
     s.run(SETTLE)
     s.set_volley(net.n_neurons)
     s.run(FULL)
     s.set_volley(100)
-    s.run(SMALL)
+    if persistent_after is None:
+        s.run(SMALL)
+    else:
+        s.run(persistent_after)
+        s.set_mode(MODE_PERSISTENT, 256, 0)
+        s.run(SMALL - persistent_after)
     out = (s.raster(), s.get_vars(), s.counters())
     s.close()
     return out
@@ -68,14 +78,27 @@ def _check(got, ref):
     assert c["drive_events"] == drive
 
 
-def test_burst_after_quiet_steps_is_deferred_late(ctx, net, oracle_run):
-    """Default policy and heavy threshold: every burst of the 100-id volley is
-    deferred late (3 bursts in its 80 steps), and the run equals the oracle."""
-    print("case: 10k IF_cond_exp, full volley then 100 ids, default policy", flush=True)
-    got = _run(ctx, net, 0.012)
+@pytest.mark.parametrize("layout", [0, 1])
+def test_burst_after_quiet_steps_is_deferred_late(ctx, net, oracle_run, layout):
+    """Default policy and heavy threshold, 2-byte count entries (layout 0) or
+    8-byte entries (layout 1): every burst of the 100-id volley is deferred
+    late (3 bursts in its 80 steps), and the run equals the oracle."""
+    import dataclasses
+    print(f"case: 10k IF_cond_exp, full volley then 100 ids, default policy, layout {layout}", flush=True)
+    got = _run(ctx, dataclasses.replace(net, pull_layout=layout))
     c = got[2]
     assert c["late_deferred"] >= 3, c
     assert c["deferred"] > c["late_deferred"], c
+    _check(got, oracle_run)
+
+
+def test_persistent_launch_pulls_a_late_deferred_step(ctx, net, oracle_run):
+    """The first burst (step 21 of the 100-id volley) is deferred late by the
+    gather launch of step 22; from step 23 the persistent loop runs, and its
+    pulls of that step's remaining delays stage the posted spike words."""
+    print("case: 10k IF_cond_exp, late deferral, then the persistent mode", flush=True)
+    got = _run(ctx, net, persistent_after=23)
+    assert got[2]["late_deferred"] >= 1, got[2]
     _check(got, oracle_run)
 
 
