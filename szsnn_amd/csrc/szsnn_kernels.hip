@@ -1175,12 +1175,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
   __shared__ uint32_t hev[64];
   const uint64_t t = *n.t_base + j;
   // the heavy pushes of step t - 1 (their delay-0 synapses are due at t), or,
-  // when its posted ids reach the deferral threshold, late deferral: step
-  // t - 1 is pulled instead, from its posted spike words (every block takes
-  // the same decision from the same count; block 0 stamps the step for the
-  // launches after this one)
+  // when its posted ids reach half the deferral threshold, late deferral:
+  // step t - 1 is pulled instead, from its posted spike words (every block
+  // takes the same decision from the same count; block 0 stamps the step for
+  // the launches after this one)
   const uint32_t hc = n.heavy_min && t > 0 ? n.heavy_cnt[(t - 1) & 1] : 0u;
-  const bool late0 = hc && n.hvy && hc >= n.defer_spikes;
+  const bool late0 = hc && n.hvy && defer_late(hc, n.defer_spikes);
   if (hc && !late0) heavy_push<BLK>(n, t - 1, hc, hev);
   uint64_t late;
   uint64_t due = due_mask<false>(n, t, late);
