@@ -15,7 +15,10 @@ Prints one JSON line (the driver's contract) with:
   roofline       the pull gather kernel at the volley load (all neurons forced
                  every step: every synapse delivered each step), each launch
                  preceded by a 2 GiB read sweep that evicts L2 and the
-                 Infinity Cache (cold), HIP-event timed on the launch stream;
+                 Infinity Cache (cold), timed with HIP events that the
+                 kernel's own dispatch records on the launch stream
+                 (hipExtLaunchKernelGGL: the kernel's duration, as rocprofv3
+                 measures it);
                  `traffic` = HBM bytes per launch from the rocprofv3
                  FETCH_SIZE / WRITE_SIZE passes of the same cold workload
                  recorded in profiles/, `achieved` = traffic / launch time,
@@ -161,7 +164,7 @@ class CacheFlush:
 def time_cold(sim, flush, n):
     """n single steps, each after a cache-evicting sweep queued on the
     library's stream: summed gather and update kernel times (ms, HIP events
-    on that stream)."""
+    recorded by the kernels' dispatches on that stream)."""
     g = u = 0.0
     for _ in range(n):
         flush()
@@ -635,7 +638,9 @@ def headline(args, torch, ctx, net, sim):
         "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
         "kernel": "gather_kernel pull, 2-byte count-format entries (volley: all 100k neurons forced each step), "
                   "cold: a 2 GiB read sweep before each launch",
-        "avg_launch_us": g_cold_s * 1e6, "traffic_detail": t_cold,
+        "avg_launch_us": g_cold_s * 1e6,
+        "timing": "HIP events recorded by each launch's dispatch on the library's stream (hipExtLaunchKernelGGL)",
+        "traffic_detail": t_cold,
         "alg_bytes_per_launch": v_cold["alg_bytes_per_launch"], "alg_gbs": v_cold["alg_gbs"],
         "alg_unit": "8 B per delivered synapse + 8 B per delivered (source, delay) row",
         "pull_entry_bytes": pull_entry_bytes, "copy_gbs": copy_gbs,

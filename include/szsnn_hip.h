@@ -172,9 +172,10 @@ int szsnn_set_vars(szsnn_net* net, const float* host_in);  /* [n_vars][N] */
 int szsnn_get_counters(szsnn_net* net, szsnn_counters* out);
 int szsnn_reset_counters(szsnn_net* net);
 
-/* Per-kernel timing with HIP events on the context stream: runs n steps
- * launched one kernel at a time (no graph) and returns the summed durations of
- * the gather and update kernels in milliseconds. */
+/* Per-kernel timing: runs n steps launched one kernel at a time (no graph)
+ * on the context stream and returns the summed durations of the gather and
+ * update kernels in milliseconds, from HIP events that each kernel's dispatch
+ * records at its start and end (hipExtLaunchKernelGGL). */
 int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, double* update_ms);
 
 /* One kernel of the current step, launched alone and waited for: the

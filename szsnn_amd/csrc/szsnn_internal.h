@@ -90,9 +90,10 @@ struct DevNet {
   uint32_t heavy_min;       // 0: every block pushes its own ids (default kHeavyPushMin)
   uint32_t* heavy_cnt;      // [2]
   uint2* heavy_list;        // [2][T] (first synapse, synapses)
-  // Late deferral: when the ids that step s posted for heavy pushes reach the
-  // deferral threshold, the gather launch of s + 1 pulls step s instead of
-  // pushing them (a burst after quiet steps, which the prediction missed).
+  // Late deferral: when the ids that step s posted for heavy pushes reach
+  // half the deferral threshold (defer_late), the gather launch of s + 1
+  // pulls step s instead of pushing them (a burst after quiet steps, which
+  // the prediction missed).
   // hvy slot s % R holds the spike words of the blocks that posted at s (zero
   // for the others, whose spikes were pushed), and the pull of a step stamped
   // late stages those words instead of the full history.
@@ -244,13 +245,17 @@ __host__ __device__ inline bool defer_late(uint32_t posted, uint32_t defer_spike
 
 // Launchers (szsnn_kernels.hip).  `j` is the step offset from *t_base.
 // gather: the pull of step t (launches nothing without a pull layout).
-hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s);
+// With events e0 and e1, the kernel's dispatch records them at its start and
+// end (the same for launch_update).
+hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // The heavy pushes posted by step t_base - 1, then both lists emptied (before
 // a persistent launch and at threshold changes; the gather launch of a step
 // pushes its previous step's list).
 hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s);
 // update: block 64, 128, 256, 512 or 1024; npl (ids per lane) 1, 2 or 4.
-hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, uint32_t npl, hipStream_t s);
+hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, uint32_t npl, hipStream_t s,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // One cooperative launch running nsteps steps (block 256 or 512; grid must
 // not exceed persistent_max_blocks_per_cu(model, block) * CUs).
 hipError_t launch_persistent(const DevNet& n, uint64_t nsteps, uint64_t* t_base, uint32_t block, uint32_t grid,

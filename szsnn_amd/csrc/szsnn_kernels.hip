@@ -19,6 +19,8 @@
 // the FP32 CPU restatement of the same equations.
 #include "szsnn_internal.h"
 
+#include <hip/hip_ext.h>
+
 namespace szsnn {
 
 namespace {
@@ -303,13 +305,13 @@ struct UpdateSmem {
   uint32_t nlist;            // spiking ids with synapses this step (list holds the first kSparseIds)
   unsigned long long spikes, drive;
   unsigned long long spk_prev;  // spikes of step t - 1
-  uint64_t due;              // due_mask(t)
 };
 
 // Block-uniform facts of step t known before it starts: the pull's due mask
 // (bit k: step t - k - 1 was deferred) and the spikes of step t - 1.  The
 // persistent loop carries them across steps (the grid barrier returns each
-// step's spike total); a single-step launch reads them with step_facts.
+// step's spike total); in a single-step launch every wave reads the due
+// mask itself (update_step) and step_facts reads the spikes.
 struct StepFacts {
   uint64_t due;
   uint64_t late;  // the due steps deferred late (due_mask)
@@ -318,11 +320,11 @@ struct StepFacts {
 };
 
 // Block-uniform facts of step t, gathered by one wave (the block's first)
-// with all its loads issued together: the pull's due mask (as due_mask)
-// into sm.due, and the spikes of step t - 1 into sm.spk_prev (LDS atomics of
-// the stripes, no wave reduction).  With `fold` (block 0), lane s also reads
-// stripe s of the rows and events due at t (packed) into fold;
-// fold_finish moves them into the counters at the end of the step.
+// with all its loads issued together: the spikes of step t - 1 into
+// sm.spk_prev (LDS atomics of the stripes, no wave reduction).  With `fold`
+// (block 0), lane s also reads stripe s of the rows and events due at t
+// (packed) into fold; fold_finish moves them into the counters at the end of
+// the step.
 template <int BLK, bool AGENT>
 __device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bool do_fold, UpdateSmem<BLK>& sm,
                                   unsigned long long& fold) {
@@ -330,14 +332,9 @@ __device__ inline void step_facts(const DevNet& n, uint64_t t, uint32_t slot, bo
   const uint32_t lane = __lane_id();
   if (lane == 0) sm.spk_prev = 0;
   // unconditional loads at always-valid addresses (no divergent branch between
-  // them, so they are in flight together); validity is applied afterwards
-  const uint32_t k = lane < n.D ? lane : 0u;
-  const uint64_t stamp = n.pull_off ? ld_step<AGENT>(&n.dstamp[(t - k - 1) % n.R]) : 0ull;
+  // them, so they are in flight together)
   const unsigned long long spk = ld_step<AGENT>(spk_counter(n.spk_ring, t - 1, lane));
   fold = do_fold ? ld_step<AGENT>(ring_counter(n.ring_cnt, slot, lane)) : 0ull;
-  const bool stamp_ok = n.pull_off && lane < n.D && t >= (uint64_t)lane + 1 && stamp_step(stamp) == t - lane;
-  const uint64_t due = __ballot(stamp_ok);
-  if (lane == 0) sm.due = due;
   if (t > 0 && spk) atomicAdd(&sm.spk_prev, spk);
 }
 
@@ -426,13 +423,33 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
   const bool folder = first && blockIdx.x == 0;
   const uint32_t lane = __lane_id();
   const uint32_t slot = (uint32_t)(t % n.R);
-  // every load the lane needs, issued before the first barrier
+  // every load the lane needs, issued before the first barrier.  Without
+  // the carried facts, every wave reads the deferral stamps first (the reads
+  // step_facts makes): when the step pulled, the delay groups' sums are
+  // loaded with the lane's other loads.
+  uint64_t pulled = 0;
+  if (!CARRY && n.pull_off) {
+    const uint32_t k = lane < n.D ? lane : 0u;
+    const uint64_t v = ld_step<AGENT>(&n.dstamp[(t - k - 1) % n.R]);
+    pulled = __ballot(lane < n.D && t >= (uint64_t)lane + 1 && stamp_step(v) == t - lane);
+  }
   long long ring_e = 0, ring_i = 0;
   uint64_t fbits = 0;
   long long* ring = n.acc + (size_t)slot * 2 * n.npad;
   if (id < n.N) {
     ring_e = ld_step<AGENT>(&ring[id]);
     ring_i = ld_step<AGENT>(&ring[n.npad + id]);
+  }
+  long long pe[kMaxPullGroups], pi[kMaxPullGroups];
+#pragma unroll
+  for (uint32_t g = 0; g < kMaxPullGroups; ++g) pe[g] = pi[g] = 0;
+  if (!CARRY && pulled && id < n.N) {
+#pragma unroll
+    for (uint32_t g = 0; g < kMaxPullGroups; ++g)
+      if (g < n.pull_groups) {
+        pe[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
+        pi[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
+      }
   }
   if (!CARRY) {
     load_lane<MODEL>(n, id, ls);
@@ -450,7 +467,6 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     if (!CARRY) step_facts<BLK, AGENT>(n, t, slot, folder, sm, fold);
   }
   __syncthreads();
-  const uint64_t due = CARRY ? facts.due : sm.due;
   const unsigned long long spk_prev = CARRY ? facts.spk_prev : sm.spk_prev;
   // step t is deferred when its predicted spikes reach the threshold (the
   // same decision in every block)
@@ -463,17 +479,16 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
     const bool forced = id < n.volley || ((fbits >> (id & 63)) & 1ull);
     if (id < n.N) {
       long long ae = ring_e, ai = ring_i;
-      if (due) {
-        // pulled step: the delay groups' sums (rewritten by every pull), all
-        // loads issued before the sum
-        long long pe[kMaxPullGroups], pi[kMaxPullGroups];
+      if (CARRY ? facts.due != 0 : pulled != 0) {
+        // pulled step: the delay groups' sums (rewritten by every pull); with
+        // carried facts they are loaded here, all before the sum
+        if (CARRY) {
 #pragma unroll
-        for (uint32_t g = 0; g < kMaxPullGroups; ++g) {
-          pe[g] = pi[g] = 0;
-          if (g < n.pull_groups) {
-            pe[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
-            pi[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
-          }
+          for (uint32_t g = 0; g < kMaxPullGroups; ++g)
+            if (g < n.pull_groups) {
+              pe[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + id]);
+              pi[g] = ld_step<AGENT>(&n.acc_pull[(size_t)g * 2 * n.npad + n.npad + id]);
+            }
         }
 #pragma unroll
         for (uint32_t g = 0; g < kMaxPullGroups; ++g) {
@@ -1401,8 +1416,23 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t value, uint64_t count) {
 
 }  // namespace
 
-hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, uint32_t npl, hipStream_t s) {
-#define SZSNN_UPD_NPL(M, B, P) update_kernel<M, B, P><<<(n.T + B * P - 1) / (B * P), B, 0, s>>>(n, j)
+namespace {
+// A launch of kernel(n, j); with events, through hipExtLaunchKernelGGL, which
+// has the kernel's dispatch record e0 at its start and e1 at its end (the
+// kernel's own duration, without the queue's gaps around it).
+template <typename K>
+void launch_timed(K kernel, uint32_t grid, uint32_t block, size_t dyn, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                  const DevNet& n, uint32_t j) {
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (uint32_t)dyn, s, e0, e1, 0u, n, j);
+  else
+    kernel<<<grid, block, dyn, s>>>(n, j);
+}
+}  // namespace
+
+hipError_t launch_update(const DevNet& n, uint32_t j, uint32_t block, uint32_t npl, hipStream_t s, hipEvent_t e0,
+                         hipEvent_t e1) {
+#define SZSNN_UPD_NPL(M, B, P) launch_timed(update_kernel<M, B, P>, (n.T + B * P - 1) / (B * P), B, 0, s, e0, e1, n, j)
 #define SZSNN_UPD_B(M, B)                           \
   switch (npl) {                                    \
     case 1: SZSNN_UPD_NPL(M, B, 1); break;          \
@@ -1474,18 +1504,24 @@ hipError_t launch_heavy(const DevNet& n, uint32_t grid, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s) {
-  if (!n.pull_off) return hipSuccess;  // no pull layout: every step pushes
+hipError_t launch_gather(const DevNet& n, uint32_t j, uint32_t block, uint32_t grid, hipStream_t s, hipEvent_t e0,
+                         hipEvent_t e1) {
+  if (!n.pull_off) {  // no pull layout: every step pushes (the events bracket nothing)
+    hipError_t e = hipSuccess;
+    if (e0) e = hipEventRecord(e0, s);
+    if (e0 && e == hipSuccess) e = hipEventRecord(e1, s);
+    return e;
+  }
   if (grid == 0 || n.pull_stage == 0) return hipErrorInvalidValue;
   const size_t dyn = pull_dyn_bytes(n, n.pull_stage);
   const bool wide = n.pull_batch == 4;
-#define SZSNN_GATHER(B)                                                        \
-  if (n.pull_stage <= 1) {                                                     \
-    if (wide) gather_kernel<B, 4, 5><<<grid, B, dyn, s>>>(n, j);               \
-    else gather_kernel<B, 2, 7><<<grid, B, dyn, s>>>(n, j);                    \
-  } else {                                                                     \
-    if (wide) gather_kernel<B, 4, 4><<<grid, B, dyn, s>>>(n, j);               \
-    else gather_kernel<B, 2, 4><<<grid, B, dyn, s>>>(n, j);                    \
+#define SZSNN_GATHER(B)                                                              \
+  if (n.pull_stage <= 1) {                                                           \
+    if (wide) launch_timed(gather_kernel<B, 4, 5>, grid, B, dyn, s, e0, e1, n, j);   \
+    else launch_timed(gather_kernel<B, 2, 7>, grid, B, dyn, s, e0, e1, n, j);        \
+  } else {                                                                           \
+    if (wide) launch_timed(gather_kernel<B, 4, 4>, grid, B, dyn, s, e0, e1, n, j);   \
+    else launch_timed(gather_kernel<B, 2, 4>, grid, B, dyn, s, e0, e1, n, j);        \
   }
   switch (block) {
     case 128: SZSNN_GATHER(128) break;

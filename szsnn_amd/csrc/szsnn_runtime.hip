@@ -973,16 +973,16 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
   if (!net || !gather_ms || !update_ms) return fail(net ? net->ctx : nullptr, SZSNN_ERR_INVALID, "szsnn_time_kernels: bad argument");
   if (int rc = check_usable(net, "szsnn_time_kernels")) return rc;
   szsnn_ctx* ctx = net->ctx;
-  // three events per step: before gather, between gather and update, after update
-  std::vector<hipEvent_t> ev(3 * n_steps);
+  // four events per step, recorded by the kernels' dispatches: gather start
+  // and end, update start and end
+  std::vector<hipEvent_t> ev(4 * n_steps);
   for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
   int rc = SZSNN_OK;
   for (uint64_t s = 0; s < n_steps && rc == SZSNN_OK; ++s) {
-    hipError_t e = hipEventRecord(ev[3 * s], ctx->stream);
-    if (e == hipSuccess) e = launch_gather(net->dn, 0, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[3 * s + 1], ctx->stream);
-    if (e == hipSuccess) e = launch_update(net->dn, 0, net->tiling.update_block, net->tiling.update_npl, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[3 * s + 2], ctx->stream);
+    hipEvent_t* e4 = &ev[4 * s];
+    hipError_t e = launch_gather(net->dn, 0, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream, e4[0], e4[1]);
+    if (e == hipSuccess)
+      e = launch_update(net->dn, 0, net->tiling.update_block, net->tiling.update_npl, ctx->stream, e4[2], e4[3]);
     if (e == hipSuccess) e = launch_advance(net->t_base, 1, ctx->stream);
     if (e != hipSuccess) rc = fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_time_kernels: ") + hipGetErrorString(e));
     net->t_host += 1;
@@ -990,10 +990,10 @@ int szsnn_time_kernels(szsnn_net* net, uint64_t n_steps, double* gather_ms, doub
   hipError_t e = hipStreamSynchronize(ctx->stream);
   double g = 0, u = 0;
   if (rc == SZSNN_OK && e == hipSuccess) {
-    for (uint64_t s = 0; s < n_steps; ++s) {
+    for (uint64_t s = 0; s < n_steps && e == hipSuccess; ++s) {
       float a = 0, b = 0;
-      (void)hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]);
-      (void)hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]);
+      e = hipEventElapsedTime(&a, ev[4 * s], ev[4 * s + 1]);
+      if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[4 * s + 2], ev[4 * s + 3]);
       g += a;
       u += b;
     }
@@ -1019,12 +1019,10 @@ int szsnn_test_kernel(szsnn_net* net, int32_t kernel, double* ms) {
     (void)hipEventDestroy(e0);
     return fail(ctx, SZSNN_ERR_HIP, std::string("szsnn_test_kernel: ") + hipGetErrorString(e));
   }
-  e = hipEventRecord(e0, ctx->stream);
-  if (e == hipSuccess)
-    e = kernel == SZSNN_KERNEL_GATHER
-            ? launch_gather(net->dn, 0, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream)
-            : launch_update(net->dn, 0, net->tiling.update_block, net->tiling.update_npl, ctx->stream);
-  if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+  // the events are recorded by the kernel's dispatch: its own duration
+  e = kernel == SZSNN_KERNEL_GATHER
+          ? launch_gather(net->dn, 0, net->tiling.gather_block, net->tiling.gather_grid, ctx->stream, e0, e1)
+          : launch_update(net->dn, 0, net->tiling.update_block, net->tiling.update_npl, ctx->stream, e0, e1);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float t = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
