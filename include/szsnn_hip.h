@@ -290,7 +290,9 @@ int szsnn_autotune(szsnn_net* net, uint32_t steps);
  *                     over the rows r with X[r][i] != 0, ascending (the weight
  *                     gradient of input_currents' sign-split currents); needs a
  *                     device workspace of szsnn_adj_input_grad_workspace(rows,
- *                     n_in) bytes (the input spikes as a bit matrix).
+ *                     n_in) bytes (the input spikes as a bit matrix, and a
+ *                     word per tile of 64 inputs x 64 rows saying whether
+ *                     the tile holds nonzero X other than 1).
  *   readout_forward : y_t = alpha y_{t-1} + drive_t over [T][BR]; y0 may be NULL.
  *   readout_backward: lam_t = scale (y_t - target_t) + alpha lam_{t+1}. */
 int szsnn_adj_forward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params, float* state,
@@ -306,9 +308,10 @@ int szsnn_adj_input_grad(void* stream, uint32_t rows, uint32_t n_in, uint32_t H,
 /* szsnn_adj_input_grad without a workspace argument and without allocating:
  * no counterpart in the reference; for callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid
- * it).  The input bit matrix lives in a 16 MiB device array of the library
- * (one per GPU, held until the process ends): n_in * ceil(rows / 64) must not
- * exceed 2^21, else SZSNN_ERR_INVALID with a message in szsnn_last_error(NULL).
+ * it).  The input bit matrix and the tile words live in a 16 MiB device
+ * array of the library (one per GPU, held until the process ends):
+ * (n_in + ceil(n_in / 64)) * ceil(rows / 64) must not exceed 2^21, else
+ * SZSNN_ERR_INVALID with a message in szsnn_last_error(NULL).
  * Every word of the scratch a call uses is written before it is read, so
  * nothing is cleared.
  * Every call on one GPU shares that array: calls that overlap on one GPU

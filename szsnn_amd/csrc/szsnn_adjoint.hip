@@ -234,115 +234,164 @@ __global__ void __launch_bounds__(kInBlock) input_currents_kernel(uint32_t n_in,
 }
 
 // Input spikes as a bit matrix: bit k of XT[i][w] is X[64 w + k][i] != 0
-// (rows x n_in row-major X; XT [n_in][ceil(rows / 64)]).  Thread (i, w)
-// reads its 64 rows of column i: each of its loads is one contiguous 256-byte
-// piece of a row across the wave.  Every word of XT is written.
+// (rows x n_in row-major X; XT [n_in][ceil(rows / 64)]), and one word per
+// tile of 64 inputs x 64 rows, XF[i / 64][w]: nonzero when a nonzero X of the
+// tile is not exactly 1 (so the gradient reads X there, and takes 1
+// elsewhere).  Thread (i, w) reads its 64 rows of column i: each of its loads
+// is one contiguous 256-byte piece of a row across the wave (the wave's 64
+// inputs).  Every word of XT and XF is written.
 __global__ void __launch_bounds__(64) input_bits_kernel(uint32_t rows, uint32_t n_in, uint32_t words,
-                                                        const float* __restrict__ X, uint64_t* __restrict__ XT) {
+                                                        const float* __restrict__ X, uint64_t* __restrict__ XT,
+                                                        uint64_t* __restrict__ XF) {
   const uint32_t i = blockIdx.x * 64 + threadIdx.x;
   const uint32_t w = blockIdx.y;
-  if (i >= n_in) return;  // no barrier in this kernel
   const uint32_t r0 = w * 64;
-  const uint32_t nr = min(64u, rows - r0);
+  const uint32_t nr = i < n_in ? min(64u, rows - r0) : 0u;
   uint64_t m = 0;
-  for (uint32_t k = 0; k < nr; ++k) m |= (uint64_t)(X[(size_t)(r0 + k) * n_in + i] != 0.0f) << k;
-  XT[(size_t)i * words + w] = m;
+  bool other = false;
+  for (uint32_t k = 0; k < nr; ++k) {
+    const float x = X[(size_t)(r0 + k) * n_in + i];
+    m |= (uint64_t)(x != 0.0f) << k;
+    other = other || (x != 0.0f && x != 1.0f);
+  }
+  if (i < n_in) XT[(size_t)i * words + w] = m;
+  const uint64_t tile = __ballot(other);
+  if (threadIdx.x == 0) XF[(size_t)blockIdx.x * words + w] = tile;
+}
+
+// Lane j's value of a 64-bit register (j uniform).
+__device__ inline uint64_t readlane64(uint64_t x, uint32_t j) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)j);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // dW1[i][j] = sum over rows r with X[r][i] != 0, ascending, of X[r][i] *
 // (W1[i][j] >= 0 ? LE[r][j] : LI[r][j]): the gradient of the sign-split
-// input currents (the E current takes the nonnegative weights).  One block
-// per input i and column chunk; each thread keeps kGradCols columns
-// (strided by the block size, so each LE/LI row is read coalesced), and the
-// block takes the rows kGradRows at a time so their loads are in flight
-// together.  Both adjoints of a column are loaded and one is selected in
-// registers: the signs of neighbouring weights are independent, so a branch
-// per column would split every load into two half-filled ones that fetch
-// the same lines.  The rows a spiking input's sum reads (about 2 % of the
-// rows per input) come from the Infinity Cache, where LE and LI stay
-// resident: about 4 GB per call for B = 64, T = 200, 1000 inputs and 2000
-// hidden neurons.
+// input currents (the E current takes the nonnegative weights).  The work is
+// split into chunks of 64 columns, one wave per (input, chunk), lane = column,
+// walking the input's rows (from XT) kGradRows at a time so that their loads
+// are in flight together; both adjoints of a column are loaded and one is
+// selected in registers.  Each input's rows are about 2 % of the rows, and an
+// LE or LI row is read by every input that spiked in it (about 20 for
+// B = 64, T = 200, 1000 inputs), so where the repeats are served from
+// decides the cost.  The grid is chunk-major per XCD (block b runs on XCD
+// b % 8, which takes chunks x, x + 8, ... in turn, each for all inputs): the
+// blocks of an XCD walk one chunk of LE and LI together and its L2 serves
+// part of the repeats (for the sizes above 2.9 GB per call come from beyond
+// the L2, against 4.4 GB with one block per input over all columns).  The
+// per-column sums add the same terms in the same order as one thread per
+// (input, column) walking all its rows would.
 constexpr int kGradBlock = 256;
-constexpr int kGradCols = 8;
+constexpr int kGradWaves = kGradBlock / 64;  // inputs per block
 constexpr int kGradRows = 8;
+constexpr uint32_t kXcds = 8;
 
 __global__ void __launch_bounds__(kGradBlock) input_grad_kernel(uint32_t n_in, uint32_t H, uint32_t words,
-                                                                uint32_t col0, const float* __restrict__ X,
+                                                                uint32_t chunks, const float* __restrict__ X,
                                                                 const float* __restrict__ W1,
                                                                 const float* __restrict__ LE,
                                                                 const float* __restrict__ LI,
                                                                 const uint64_t* __restrict__ XT,
+                                                                const uint64_t* __restrict__ XF,
                                                                 float* __restrict__ dW1) {
-  const uint32_t i = blockIdx.x, tid = threadIdx.x;
-  bool pos[kGradCols];
-  float acc[kGradCols];
-#pragma unroll
-  for (int c = 0; c < kGradCols; ++c) {
-    const uint32_t j = col0 + c * kGradBlock + tid;
-    pos[c] = j < H ? W1[(size_t)i * H + j] >= 0.0f : true;
-    acc[c] = 0.0f;
-  }
+  const uint32_t groups = (n_in + kGradWaves - 1) / kGradWaves;
+  const uint32_t xcd = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  const uint32_t chunk = (k / groups) * kXcds + xcd;
+  if (chunk >= chunks) return;  // no barrier in this kernel
+  const uint32_t i = (k % groups) * kGradWaves + (threadIdx.x >> 6);
+  if (i >= n_in) return;
+  const uint32_t j0 = chunk * 64 + __lane_id();
+  const uint32_t j = min(j0, H - 1);  // past H: a valid column, never stored
+  const bool pos = W1[(size_t)i * H + j] >= 0.0f;
+  float acc = 0.0f;
   const uint64_t* xt = XT + (size_t)i * words;
-  uint32_t w = 0;
-  uint64_t m = words ? xt[0] : 0ull;
+  const uint64_t* xf = XF + (size_t)(i / 64) * words;
+  // The input's bit words are read 64 at a time, one per lane, and walked
+  // from registers (the wave serves one input, so the walk is uniform):
+  // lw/lf the lane's word and tile flag of the round starting at word w0,
+  // nz the round's words not yet taken that hold rows.
+  const uint32_t lane = __lane_id();
+  uint32_t w0 = 0;
+  uint64_t lw = 0, lf = 0, nz = 0;
+  auto load_round = [&]() {
+    const uint32_t wl = w0 + lane;
+    lw = wl < words ? xt[wl] : 0ull;
+    lf = wl < words ? xf[wl] : 0ull;
+    nz = __ballot(lw != 0ull);
+  };
+  load_round();
+  uint64_t m = 0;    // rows left in the current word (bit k: row 64 w + k)
+  uint32_t wc = 0;   // the current word
+  bool fc = false;   // its tile holds values other than 1
   for (;;) {
-    // the next kGradRows rows of input i (r = 0xffffffff: none)
+    // the next kGradRows rows of input i (r = 0xffffffff: none), and whether
+    // each one's X has to be read (else it is 1)
     uint32_t r[kGradRows];
+    bool rd[kGradRows];
 #pragma unroll
     for (int q = 0; q < kGradRows; ++q) {
-      while (!m && w + 1 < words) m = xt[++w];
+      while (!m) {
+        if (!nz) {
+          if (w0 + 64 >= words) break;
+          w0 += 64;
+          load_round();
+          continue;
+        }
+        const uint32_t l = (uint32_t)__ffsll((unsigned long long)nz) - 1;
+        nz &= nz - 1;
+        m = readlane64(lw, l);
+        fc = readlane64(lf, l) != 0ull;
+        wc = w0 + l;
+      }
+      rd[q] = fc;
       if (m) {
-        r[q] = w * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1;
+        r[q] = wc * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1;
         m &= m - 1;
       } else {
         r[q] = 0xffffffffu;
       }
     }
     if (r[0] == 0xffffffffu) break;
-    float x[kGradRows], v[kGradRows][kGradCols];
+    // their loads, all in flight together, then the adds in row order
+    float x[kGradRows], v[kGradRows];
 #pragma unroll
     for (int q = 0; q < kGradRows; ++q) {
       const bool ok = r[q] != 0xffffffffu;
       const uint32_t rr = ok ? r[q] : r[0];
-      x[q] = ok ? X[(size_t)rr * n_in + i] : 0.0f;
-      const float* le = LE + (size_t)rr * H;
-      const float* li = LI + (size_t)rr * H;
-#pragma unroll
-      for (int c = 0; c < kGradCols; ++c) {
-        const uint32_t j = min(col0 + c * kGradBlock + tid, H - 1);  // past H: a valid column, never stored
-        const float e = le[j], in = li[j];
-        v[q][c] = pos[c] ? e : in;
-      }
+      x[q] = !ok ? 0.0f : rd[q] ? X[(size_t)rr * n_in + i] : 1.0f;
+      const float e = LE[(size_t)rr * H + j], in = LI[(size_t)rr * H + j];
+      v[q] = pos ? e : in;
     }
 #pragma unroll
-    for (int q = 0; q < kGradRows; ++q)
-#pragma unroll
-      for (int c = 0; c < kGradCols; ++c) acc[c] += x[q] * v[q][c];
+    for (int q = 0; q < kGradRows; ++q) acc += x[q] * v[q];
   }
-#pragma unroll
-  for (int c = 0; c < kGradCols; ++c) {
-    const uint32_t j = col0 + c * kGradBlock + tid;
-    if (j < H) dW1[(size_t)i * H + j] = acc[c];
-  }
+  if (j0 < H) dW1[(size_t)i * H + j0] = acc;
 }
 
-// Bytes of the input gradient's scratch for `rows` rows of n_in inputs:
-// the input bit matrix.
-size_t input_grad_bytes(uint32_t rows, uint32_t n_in) { return (size_t)n_in * ((rows + 63) / 64) * 8; }
+// Bytes of the input gradient's scratch for `rows` rows of n_in inputs: the
+// input bit matrix and the tiles' flags.
+size_t input_grad_bytes(uint32_t rows, uint32_t n_in) {
+  return ((size_t)n_in + (n_in + 63) / 64) * ((rows + 63) / 64) * 8;
+}
 
 int launch_input_grad(hipStream_t st, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
                       const float* LE, const float* LI, float* dW1, uint64_t* XT) {
   const uint32_t words = (rows + 63) / 64;
-  input_bits_kernel<<<dim3((n_in + 63) / 64, words), 64, 0, st>>>(rows, n_in, words, X, XT);
+  uint64_t* XF = XT + (size_t)n_in * words;
+  input_bits_kernel<<<dim3((n_in + 63) / 64, words), 64, 0, st>>>(rows, n_in, words, X, XT, XF);
   hipError_t e = hipGetLastError();
-  for (uint32_t col0 = 0; e == hipSuccess && col0 < H; col0 += kGradCols * kGradBlock) {
-    input_grad_kernel<<<n_in, kGradBlock, 0, st>>>(n_in, H, words, col0, X, W1, LE, LI, XT, dW1);
-    e = hipGetLastError();
-  }
+  const uint32_t chunks = (H + 63) / 64;
+  const uint32_t groups = (n_in + kGradWaves - 1) / kGradWaves;
+  const uint32_t blocks = kXcds * ((chunks + kXcds - 1) / kXcds) * groups;
+  if (e == hipSuccess)
+    input_grad_kernel<<<blocks, kGradBlock, 0, st>>>(n_in, H, words, chunks, X, W1, LE, LI, XT, XF, dW1);
+  if (e == hipSuccess) e = hipGetLastError();
   return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP;
 }
 
-// Scratch of szsnn_adj_input_grad_noalloc: the input bit matrix, in a
+// Scratch of szsnn_adj_input_grad_noalloc: the input bit matrix and the tile
+// words, in a
 // file-scope device array (each device holds its own copy at its own
 // address, looked up once per device with hipGetSymbolAddress).
 constexpr size_t kScratchWords = (size_t)2 << 20;  // 16 MiB
@@ -404,7 +453,8 @@ int szsnn_adj_input_grad_noalloc(void* stream, uint32_t rows, uint32_t n_in, uin
   if (!rows || !n_in || !H || !X || !W1 || !LE || !LI || !dW1) return SZSNN_ERR_INVALID;
   if (input_grad_bytes(rows, n_in) > kScratchWords * 8) {
     szsnn::set_global_error(
-        "szsnn_adj_input_grad_noalloc: n_in * ceil(rows / 64) exceeds the 2^21 words (16 MiB) of its scratch; "
+        "szsnn_adj_input_grad_noalloc: (n_in + ceil(n_in / 64)) * ceil(rows / 64) exceeds the 2^21 words (16 MiB) "
+        "of its scratch; "
         "use szsnn_adj_input_grad with a workspace");
     return SZSNN_ERR_INVALID;
   }

@@ -193,7 +193,10 @@ def test_input_currents_equal_sign_split_products(cuda, rows, n_in, H):
     from szsnn_amd import _lib
     lib = _lib.load()
     g = torch.Generator().manual_seed(rows * 7 + H)
-    X = (torch.rand(rows, n_in, generator=g) < 0.03).float() * (1.0 + (torch.rand(rows, n_in, generator=g) < 0.2))
+    # values of 2 in the inputs of every other 64-input tile (the kernel reads
+    # X in tiles that hold a value other than 0 or 1 and takes 1 elsewhere)
+    two = (torch.rand(rows, n_in, generator=g) < 0.2) & ((torch.arange(n_in) % 128) < 64)
+    X = (torch.rand(rows, n_in, generator=g) < 0.03).float() * (1.0 + two)
     X[rows // 2] = 0.0
     W = torch.randn(n_in, H, generator=g) * 0.1
     Xd, Wd = X.to(cuda).contiguous(), W.to(cuda).contiguous()
@@ -217,15 +220,19 @@ def test_input_currents_equal_sign_split_products(cuda, rows, n_in, H):
 def test_input_grad_equals_sign_selected_products(cuda, rows, n_in, H):
     """szsnn_adj_input_grad (rows where the input spiked only, ascending)
     against the FP64 products X^T LE and X^T LI selected by the weight's
-    sign; inputs with values other than 0/1, an input that never spikes,
-    row counts not multiples of 64, widths beyond one column pass, zero
-    weights (sign >= 0: the E adjoint)."""
+    sign, and equal to the same FP32 operations in ascending row order;
+    inputs with values other than 0/1, an input that never spikes, row
+    counts not multiples of 64, widths beyond one column chunk, zero weights
+    (sign >= 0: the E adjoint)."""
     import ctypes
     import torch
     from szsnn_amd import _lib
     lib = _lib.load()
     g = torch.Generator().manual_seed(rows * 3 + H)
-    X = (torch.rand(rows, n_in, generator=g) < 0.03).float() * (1.0 + (torch.rand(rows, n_in, generator=g) < 0.2))
+    # values of 2 in the inputs of every other 64-input tile (the kernel reads
+    # X in tiles that hold a value other than 0 or 1 and takes 1 elsewhere)
+    two = (torch.rand(rows, n_in, generator=g) < 0.2) & ((torch.arange(n_in) % 128) < 64)
+    X = (torch.rand(rows, n_in, generator=g) < 0.03).float() * (1.0 + two)
     X[:, n_in // 2] = 0.0
     W = torch.randn(n_in, H, generator=g) * 0.1
     W[0, : min(H, 7)] = 0.0
@@ -233,7 +240,7 @@ def test_input_grad_equals_sign_selected_products(cuda, rows, n_in, H):
     LI = torch.randn(rows, H, generator=g)
     d = {k: v.to(cuda).contiguous() for k, v in (("X", X), ("W", W), ("LE", LE), ("LI", LI))}
     nbytes = lib.szsnn_adj_input_grad_workspace(rows, n_in)
-    assert nbytes == n_in * ((rows + 63) // 64) * 8
+    assert nbytes == (n_in + (n_in + 63) // 64) * ((rows + 63) // 64) * 8  # bits, and a word per 64 x 64 tile
     ws = torch.empty(nbytes // 8, dtype=torch.int64, device=cuda)
     dW = torch.full((n_in, H), float("nan"), device=cuda)
     ptr = lambda t: ctypes.c_void_p(t.data_ptr())
@@ -251,3 +258,15 @@ def test_input_grad_equals_sign_selected_products(cuda, rows, n_in, H):
     err = ((dW.cpu().double() - ref).abs() / scale).max()
     assert float(err) <= 1e-5, float(err)
     assert float(dW[n_in // 2].abs().max()) == 0.0
+    # the same FP32 operations in the same order: per (input, column), the
+    # products x * adjoint of the spiking rows added in ascending row order
+    import numpy as np
+    Xn, Wn, LEn, LIn = X.numpy(), W.numpy(), LE.numpy(), LI.numpy()
+    got = dW.cpu().numpy()
+    for i in range(n_in):
+        rr = np.flatnonzero(Xn[:, i])
+        if rr.size == 0:
+            continue
+        terms = Xn[rr, i][:, None] * np.where(Wn[i] >= 0, LEn[rr], LIn[rr])
+        want = np.add.accumulate(terms, axis=0, dtype=np.float32)[-1]
+        np.testing.assert_array_equal(got[i], want, err_msg=f"input {i}")
