@@ -558,14 +558,21 @@ __device__ inline unsigned long long update_step(const DevNet& n, uint64_t t, ui
       }
       if (c) atomicAdd(&sm.cnt[lane], c);
     } else {
-      uint64_t uni = bits;
+      // lane k keeps the count of delay k from one ballot per delay (a
+      // select, no branch per delay), then one LDS add per lane.  The first
+      // 32 delays unrolled (bits of delays past D are zero: counts of 0)
+      const uint32_t blo = (uint32_t)bits, bhi = (uint32_t)(bits >> 32);
+      uint32_t c = 0;
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) uni |= __shfl_xor(uni, off);
-      for (uint64_t u = uni; u; u &= u - 1) {
-        const uint32_t k = (uint32_t)__ffsll((unsigned long long)u) - 1;
-        const uint64_t m = __ballot((bits >> k) & 1ull);
-        if (lane == 0) atomicAdd(&sm.cnt[k], (uint32_t)__popcll(m));
+      for (uint32_t k = 0; k < 32; ++k) {
+        const uint32_t pk = (uint32_t)__popcll(__ballot((blo >> k) & 1u));
+        c = lane == k ? pk : c;
       }
+      for (uint32_t k = 32; k < n.D; ++k) {
+        const uint32_t pk = (uint32_t)__popcll(__ballot((bhi >> (k - 32)) & 1u));
+        c = lane == k ? pk : c;
+      }
+      if (c) atomicAdd(&sm.cnt[lane], c);
     }
   }
   // spiking ids with synapses, on a pushed step: appended to the block's list

@@ -68,7 +68,19 @@ def _run(ctx, net, policy=None, heavy=None, persistent_after=None):
     return out
 
 
-def _check(got, ref):
+def _rows(net, raster):
+    """Rows delivered within the run: per (step s, delay k) with s + k + 1
+    inside it, the spiking ids of step s that have a synapse of delay k."""
+    from szsnn_amd.simulation import raster_to_bool
+    T = net.n_neurons + net.n_sources
+    has = np.zeros((T, net.max_delay), np.int64)
+    has[net.syn_src.astype(np.int64), net.syn_delay.astype(np.int64)] = 1
+    per = raster_to_bool(raster, T).astype(np.int64) @ has   # [steps, D]
+    steps = per.shape[0]
+    return sum(int(per[: max(steps - 1 - k, 0), k].sum()) for k in range(net.max_delay))
+
+
+def _check(got, ref, net):
     r, v, c = got
     ro, vo, events, drive = ref
     bad = np.flatnonzero((r != ro).any(axis=1))
@@ -76,6 +88,9 @@ def _check(got, ref):
     np.testing.assert_array_equal(v, vo)
     assert c["events"] == events > 0
     assert c["drive_events"] == drive
+    # the update counts rows per delay with a ballot per delay when a wave
+    # has many spikes (the full volley) and per spiking lane when few
+    assert c["rows"] == _rows(net, r)
 
 
 @pytest.mark.parametrize("layout", [0, 1])
@@ -90,7 +105,7 @@ def test_burst_after_quiet_steps_is_deferred_late(ctx, net, oracle_run, layout):
     c = got[2]
     assert c["late_deferred"] >= (SMALL // 2 if layout == 0 else 3), c
     assert c["deferred"] > c["late_deferred"], c
-    _check(got, oracle_run)
+    _check(got, oracle_run, net)
 
 
 def test_persistent_launch_pulls_a_late_deferred_step(ctx, net, oracle_run):
@@ -101,7 +116,7 @@ def test_persistent_launch_pulls_a_late_deferred_step(ctx, net, oracle_run):
     print("case: 10k IF_cond_exp, late deferral, then the persistent mode", flush=True)
     got = _run(ctx, net, persistent_after=23)
     assert got[2]["late_deferred"] >= 1, got[2]
-    _check(got, oracle_run)
+    _check(got, oracle_run, net)
 
 
 def test_late_deferral_equals_push_only(ctx, net, oracle_run):
@@ -110,7 +125,7 @@ def test_late_deferral_equals_push_only(ctx, net, oracle_run):
     print("case: 10k IF_cond_exp, full volley then 100 ids, push only / no heavy pushes", flush=True)
     push = _run(ctx, net, 2.0)
     assert push[2]["deferred"] == 0 and push[2]["late_deferred"] == 0
-    _check(push, oracle_run)
+    _check(push, oracle_run, net)
     noheavy = _run(ctx, net, 0.012, heavy=0)
     assert noheavy[2]["late_deferred"] == 0
-    _check(noheavy, oracle_run)
+    _check(noheavy, oracle_run, net)
