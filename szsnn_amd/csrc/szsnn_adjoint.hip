@@ -36,6 +36,9 @@ void set_global_error(const char* msg);  // szsnn_runtime.hip: the message szsnn
 namespace {
 
 constexpr int kMaxSeg = 64;
+constexpr uint32_t kStepBatch = 8;      // recurrence steps whose loads are issued together
+constexpr uint32_t kReadoutBatch = 16;  // the same for the readouts (fewer threads: more steps in
+                                        // flight) and for the input bit matrix's rows
 
 // params: [9][H] IF_curr_exp parameters per hidden neuron (per-step units)
 struct HidParams {
@@ -73,24 +76,38 @@ __global__ void adj_forward_kernel(uint32_t BH, uint32_t H, uint32_t K, const fl
     si = state[2 * BH + idx];
     fired = state[3 * BH + idx];
   }
-  for (uint32_t t = 0; t < K; ++t) {
-    bool spike = false;
-    if (live) {
-      se = (q.ae * se) + AE[(size_t)t * BH + idx];
-      si = (q.ai * si) + AI[(size_t)t * BH + idx];
-      const float I = (se + si) + q.i_offset;
-      const bool refr = fired <= q.tau_refrac;
-      fired = fired + 1.0f;
-      if (!refr) v = v + ((I * q.rc) + ((q.v_rest - v) * q.rt));
-      spike = !refr && v >= q.v_thresh;
-      if (spike) {
-        v = q.v_reset;
-        fired = 1.0f;
-      }
-      h[(size_t)t * BH + idx] = v - q.v_rest;
+  // kStepBatch steps at a time: their inputs' loads in flight together (the
+  // recurrence itself does not wait on memory)
+  for (uint32_t t0 = 0; t0 < K; t0 += kStepBatch) {
+    float ae[kStepBatch], ai[kStepBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kStepBatch; ++u) {
+      const bool ok = live && t0 + u < K;
+      ae[u] = ok ? AE[(size_t)(t0 + u) * BH + idx] : 0.0f;
+      ai[u] = ok ? AI[(size_t)(t0 + u) * BH + idx] : 0.0f;
     }
-    const uint64_t m = __ballot(spike);
-    if (__lane_id() == 0 && (idx >> 6) < words) bits[(size_t)t * words + (idx >> 6)] = m;
+#pragma unroll
+    for (uint32_t u = 0; u < kStepBatch; ++u) {
+      const uint32_t t = t0 + u;
+      if (t >= K) break;  // uniform
+      bool spike = false;
+      if (live) {
+        se = (q.ae * se) + ae[u];
+        si = (q.ai * si) + ai[u];
+        const float I = (se + si) + q.i_offset;
+        const bool refr = fired <= q.tau_refrac;
+        fired = fired + 1.0f;
+        if (!refr) v = v + ((I * q.rc) + ((q.v_rest - v) * q.rt));
+        spike = !refr && v >= q.v_thresh;
+        if (spike) {
+          v = q.v_reset;
+          fired = 1.0f;
+        }
+        h[(size_t)t * BH + idx] = v - q.v_rest;
+      }
+      const uint64_t m = __ballot(spike);
+      if (__lane_id() == 0 && (idx >> 6) < words) bits[(size_t)t * words + (idx >> 6)] = m;
+    }
   }
   if (live) {
     state[idx] = v;
@@ -113,31 +130,50 @@ __global__ void adj_backward_kernel(uint32_t BH, uint32_t H, uint32_t K, const f
   const uint32_t words = (BH + 63) / 64;
   const uint32_t j = idx % H;
   HidParams q = load_params(params, H, j);
-  // recompute the refractory mask of each step from the checkpoint
+  // recompute the refractory mask of each step from the checkpoint (the
+  // spike words of kStepBatch steps loaded together)
   uint64_t rmask = 0, smask = 0;
   float fired = ckpt[3 * BH + idx];
-  for (uint32_t t = 0; t < K; ++t) {
-    const bool refr = fired <= q.tau_refrac;
-    fired = fired + 1.0f;
-    const bool spike = (bits[(size_t)t * words + (idx >> 6)] >> (idx & 63)) & 1ull;
-    if (spike) fired = 1.0f;
-    rmask |= (uint64_t)refr << t;
-    smask |= (uint64_t)spike << t;
+  for (uint32_t t0 = 0; t0 < K; t0 += kStepBatch) {
+    uint64_t w[kStepBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kStepBatch; ++u)
+      w[u] = t0 + u < K ? bits[(size_t)(t0 + u) * words + (idx >> 6)] : 0ull;
+#pragma unroll
+    for (uint32_t u = 0; u < kStepBatch; ++u) {
+      const uint32_t t = t0 + u;
+      if (t >= K) break;  // uniform
+      const bool refr = fired <= q.tau_refrac;
+      fired = fired + 1.0f;
+      const bool spike = (w[u] >> (idx & 63)) & 1ull;
+      if (spike) fired = 1.0f;
+      rmask |= (uint64_t)refr << t;
+      smask |= (uint64_t)spike << t;
+    }
   }
   float cv = carry[idx], ce = carry[BH + idx], ci = carry[2 * BH + idx];
-  for (int t = (int)K - 1; t >= 0; --t) {
-    const bool refr = (rmask >> t) & 1ull;
-    const bool spike = (smask >> t) & 1ull;
-    const float lv = g[(size_t)t * BH + idx] + cv;
-    const float lvp = spike ? 0.0f : lv;
-    const float dse = refr ? 0.0f : q.rc;
-    const float lse = lvp * dse + ce;
-    const float lsi = lvp * dse + ci;
-    LE[(size_t)t * BH + idx] = lse;
-    LI[(size_t)t * BH + idx] = lsi;
-    cv = lvp * (refr ? 1.0f : 1.0f - q.rt);
-    ce = q.ae * lse;
-    ci = q.ai * lsi;
+  // backwards, kStepBatch steps at a time: their g loads in flight together
+  for (int t1 = (int)K - 1; t1 >= 0; t1 -= (int)kStepBatch) {
+    float gv[kStepBatch];
+#pragma unroll
+    for (int u = 0; u < (int)kStepBatch; ++u) gv[u] = t1 - u >= 0 ? g[(size_t)(t1 - u) * BH + idx] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < (int)kStepBatch; ++u) {
+      const int t = t1 - u;
+      if (t < 0) break;  // uniform
+      const bool refr = (rmask >> t) & 1ull;
+      const bool spike = (smask >> t) & 1ull;
+      const float lv = gv[u] + cv;
+      const float lvp = spike ? 0.0f : lv;
+      const float dse = refr ? 0.0f : q.rc;
+      const float lse = lvp * dse + ce;
+      const float lsi = lvp * dse + ci;
+      LE[(size_t)t * BH + idx] = lse;
+      LI[(size_t)t * BH + idx] = lsi;
+      cv = lvp * (refr ? 1.0f : 1.0f - q.rt);
+      ce = q.ae * lse;
+      ci = q.ai * lsi;
+    }
   }
   carry[idx] = cv;
   carry[BH + idx] = ce;
@@ -150,9 +186,16 @@ __global__ void readout_forward_kernel(uint32_t BR, uint32_t T, float alpha, con
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= BR) return;
   float acc = y0 ? y0[idx] : 0.0f;
-  for (uint32_t t = 0; t < T; ++t) {
-    acc = (alpha * acc) + drive[(size_t)t * BR + idx];
-    y[(size_t)t * BR + idx] = acc;
+  for (uint32_t t0 = 0; t0 < T; t0 += kReadoutBatch) {  // the batch's loads in flight together
+    float d[kReadoutBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kReadoutBatch; ++u) d[u] = t0 + u < T ? drive[(size_t)(t0 + u) * BR + idx] : 0.0f;
+#pragma unroll
+    for (uint32_t u = 0; u < kReadoutBatch; ++u) {
+      if (t0 + u >= T) break;
+      acc = (alpha * acc) + d[u];
+      y[(size_t)(t0 + u) * BR + idx] = acc;
+    }
   }
 }
 
@@ -164,10 +207,20 @@ __global__ void readout_backward_kernel(uint32_t BR, uint32_t T, float alpha, fl
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= BR) return;
   float acc = 0.0f;
-  for (int t = (int)T - 1; t >= 0; --t) {
-    const size_t o = (size_t)t * BR + idx;
-    acc = (scale * (y[o] - target[o])) + (alpha * acc);
-    lam[o] = acc;
+  for (int t1 = (int)T - 1; t1 >= 0; t1 -= (int)kReadoutBatch) {  // the batch's loads in flight together
+    float yv[kReadoutBatch], tv[kReadoutBatch];
+#pragma unroll
+    for (int u = 0; u < (int)kReadoutBatch; ++u) {
+      const bool ok = t1 - u >= 0;
+      yv[u] = ok ? y[(size_t)(t1 - u) * BR + idx] : 0.0f;
+      tv[u] = ok ? target[(size_t)(t1 - u) * BR + idx] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < (int)kReadoutBatch; ++u) {
+      if (t1 - u < 0) break;
+      acc = (scale * (yv[u] - tv[u])) + (alpha * acc);
+      lam[(size_t)(t1 - u) * BR + idx] = acc;
+    }
   }
 }
 
@@ -249,10 +302,15 @@ __global__ void __launch_bounds__(64) input_bits_kernel(uint32_t rows, uint32_t 
   const uint32_t nr = i < n_in ? min(64u, rows - r0) : 0u;
   uint64_t m = 0;
   bool other = false;
-  for (uint32_t k = 0; k < nr; ++k) {
-    const float x = X[(size_t)(r0 + k) * n_in + i];
-    m |= (uint64_t)(x != 0.0f) << k;
-    other = other || (x != 0.0f && x != 1.0f);
+  for (uint32_t k0 = 0; k0 < nr; k0 += kReadoutBatch) {  // the batch's loads in flight together
+    float x[kReadoutBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kReadoutBatch; ++u) x[u] = k0 + u < nr ? X[(size_t)(r0 + k0 + u) * n_in + i] : 0.0f;
+#pragma unroll
+    for (uint32_t u = 0; u < kReadoutBatch; ++u) {
+      m |= (uint64_t)(x[u] != 0.0f) << (k0 + u);
+      other = other || (x[u] != 0.0f && x[u] != 1.0f);
+    }
   }
   if (i < n_in) XT[(size_t)i * words + w] = m;
   const uint64_t tile = __ballot(other);
