@@ -10,7 +10,7 @@ for --settle-steps simulation steps (default 10000: the whole 1000 ms of the
 configuration from its initial state, timed and reported as `full_run`), then
 --warmup untimed chunks, then --steps timed chunks; the run fails if the timed
 window delivers no synaptic event.
-Prints one JSON line (the driver's contract) with:
+Prints one JSON line with:
   value          synaptic events/s over the timed steps (natural dynamics)
   roofline       the pull gather kernel at the volley load (all neurons forced
                  every step: every synapse delivered each step), each launch
@@ -301,7 +301,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
+        print(json.dumps({"error": "szsnn_amd runs on one GPU; --gpus above 1 is not supported (sharding is design only)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
