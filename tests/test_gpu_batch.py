@@ -1,5 +1,6 @@
 """Batched mode on the GPU: every instance of a batch evolves exactly as the
-same network run alone (bit-exact rasters and state), and as the FP32 oracle."""
+same network run alone (bit-exact rasters and state), and as the FP32 oracle;
+a batch with its gather policy switched on equals the default batch."""
 import numpy as np
 import pytest
 
@@ -62,3 +63,33 @@ def test_batched_with_sources_equals_plain(ctx):
         np.testing.assert_array_equal(ir, pr)
         total += int(ir.sum())
     assert total == bs.counters()["spikes"]
+
+
+@pytest.mark.parametrize("policy,heavy", [(0.0, 0), (0.012, 1)])
+def test_batch_with_pulls_and_heavy_pushes_equals_default(ctx, policy, heavy):
+    """A batch with its gather policy switched on (every step pulled; or
+    the plain runs' fraction with every spiking block posting heavy pushes,
+    so steps are deferred, predicted or late) equals the default batch,
+    which pushes every step from its update blocks: the pull's target
+    blocks span instance boundaries."""
+    from szsnn_amd.batch import BatchSimulation, batched_izhikevich
+    nets = batched_izhikevich(seed=12, n_instances=5, n_min=300, n_max=2500, fanout=40)
+    steps = 250
+    print(f"case: batch with policy {policy}, heavy threshold {heavy}", flush=True)
+    runs = []
+    for on in (False, True):
+        bs = BatchSimulation(ctx, nets)
+        if on:
+            assert bs.gather_policy()["has_pull"]
+            bs.set_heavy_push(heavy)
+            bs.set_gather_policy(policy)
+        bs.set_recording(0, steps)
+        bs.run(steps)
+        runs.append((bs.raster(), bs.get_vars(), bs.counters()))
+        bs.close()
+    np.testing.assert_array_equal(runs[1][0], runs[0][0])
+    np.testing.assert_array_equal(runs[1][1], runs[0][1])
+    for k in ("events", "spikes", "rows", "drive_events"):
+        assert runs[1][2][k] == runs[0][2][k], k
+    assert runs[0][2]["deferred"] == 0 and runs[1][2]["deferred"] > 0
+    assert runs[0][2]["spikes"] > 0
