@@ -292,7 +292,8 @@ int szsnn_autotune(szsnn_net* net, uint32_t steps);
  *                     device workspace of szsnn_adj_input_grad_workspace(rows,
  *                     n_in) bytes (the input spikes as a bit matrix, and a
  *                     word per tile of 64 inputs x 64 rows saying whether
- *                     the tile holds nonzero X other than 1).
+ *                     the tile holds nonzero X other than 1); rows at most
+ *                     65535 * 64.
  *   readout_forward : y_t = alpha y_{t-1} + drive_t over [T][BR]; y0 may be NULL.
  *   readout_backward: lam_t = scale (y_t - target_t) + alpha lam_{t+1}. */
 int szsnn_adj_forward_segment(void* stream, uint32_t B, uint32_t H, uint32_t K, const float* params, float* state,

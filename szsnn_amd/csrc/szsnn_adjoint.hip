@@ -26,6 +26,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <string>
 
 #include "../../include/szsnn_hip.h"
 
@@ -462,6 +463,15 @@ hipError_t g_scratch_err[kMaxDevices];
 
 int launch_check(hipError_t e) { return e == hipSuccess ? SZSNN_OK : SZSNN_ERR_HIP; }
 
+// The input bit matrix's kernel takes one grid row per 64 rows: at most
+// 65535 of them.
+constexpr uint32_t kMaxGradRows = 65535u * 64u;
+bool input_grad_rows_ok(uint32_t rows, const char* who) {
+  if (rows <= kMaxGradRows) return true;
+  szsnn::set_global_error((std::string(who) + ": rows exceeds 65535 * 64").c_str());
+  return false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -502,6 +512,7 @@ size_t szsnn_adj_input_grad_workspace(uint32_t rows, uint32_t n_in) { return inp
 int szsnn_adj_input_grad(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X, const float* W1,
                          const float* LE, const float* LI, float* dW1, void* workspace, size_t workspace_bytes) {
   if (!rows || !n_in || !H || !X || !W1 || !LE || !LI || !dW1 || !workspace) return SZSNN_ERR_INVALID;
+  if (!input_grad_rows_ok(rows, "szsnn_adj_input_grad")) return SZSNN_ERR_INVALID;
   if (workspace_bytes < input_grad_bytes(rows, n_in)) return SZSNN_ERR_INVALID;
   return launch_input_grad((hipStream_t)stream, rows, n_in, H, X, W1, LE, LI, dW1, (uint64_t*)workspace);
 }
@@ -509,6 +520,7 @@ int szsnn_adj_input_grad(void* stream, uint32_t rows, uint32_t n_in, uint32_t H,
 int szsnn_adj_input_grad_noalloc(void* stream, uint32_t rows, uint32_t n_in, uint32_t H, const float* X,
                                  const float* W1, const float* LE, const float* LI, float* dW1) {
   if (!rows || !n_in || !H || !X || !W1 || !LE || !LI || !dW1) return SZSNN_ERR_INVALID;
+  if (!input_grad_rows_ok(rows, "szsnn_adj_input_grad_noalloc")) return SZSNN_ERR_INVALID;
   if (input_grad_bytes(rows, n_in) > kScratchWords * 8) {
     szsnn::set_global_error(
         "szsnn_adj_input_grad_noalloc: (n_in + ceil(n_in / 64)) * ceil(rows / 64) exceeds the 2^21 words (16 MiB) "

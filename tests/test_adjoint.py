@@ -270,3 +270,21 @@ def test_input_grad_equals_sign_selected_products(cuda, rows, n_in, H):
         terms = Xn[rr, i][:, None] * np.where(Wn[i] >= 0, LEn[rr], LIn[rr])
         want = np.add.accumulate(terms, axis=0, dtype=np.float32)[-1]
         np.testing.assert_array_equal(got[i], want, err_msg=f"input {i}")
+
+
+@pytest.mark.gpu
+def test_input_grad_rejects_rows_beyond_the_bit_kernel_grid(cuda):
+    """More than 65535 * 64 rows: both entry points return SZSNN_ERR_INVALID
+    with a message, before any launch."""
+    import ctypes
+    import torch
+    from szsnn_amd import _lib
+    lib = _lib.load()
+    rows, n_in, H = 65535 * 64 + 1, 3, 2
+    t = torch.zeros(16, device=cuda)
+    p = ctypes.c_void_p(t.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.szsnn_adj_input_grad(stream, rows, n_in, H, p, p, p, p, p, p, 1 << 40) == -1
+    assert b"65535 * 64" in lib.szsnn_last_error(None)
+    assert lib.szsnn_adj_input_grad_noalloc(stream, rows, n_in, H, p, p, p, p, p) == -1
+    assert b"65535 * 64" in lib.szsnn_last_error(None)
