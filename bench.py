@@ -404,7 +404,8 @@ def profile_pass(args, torch, ctx, net, sim):
     rw = (c1["rows"] - c0["rows"]) / n
     per = gather_alg_bytes(ev, rw)
     print(json.dumps({"profile_pass": True, "cold": bool(args.cold), "gather_block": args.gather_block,
-                      "gather_grid": args.gather_grid, "launches": n, "gather_us": g_ms * 1e3 / n,
+                      "gather_grid": args.gather_grid, "pull_slices": int(sim.info.pull_slices),
+                      "pull_groups": int(sim.info.pull_groups), "launches": n, "gather_us": g_ms * 1e3 / n,
                       "update_us": u_ms * 1e3 / n, "events_per_launch": ev, "rows_per_launch": rw,
                       "alg_bytes_per_launch": per, "alg_gbs": per / (g_ms * 1e-3 / n) / 1e9}), flush=True)
 
@@ -562,6 +563,7 @@ def headline(args, torch, ctx, net, sim):
     # and cold (a cache-evicting sweep before every launch)
     v_warm = volley(default_policy)
     v_warm["pull_groups"] = groups
+    v_warm["pull_slices"] = int(sim.info.pull_slices)
     log(f"[bench] volley pull, warm (policy {default_policy}): {v_warm}")
     v_cold = volley(default_policy, cold=True)
     log(f"[bench] volley pull, cold: {v_cold}")

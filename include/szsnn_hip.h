@@ -98,6 +98,8 @@ typedef struct szsnn_net_info {
   uint64_t device_bytes;      /* device memory held by the network */
   uint64_t raster_words;      /* u64 words per raster row = ceil((N+M)/64) */
   uint64_t pull_entry_bytes;  /* bytes per synapse entry of the pull layout (2, 4, 8; 0: none) */
+  uint64_t pull_slices;       /* slices of 64 targets per pull block (4..8, chosen at upload so the
+                                 gather's work items spread evenly over the CUs; 0: no pull layout) */
 } szsnn_net_info;
 
 /* Counters since upload (or the last szsnn_reset_counters). */

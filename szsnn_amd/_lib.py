@@ -78,6 +78,7 @@ class NetInfo(ctypes.Structure):
         ("device_bytes", ctypes.c_uint64),
         ("raster_words", ctypes.c_uint64),
         ("pull_entry_bytes", ctypes.c_uint64),
+        ("pull_slices", ctypes.c_uint64),
     ]
 
 

@@ -6,12 +6,16 @@
 
 namespace szsnn {
 
-// Pull layout: targets in blocks of kPullTB, each (block, delay) segment cut
-// into slices of 64 targets (one target per lane).
+// Pull layout: targets in blocks of 64 * slices (slices from kPullMinSlices
+// to kPullSlices, chosen per network), each (block, delay) segment cut into
+// slices of 64 targets (one target per lane).  A slice lane's word holds its
+// target within the block in the low kPullShift bits.
 constexpr uint32_t kPullShift = 9;
-constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block
+constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block, at most
+constexpr uint32_t kPullTMask = kPullTB - 1;     // a slice lane's target field
 constexpr uint32_t kPullMaxWords = 6144;         // 48 KiB of staged bitmap (sources spanning 393k ids)
-constexpr uint32_t kPullSlices = kPullTB / 64;   // slices of 64 targets per (block, delay) segment
+constexpr uint32_t kPullSlices = kPullTB / 64;   // slices of 64 targets per (block, delay) segment, at most
+constexpr uint32_t kPullMinSlices = 4;           // at least (no fewer than a 256-thread block's waves)
 constexpr uint32_t kPullMaxWeights = 255;        // distinct nonzero weights of narrow entries
 constexpr uint32_t kMaxPullGroups = 8;           // delay groups of the pull (work items per target block)
 

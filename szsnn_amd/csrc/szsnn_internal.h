@@ -100,16 +100,16 @@ struct DevNet {
   uint64_t* hvy;            // [R][W] or null (no heavy pushes)
   const uint64_t* forced_bits;   // [W] forced ids this step, or null
   uint64_t* hist;           // [R][W] spike bitmaps of the last R steps (slot t % R)
-  // pull layout (dense gather), a sliced ELLPACK: target blocks of kPullTB
-  // neurons; segment (b, k) holds the delay-k synapses into block b in
-  // kPullSlices slices of 64 targets, slice s at entries pull_off[(b*D+k)*
-  // kPullSlices + s] .. (next), lane L serving target pull_lane[(b*D+k)*
-  // kPullTB + s*64 + L] % kPullTB of the block, whose entries fill the first
-  // pull_lane[...] >> kPullShift vectors of its column; 16-byte vectors
-  // [L_s/V][64][V].  Entry formats (pull_fmt): see kPullWide, kPullCount,
+  // pull layout (dense gather), a sliced ELLPACK: target blocks of
+  // pull_tb = 64 * pull_slices neurons; segment (b, k) holds the delay-k
+  // synapses into block b in pull_slices slices of 64 targets, slice s at
+  // entries pull_off[(b*D+k)*pull_slices + s] .. (next), lane L serving
+  // target pull_lane[(b*D+k)*pull_tb + s*64 + L] & kPullTMask of the block,
+  // whose entries fill the first pull_lane[...] >> kPullShift vectors of its
+  // column; 16-byte vectors [L_s/V][64][V].  Entry formats (pull_fmt): see kPullWide, kPullCount,
   // kPullTable; rel counts from source pull_lo[b]*64.
-  const uint32_t* pull_off; // [NB*D*kPullSlices + 1] or null (no pull layout)
-  const uint32_t* pull_lane;  // [NB*D*kPullTB] target | vectors << kPullShift
+  const uint32_t* pull_off; // [NB*D*pull_slices + 1] or null (no pull layout)
+  const uint32_t* pull_lane;  // [NB*D*pull_tb] target | vectors << kPullShift
   const uint2* pull_syn;    // wide entries, or null
   const uint32_t* pull_syn32;  // table-format entries, or null
   const uint16_t* pull_syn16;  // count-format entries, or null
@@ -118,6 +118,8 @@ struct DevNet {
   uint32_t pull_wn;         // count, table: weight-table entries
   const uint32_t* pull_lo;  // [NB] first raster word of the block's sources
   uint32_t pull_nb;         // NB
+  uint32_t pull_slices;     // slices per segment (kPullMinSlices .. kPullSlices)
+  uint32_t pull_tb;         // targets per block, 64 * pull_slices
   uint32_t pull_null;       // the null vector (16 bytes of padding entries after the last slice), in vectors
   uint32_t pull_words;      // LDS words staged per block (max over blocks)
   uint32_t pull_stage;      // gather launches: delays whose bitmaps are staged at once (>= 1)

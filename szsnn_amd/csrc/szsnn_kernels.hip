@@ -755,6 +755,20 @@ __device__ inline void stage_words(uint64_t* dst, const uint64_t* src, uint32_t 
   }
 }
 
+// A wave's column in a pull work item: delay k (from the group's first) and
+// slice s, i.e. column q = k * slices + s of the item's stream; a wave walks
+// q = wave, wave + waves, ... (stepped without a division).
+struct PullCol {
+  uint32_t k, s;
+};
+__device__ inline void col_step(PullCol& c, uint32_t by, uint32_t slices) {
+  c.s += by;
+  while (c.s >= slices) {
+    c.s -= slices;
+    ++c.k;
+  }
+}
+
 // Dense gather (pull) for target block `b` and delay group `g` over the
 // sliced layout.  The group's due delays are taken in passes of up to
 // `stage` delays: a pass stages the spike bitmaps of steps t-k-1 (the
@@ -762,7 +776,7 @@ __device__ inline void stage_words(uint64_t* dst, const uint64_t* src, uint32_t 
 // block barrier, and the waves then walk their columns of every delay of the
 // pass with no barrier in between.  A column is one slice of 64 targets of
 // one delay, one target per lane; the item's columns are numbered q = (k -
-// k0) * kPullSlices + slice and wave w walks q = w, w + waves, ... as one
+// k0) * pull_slices + slice and wave w walks q = w, w + waves, ... as one
 // stream of 16-byte loads in batches of KB vectors per lane (the wave's j-th
 // load is contiguous), held in two buffers that take turns, so one batch is
 // in flight while the other is summed; the batch after a column's last is
@@ -804,20 +818,20 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   const uint32_t nw = min(n.pull_words, n.W - lo);
   const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
   const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
-  const uint32_t qend = (k1 - k0) * kPullSlices;
+  const uint32_t S = n.pull_slices, TB = n.pull_tb, nk = k1 - k0;
   const uint4* vec = FMT == kPullWide ? reinterpret_cast<const uint4*>(n.pull_syn)
                                       : reinterpret_cast<const uint4*>(n.pull_syn32);
   const uint32_t lane = __lane_id();
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
   uint32_t events = 0;
-  // col_from moves q to the wave's next nonempty column of a due delay at or
-  // after it (q >= qend: none) and gives its first entry s0 and its vectors
-  // per lane nv (the slice's longest lane).
-  auto col_from = [&](uint32_t& q, uint32_t& s0, uint32_t& nv) {
-    for (; q < qend; q += kWaves) {
-      const uint32_t k = k0 + q / kPullSlices;
+  // col_from moves column c to the wave's next nonempty column of a due delay
+  // at or after it (c.k >= nk: none) and gives its first entry s0 and its
+  // vectors per lane nv (the slice's longest lane).
+  auto col_from = [&](PullCol& c, uint32_t& s0, uint32_t& nv) {
+    for (; c.k < nk; col_step(c, kWaves, S)) {
+      const uint32_t k = k0 + c.k;
       if (!((due >> k) & 1ull)) continue;
-      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices + q % kPullSlices;
+      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * S + c.s;
       s0 = ld_const(&so[0]);
       const uint32_t s1 = ld_const(&so[1]);
       if (s1 != s0) {
@@ -827,11 +841,11 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
     }
     nv = 0;
   };
-  // The lane's pull_lane word of column q, loaded unconditionally: past the
+  // The lane's pull_lane word of column c, loaded unconditionally: past the
   // end of the stream it reads a valid word and ignores it.
-  auto lane_word = [&](uint32_t q) {
-    const uint32_t qq = min(q, qend - 1);
-    return n.pull_lane[(size_t)(b * n.D + k0 + qq / kPullSlices) * kPullTB + (qq % kPullSlices) * 64 + lane];
+  auto lane_word = [&](PullCol c) {
+    if (c.k >= nk) c = PullCol{nk - 1, S - 1};
+    return n.pull_lane[(size_t)(b * n.D + k0 + c.k) * TB + c.s * 64 + lane];
   };
   // Entry loads run a batch ahead of the sums in two buffers, A and B, that
   // take turns (no register copies, so the compiler waits for the batch
@@ -845,7 +859,9 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   // padding vectors are stored, and beyond the slice the null vector),
   // loaded into A before the next pass is staged.
   const uint4* null_vec = vec + n.pull_null;
-  uint32_t cq = wave, cs0 = 0, cnv = 0;
+  PullCol cq{0, 0};
+  col_step(cq, wave, S);
+  uint32_t cs0 = 0, cnv = 0;
   col_from(cq, cs0, cnv);
   uint32_t cln = lane_word(cq);
   uint4 qa[kB], qb[kB];
@@ -868,14 +884,16 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
       stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring + ((t - k - 1) % n.R) * n.W + lo, nw);
     }
     __syncthreads();
-    while (cq < qend && k0 + cq / kPullSlices < kend) {  // the wave's columns of this pass
-      uint32_t nq = cq + kWaves, ns0 = 0, nnv = 0;
+    while (cq.k < nk && k0 + cq.k < kend) {  // the wave's columns of this pass
+      PullCol nq = cq;
+      col_step(nq, kWaves, S);
+      uint32_t ns0 = 0, nnv = 0;
       col_from(nq, ns0, nnv);
       const uint32_t nln = lane_word(nq);
       // one delay per pass: the bitmap at a fixed LDS address (folded into
       // the reads' offsets)
-      const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq / kPullSlices - kk) * bstride * 2 : bits0;
-      const uint32_t tl = cln & (kPullTB - 1), mine = cln >> kPullShift;  // target, the lane's own vectors
+      const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq.k - kk) * bstride * 2 : bits0;
+      const uint32_t tl = cln & kPullTMask, mine = cln >> kPullShift;  // target, the lane's own vectors
       const uint4* p = vec + cs0 / kV + lane;    // the lane's vector j at p[64 * j]
       const uint4* pn = vec + ns0 / kV + lane;   // the next column's
       const uint32_t nv = cnv;
@@ -934,8 +952,8 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   }
   __syncthreads();
   long long* out = n.acc_pull + (size_t)g * 2 * n.npad;
-  for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
-    const uint32_t tgt = b * kPullTB + i;
+  for (uint32_t i = threadIdx.x; i < TB; i += BLK) {
+    const uint32_t tgt = b * TB + i;
     if (tgt < n.N) {
       st_agent(&out[tgt], (long long)sums[i]);
       st_agent(&out[n.npad + tgt], (long long)sums[kPullTB + i]);  // two's complement: negative sums
@@ -1003,19 +1021,19 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
   const uint32_t nw = min(n.pull_words, n.W - lo);
   const uint32_t k0 = (uint32_t)(((uint64_t)g * n.D) / n.pull_groups);
   const uint32_t k1 = (uint32_t)(((uint64_t)(g + 1) * n.D) / n.pull_groups);
-  const uint32_t qend = (k1 - k0) * kPullSlices;
+  const uint32_t S = n.pull_slices, TB = n.pull_tb, nk = k1 - k0;
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<uint16_t*>(n.pull_syn16), (short)0, (int)((n.pull_null + 1u) * 16u), kRawBufferFlags);
   const uint32_t lane = __lane_id();
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
-  // col_from moves q to the wave's next nonempty column of a due delay at or
-  // after it (q >= qend: none) and gives its first entry s0 and its vectors
-  // per lane nv (the slice's longest lane).
-  auto col_from = [&](uint32_t& q, uint32_t& s0, uint32_t& nv) {
-    for (; q < qend; q += kWaves) {
-      const uint32_t k = k0 + q / kPullSlices;
+  // col_from moves column c to the wave's next nonempty column of a due delay
+  // at or after it (c.k >= nk: none) and gives its first entry s0 and its
+  // vectors per lane nv (the slice's longest lane).
+  auto col_from = [&](PullCol& c, uint32_t& s0, uint32_t& nv) {
+    for (; c.k < nk; col_step(c, kWaves, S)) {
+      const uint32_t k = k0 + c.k;
       if (!((due >> k) & 1ull)) continue;
-      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * kPullSlices + q % kPullSlices;
+      const uint32_t* so = n.pull_off + (size_t)(b * n.D + k) * S + c.s;
       s0 = ld_const(&so[0]);
       const uint32_t s1 = ld_const(&so[1]);
       if (s1 != s0) {
@@ -1025,11 +1043,11 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
     }
     nv = 0;
   };
-  // The lane's pull_lane word of column q, loaded unconditionally: past the
+  // The lane's pull_lane word of column c, loaded unconditionally: past the
   // end of the stream it reads a valid word and ignores it.
-  auto lane_word = [&](uint32_t q) {
-    const uint32_t qq = min(q, qend - 1);
-    return n.pull_lane[(size_t)(b * n.D + k0 + qq / kPullSlices) * kPullTB + (qq % kPullSlices) * 64 + lane];
+  auto lane_word = [&](PullCol c) {
+    if (c.k >= nk) c = PullCol{nk - 1, S - 1};
+    return n.pull_lane[(size_t)(b * n.D + k0 + c.k) * TB + c.s * 64 + lane];
   };
   // byte offset of the lane's vector 0 in the slice that starts at entry s0
   // (slices start at whole vectors of all 64 lanes)
@@ -1041,7 +1059,9 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
   // take turns, as in pull_body; the batch after a column's last is the next
   // column's first (its slice's vectors, before its lane words are known;
   // a slice's stored padding is zero skips).
-  uint32_t cq = wave, cs0 = 0, cnv = 0;
+  PullCol cq{0, 0};
+  col_step(cq, wave, S);
+  uint32_t cs0 = 0, cnv = 0;
   col_from(cq, cs0, cnv);
   uint32_t cln = lane_word(cq);
   vec4u qa[kB], qb[kB];
@@ -1063,9 +1083,9 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
       stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring + ((t - k - 1) % n.R) * n.W + lo, nw);
     }
     __syncthreads();
-    while (cq < qend && k0 + cq / kPullSlices < kend) {  // the wave's columns of this pass
-      const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq / kPullSlices - kk) * bstride * 2 : bits0;
-      const uint32_t tl = cln & (kPullTB - 1), mine = cln >> kPullShift;  // target, the lane's own vectors
+    while (cq.k < nk && k0 + cq.k < kend) {  // the wave's columns of this pass
+      const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq.k - kk) * bstride * 2 : bits0;
+      const uint32_t tl = cln & kPullTMask, mine = cln >> kPullShift;  // target, the lane's own vectors
       const uint32_t vcur = voff_of(cs0);
       const uint32_t nv = cnv;
       // the column's second batch goes out before the next column's facts
@@ -1078,7 +1098,9 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
         }
       };
       load_b(0);
-      uint32_t nq = cq + kWaves, ns0 = 0, nnv = 0;
+      PullCol nq = cq;
+      col_step(nq, kWaves, S);
+      uint32_t ns0 = 0, nnv = 0;
       col_from(nq, ns0, nnv);
       const uint32_t nln = lane_word(nq);
       const uint32_t vnext = voff_of(ns0);
@@ -1151,8 +1173,8 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
 #pragma unroll
   for (uint32_t i = 0; i < kPullCountFields; ++i) wv[i] = i + 1 < n.pull_wn ? ld_const(&n.pull_wtab[i + 1]) : 0;
   uint32_t events = 0;
-  for (uint32_t i = threadIdx.x; i < kPullTB; i += BLK) {
-    const uint32_t tgt = b * kPullTB + i;
+  for (uint32_t i = threadIdx.x; i < TB; i += BLK) {
+    const uint32_t tgt = b * TB + i;
     if (tgt < n.N) {
       long long e = 0, in = 0;
 #pragma unroll

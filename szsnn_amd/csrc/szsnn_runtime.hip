@@ -451,8 +451,19 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   const uint64_t nrows = T * D;
 
   // --- device formats: the push entries with per-id ranges and the pull layout (szsnn_layout.h)
+  // gather slots: resident blocks per CU at the default block size times CUs
+  // (an occupancy query on the context's device)
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SZSNN_ERR_HIP, "szsnn_net_upload: hipSetDevice failed");
+  const uint64_t cus = (uint64_t)std::max(ctx->cu_count, 1);
+  auto gather_slots = [&](uint32_t words, uint32_t fmt, uint32_t wn) {
+    const int per_cu = gather_max_blocks_per_cu(pull_dyn_bytes_for(words, fmt, wn, 1), 256, 1);
+    return (uint64_t)std::max(per_cu, 1) * cus;
+  };
   HostLayout L;
-  if (!build_layout(*d, L)) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: " + L.error);
+  const SliceChooser choose = [&](uint32_t fmt, uint32_t wn, uint32_t words) {
+    return choose_pull_slices(N, D, gather_slots(words, fmt, wn), cus);
+  };
+  if (!build_layout(*d, L, choose)) return fail(ctx, SZSNN_ERR_INVALID, "szsnn_net_upload: " + L.error);
   const uint64_t S = L.row_off[nrows];
   const std::vector<uint32_t>& syn_off = L.syn_off;
   const uint64_t active = L.active;
@@ -500,21 +511,9 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   float *params = nullptr, *vars = nullptr, *noise = nullptr, *src_rate = nullptr, *src_last = nullptr;
   long long *acc = nullptr, *acc_pull = nullptr;
   uint32_t* barrier = nullptr;
-  // delay groups of the pull gather: as many (target block, group) work items
-  // as one round of resident gather blocks (default block size) holds, the
-  // delays split evenly between the groups where that keeps at least half
-  // of them (equal items end together)
-  uint32_t groups = 1;
-  if (!pull_off.empty()) {
-    const int per_cu =
-        gather_max_blocks_per_cu(pull_dyn_bytes_for(L.pull_words, L.pull_fmt, (uint32_t)L.pull_wtab.size(), 1), 256, 1);
-    const uint64_t slots = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(ctx->cu_count, 1);
-    const uint64_t want =
-        std::max<uint64_t>(1, std::min<uint64_t>({slots / std::max<uint64_t>(NB, 1), (uint64_t)D, (uint64_t)kMaxPullGroups}));
-    uint64_t even = want;
-    while (D % even) --even;
-    groups = (uint32_t)(2 * even >= want ? even : want);
-  }
+  // delay groups of the pull gather, for the staged words the layout needs
+  const uint32_t groups =
+      pull_off.empty() ? 1u : pull_groups_for(NB, D, gather_slots(L.pull_words, L.pull_fmt, (uint32_t)L.pull_wtab.size()));
   uint64_t *keys = nullptr, *src_thresh = nullptr, *drive_cdf = nullptr, *dbits = nullptr, *dstamp = nullptr;
   uint32_t* soff = nullptr;
   uint2* dsyn = nullptr;
@@ -659,6 +658,8 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   n.pull_wn = (uint32_t)L.pull_wtab.size();
   n.pull_lo = d_pull_lo;
   n.pull_nb = d_pull_off ? (uint32_t)NB : 0;
+  n.pull_slices = L.pull_slices;
+  n.pull_tb = 64 * L.pull_slices;
   // the null vector follows the last slice (its entries per vector by format)
   n.pull_null = d_pull_off ? pull_off.back() / (L.pull_fmt == kPullWide ? 2u : L.pull_fmt == kPullTable ? 4u : 8u) : 0;
   n.pull_words = pull_words;
@@ -681,6 +682,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   net->info.pull_groups = d_pull_off ? groups : 0;
   net->info.pull_entry_bytes = !d_pull_off ? 0 : L.pull_fmt == kPullWide ? 8 : L.pull_fmt == kPullTable ? 4 : 2;
   net->info.raster_words = n.W;
+  net->info.pull_slices = d_pull_off ? L.pull_slices : 0;
   net->tiling.gather_grid = default_gather_grid(net, net->tiling.gather_block);
   net->stage_max = stage_limit(n);
   n.pull_stage = 1;

@@ -36,9 +36,10 @@ struct Net {
 };
 
 // nweights > 0: weights drawn from that many values (a narrow pull layout
-// is possible); 0: continuous weights (wide entries)
+// is possible); 0: continuous weights (wide entries).  block_gaps: no
+// synapses into every third block of TB targets.
 static Net random_net(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, uint64_t S, bool block_gaps,
-                      uint32_t nweights) {
+                      uint32_t nweights, uint32_t TB) {
   Net n{N, M, D, {}, {}, {}, {}};
   std::uniform_int_distribution<uint32_t> us(0, N + M - 1), ut(0, N - 1), ud(0, D - 1);
   std::uniform_real_distribution<double> uw(-1.0, 1.0);
@@ -47,7 +48,7 @@ static Net random_net(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, 
   std::uniform_int_distribution<uint32_t> ui(0, nweights ? nweights - 1 : 0);
   for (uint64_t s = 0; s < S; ++s) {
     uint32_t t = ut(rng);
-    if (block_gaps && ((t >> kPullShift) % 3 == 1)) continue;  // leave some target blocks empty
+    if (block_gaps && ((t / TB) % 3 == 1)) continue;  // leave some target blocks empty
     n.src.push_back(us(rng));
     n.tgt.push_back(t);
     double x = nweights ? table[ui(rng)] : uw(rng);
@@ -131,8 +132,9 @@ static void push_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
 // pull replay for step t with history bitmaps hist[R][W] and due mask `due`
 // (bit k: step t-k-1 was deferred); G delay groups, each writing its slot of
 // `out` ([G][2][N]); the caller sums the slots.  Follows pull_body's indexing:
-// per (block, delay) segment kPullSlices slices, lane L of slice s serving
-// target pull_lane[seg*kPullTB + s*64 + L] % kPullTB, its vector j at 16-byte
+// per (block, delay) segment SL = pull_slices slices, lane L of slice s
+// serving target pull_lane[seg*TB + s*64 + L] & kPullTMask (TB = 64 * SL
+// targets per block), its vector j at 16-byte
 // index s0/V + 64*j + L, read for j < pull_lane[...] >> kPullShift (the rest
 // of the column must be padding); the spike bit of rel from 32-bit word
 // rel / 32 of the staged bitmap, padding from the zero word after it; FP64
@@ -142,14 +144,17 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
                         std::vector<long long>& out, uint64_t& events) {
   const uint32_t words = L.pull_words;
   std::vector<uint64_t> bits(words);
-  std::vector<double> sums(2 * kPullTB);
+  const uint32_t SL = L.pull_slices, TB = 64 * SL;
+  CHECK(SL >= kPullMinSlices && SL <= kPullSlices, "slices %u", SL);
+  std::vector<double> sums(2 * TB);
   const uint32_t fmt = L.pull_fmt;
   const uint32_t V = fmt == kPullWide ? 2 : fmt == kPullTable ? 4 : 8;
   const uint32_t pad_rel = words * 64;
   const uint16_t skip0 = count_entry(kPullSkipTag, 0, 0);
   const size_t nent = fmt == kPullWide ? L.pull_syn.size() : fmt == kPullTable ? L.pull_syn32.size() : L.pull_syn16.size();
-  CHECK(L.pull_off.size() == (size_t)L.NB * D * kPullSlices + 1, "pull_off size");
-  CHECK(L.pull_lane.size() == (size_t)L.NB * D * kPullTB, "pull_lane size");
+  CHECK(L.NB == (N + TB - 1) / TB, "blocks");
+  CHECK(L.pull_off.size() == (size_t)L.NB * D * SL + 1, "pull_off size");
+  CHECK(L.pull_lane.size() == (size_t)L.NB * D * TB, "pull_lane size");
   // the slices, then one null vector of padding entries (what loads past a
   // lane's own vectors read)
   CHECK(L.pull_off.back() + V == nent, "entries %zu vs offsets %u", nent, L.pull_off.back());
@@ -172,29 +177,29 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
       if (!((due >> k) & 1ull)) continue;
       CHECK(t >= (uint64_t)k + 1, "due delay before the first step");
       const size_t seg = (size_t)b * D + k;
-      const uint32_t* so = &L.pull_off[seg * kPullSlices];
+      const uint32_t* so = &L.pull_off[seg * SL];
       // the slices' lanes serve every target of the block once
-      std::vector<uint8_t> seen(kPullTB, 0);
-      for (uint32_t i = 0; i < kPullTB; ++i) {
-        const uint32_t tl = L.pull_lane[seg * kPullTB + i] & (kPullTB - 1);
-        CHECK(tl < kPullTB && !seen[tl], "perm of segment %zu is not a permutation", seg);
+      std::vector<uint8_t> seen(TB, 0);
+      for (uint32_t i = 0; i < TB; ++i) {
+        const uint32_t tl = L.pull_lane[seg * TB + i] & kPullTMask;
+        CHECK(tl < TB && !seen[tl], "perm of segment %zu is not a permutation", seg);
         seen[tl] = 1;
       }
-      if (so[0] == so[kPullSlices]) continue;
+      if (so[0] == so[SL]) continue;
       const uint64_t hbase = ((t - k - 1) % R) * W + lo;
       for (uint32_t w = 0; w < nw; ++w) {
         CHECK(hbase + w < hist.size(), "hist read");
         bits[w] = hist[hbase + w];
       }
-      for (uint32_t sl = 0; sl < kPullSlices; ++sl) {
+      for (uint32_t sl = 0; sl < SL; ++sl) {
         const uint32_t s0 = so[sl], s1 = so[sl + 1];
         CHECK(s0 <= s1 && s1 <= nent, "slice bounds");
         CHECK(s0 % V == 0 && (s1 - s0) % (64 * V) == 0, "slice not made of whole vectors per lane");
         if (s0 == s1) continue;
         const uint32_t nv = (s1 - s0) / (64 * V);
         for (uint32_t lane = 0; lane < 64; ++lane) {
-          const uint32_t ln = L.pull_lane[seg * kPullTB + sl * 64 + lane];
-          const uint32_t tl = ln & (kPullTB - 1), mine = ln >> kPullShift;
+          const uint32_t ln = L.pull_lane[seg * TB + sl * 64 + lane];
+          const uint32_t tl = ln & kPullTMask, mine = ln >> kPullShift;
           CHECK(mine <= nv, "lane vectors %u beyond the slice's %u", mine, nv);
           // the column beyond the lane's own vectors is padding
           for (uint32_t j = mine; j < nv; ++j)
@@ -249,29 +254,39 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
                 if (w != 0) ++events;
               }
             }
-          if (b * kPullTB + tl >= N) CHECK(ae == 0.0 && ai == 0.0, "sum for a target beyond N");
+          if (b * TB + tl >= N) CHECK(ae == 0.0 && ai == 0.0, "sum for a target beyond N");
           sums[tl] += ae;
-          sums[kPullTB + tl] += ai;
+          sums[TB + tl] += ai;
         }
       }
     }
-    for (uint32_t i = 0; i < kPullTB; ++i) {
-      const uint32_t tgt = b * kPullTB + i;
+    for (uint32_t i = 0; i < TB; ++i) {
+      const uint32_t tgt = b * TB + i;
       if (tgt < N) {
         CHECK((size_t)g * 2 * N + N + tgt < out.size(), "slot write");
         out[(size_t)g * 2 * N + tgt] = (long long)sums[i];
-        out[(size_t)g * 2 * N + N + tgt] = (long long)sums[kPullTB + i];
+        out[(size_t)g * 2 * N + N + tgt] = (long long)sums[TB + i];
       }
     }
   }
 }
 
+// slices: the pull layout's slices per segment, set before the build (odd
+// cases) or by the builder's callback once the format is known (even ones)
 static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D, uint64_t S, bool gaps,
-                       uint32_t nweights, int32_t pull_layout) {
-  Net net = random_net(rng, N, M, D, S, gaps, nweights);
+                       uint32_t nweights, int32_t pull_layout, uint32_t slices) {
+  Net net = random_net(rng, N, M, D, S, gaps, nweights, 64 * slices);
   const szsnn_net_desc d = desc_of(net, pull_layout);
   HostLayout L;
-  CHECK(build_layout(d, L), "build_layout: %s", L.error.c_str());
+  uint32_t asked_fmt = ~0u;
+  if (slices % 2) {
+    L.pull_slices = slices;
+    CHECK(build_layout(d, L), "build_layout: %s", L.error.c_str());
+  } else {
+    CHECK(build_layout(d, L, [&](uint32_t fmt, uint32_t, uint32_t) { asked_fmt = fmt; return slices; }),
+          "build_layout: %s", L.error.c_str());
+  }
+  CHECK(L.pull_slices == slices, "slices %u, asked for %u", L.pull_slices, slices);
   const uint32_t T = N + M, W = (T + 63) / 64, R = D + 1;
   const uint64_t Ss = L.row_off.back();
   CHECK(L.row_off.size() == (size_t)T * D + 1, "row_off size");
@@ -302,6 +317,7 @@ static void check_case(std::mt19937_64& rng, uint32_t N, uint32_t M, uint32_t D,
   const uint32_t fmt_expected = !narrow_expected ? kPullWide
                                 : distinct.size() <= kPullCountFields ? kPullCount : kPullTable;
   CHECK(L.pull_fmt == fmt_expected, "format %u expected %u", L.pull_fmt, fmt_expected);
+  CHECK(slices % 2 || asked_fmt == fmt_expected, "the callback saw format %u, expected %u", asked_fmt, fmt_expected);
   CHECK(L.pull_fmt == kPullWide || (L.pull_wtab.size() == distinct.size() + 1 && L.pull_wtab[0] == 0),
         "weight table");
   // random spike history up to step t - 1
@@ -367,12 +383,57 @@ int main() {
                      {1000, 100, 32, 30000, true}, {5000, 10, 16, 60000, false}, {3000, 500, 1, 20000, true},
                      // sparse columns over wide source ranges: count-format gaps beyond one delta (skips)
                      {600, 60000, 4, 3000, false}, {300, 390000, 2, 2000, false}};
-  for (const C& c : cases) {
-    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 0, 0);   // continuous weights: wide unless few synapses
-    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 0);   // three weights: count format
-    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 4, 0);   // four weights: table format
-    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 200, 0); // 200 weights: table format
-    check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 1);   // wide entries asked for
+  // every case at kPullSlices and, in turn, at each other slice count
+  uint32_t turn = 0;
+  for (const C& c : cases)
+    for (uint32_t sl : {kPullSlices, kPullMinSlices + turn++ % (kPullSlices - kPullMinSlices)}) {
+      check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 0, 0, sl);    // continuous weights: wide unless few synapses
+      check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 0, sl);    // three weights: count format
+      check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 4, 0, sl);    // four weights: table format
+      check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 200, 0, sl);  // 200 weights: table format
+      check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 1, sl);    // wide entries asked for
+    }
+  // the slice chooser against a brute force of its rule: in range, the
+  // delay groups of kPullSlices kept, and no allowed count with fewer
+  // rounds, or as few rounds and fewer busiest-CU columns, or as many and
+  // more slices
+  for (uint64_t N : {1ull, 63ull, 640ull, 10000ull, 100000ull, 250000ull, 1000000ull, 5000000ull})
+    for (uint64_t D : {1ull, 2ull, 7ull, 16ull, 32ull, 64ull})
+      for (uint64_t per_cu : {1ull, 4ull, 7ull, 8ull}) {
+        const uint64_t cus = 256, slots = per_cu * cus;
+        const uint32_t s = choose_pull_slices(N, D, slots, cus);
+        CHECK(s >= kPullMinSlices && s <= kPullSlices, "chose %u", s);
+        auto cost = [&](uint32_t x, uint64_t& rounds, uint64_t& cols) {
+          const uint64_t nb = (N + 64 * x - 1) / (64 * x);
+          const uint32_t g = pull_groups_for(nb, D, slots);
+          rounds = (nb * g + slots - 1) / slots;
+          cols = (nb * g + cus - 1) / cus * x * ((D + g - 1) / g);
+          return g;
+        };
+        uint64_t r8, c8, rs, cs;
+        const uint32_t g8 = cost(kPullSlices, r8, c8);
+        CHECK(cost(s, rs, cs) >= g8, "fewer groups than at kPullSlices");
+        for (uint32_t x = kPullMinSlices; x <= kPullSlices; ++x) {
+          uint64_t r, c;
+          if (cost(x, r, c) < g8) continue;
+          CHECK(!(r < rs || (r == rs && c < cs) || (r == rs && c == cs && x > s)),
+                "N=%llu D=%llu slots=%llu: %u slices beat the chosen %u", (unsigned long long)N,
+                (unsigned long long)D, (unsigned long long)slots, x, s);
+        }
+      }
+  // the 100000-neuron network with 32 delays, 7 resident gather blocks on
+  // each of 256 CUs: 7 slices, 224 blocks in 8 groups, 7 items on every CU
+  CHECK(choose_pull_slices(100000, 32, 7 * 256, 256) == 7, "100k network");
+  CHECK(pull_groups_for((100000 + 447) / 448, 32, 7 * 256) == 8, "100k network groups");
+  // slice counts outside kPullMinSlices .. kPullSlices are refused
+  for (uint32_t sl : {kPullMinSlices - 1, kPullSlices + 1}) {
+    Net n = random_net(rng, 100, 0, 2, 500, false, 3, 64 * kPullSlices);
+    HostLayout L;
+    L.pull_slices = sl;
+    CHECK(!build_layout(desc_of(n), L) && !L.error.empty(), "slices %u accepted", sl);
+    HostLayout L2;
+    CHECK(!build_layout(desc_of(n), L2, [&](uint32_t, uint32_t, uint32_t) { return sl; }) && !L2.error.empty(),
+          "slices %u accepted from the callback", sl);
   }
   // count format at the delta boundaries: one target whose sources sit at
   // word gaps 0 (same word, another bit; then a duplicate), 511 (one delta),
@@ -400,8 +461,8 @@ int main() {
     CHECK(build_layout(d, L), "build: %s", L.error.c_str());
     CHECK(L.pull_fmt == kPullCount && L.pull_lo.size() == 1 && L.pull_lo[0] == 0, "count format expected");
     uint32_t tl = kPullTB, mine = 0, lane = 0, sl = 0;
-    for (uint32_t i = 0; i < kPullTB; ++i)
-      if ((L.pull_lane[i] & (kPullTB - 1)) == 0) {
+    for (uint32_t i = 0; i < 64 * L.pull_slices; ++i)
+      if ((L.pull_lane[i] & kPullTMask) == 0) {
         tl = 0;
         sl = i / 64;
         lane = i % 64;
