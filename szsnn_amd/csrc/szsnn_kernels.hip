@@ -740,10 +740,12 @@ __device__ inline uint32_t comp(const uint4& v, uint32_t c) {
 // load issued before the first store: one round trip per pass (a loop that
 // stored each word before loading the next would wait once per word, and
 // each wait also drains the entry loads already in flight).
+constexpr uint32_t kStageU = 8;  // words per thread and round: 2048 words per round at 256 threads
+
 template <int BLK, bool AGENT>
-__device__ inline void stage_words(uint64_t* dst, const uint64_t* src, uint32_t nw) {
-  constexpr uint32_t kU = 8;  // words per thread and round: 2048 words per round at 256 threads
-  for (uint32_t w0 = threadIdx.x; w0 < nw; w0 += BLK * kU) {
+__device__ inline void stage_words(uint64_t* dst, const uint64_t* src, uint32_t nw, uint32_t from = 0) {
+  constexpr uint32_t kU = kStageU;
+  for (uint32_t w0 = from + threadIdx.x; w0 < nw; w0 += BLK * kU) {
     uint64_t v[kU];
 #pragma unroll
     for (uint32_t u = 0; u < kU; ++u) v[u] = ld_step<AGENT>(&src[min(w0 + u * BLK, nw - 1u)]);  // no branch per load
@@ -754,6 +756,39 @@ __device__ inline void stage_words(uint64_t* dst, const uint64_t* src, uint32_t 
     }
   }
 }
+
+// stage_words split at its first round: the loads into registers, issued
+// while the previous pass is still read, and the stores (with any further
+// rounds) once the block is done with the previous bitmap.  Entry loads
+// issued after load() stay in flight while the wave waits for its words.
+// The thread index is taken through an empty asm statement, so that the
+// word offsets are formed where they are used instead of being hoisted out
+// of the pass loop into registers held across the columns.
+template <int BLK, bool AGENT>
+struct StagedWords {
+  uint64_t v[kStageU];
+  __device__ static inline uint32_t tid() {
+    uint32_t x = threadIdx.x;
+    asm volatile("" : "+v"(x));
+    return x;
+  }
+  // nw >= 1 (a block's source range holds at least one word); no branch, so
+  // the compiler's wait counts stay exact across the loads
+  __device__ inline void load(const uint64_t* src, uint32_t nw) {
+    const uint32_t i = tid();
+#pragma unroll
+    for (uint32_t u = 0; u < kStageU; ++u) v[u] = ld_step<AGENT>(&src[min(i + u * BLK, nw - 1u)]);
+  }
+  __device__ inline void store(uint64_t* dst, const uint64_t* src, uint32_t nw) {
+    const uint32_t i = tid();
+#pragma unroll
+    for (uint32_t u = 0; u < kStageU; ++u) {
+      const uint32_t w = i + u * BLK;
+      if (w < nw) dst[w] = v[u];
+    }
+    if (nw > BLK * kStageU) stage_words<BLK, AGENT>(dst, src, nw, BLK * kStageU);
+  }
+};
 
 // A wave's column in a pull work item: delay k (from the group's first) and
 // slice s, i.e. column q = k * slices + s of the item's stream; a wave walks
@@ -857,7 +892,8 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   // batches, and the batch after a column's last is the next column's first
   // (its slice's vectors, before its lane words are known; the slice's
   // padding vectors are stored, and beyond the slice the null vector),
-  // loaded into A before the next pass is staged.
+  // loaded into A before the next pass is staged; with one delay per pass,
+  // when that column is in the next pass, after the loads of its bitmap.
   const uint4* null_vec = vec + n.pull_null;
   PullCol cq{0, 0};
   col_step(cq, wave, S);
@@ -865,31 +901,52 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   col_from(cq, cs0, cnv);
   uint32_t cln = lane_word(cq);
   uint4 qa[kB], qb[kB];
-  {
-    // the lane's own vectors: these loads follow the lane word's
+  // batch A of the column cq: the lane's own first vectors (these loads
+  // follow the lane word's)
+  auto load_first = [&]() {
     const uint4* p0 = vec + cs0 / kV + lane;
     const uint32_t m0 = cnv ? cln >> kPullShift : 0u;
 #pragma unroll
     for (uint32_t u = 0; u < kB; ++u) qa[u] = *(u < m0 ? p0 + 64 * u : null_vec);
-  }
-  uint32_t kk = k0;
-  for (;;) {  // block-uniform: every wave stages every pass
-    while (kk < k1 && !((due >> kk) & 1ull)) ++kk;
-    if (kk >= k1) break;
-    const uint32_t kend = min(kk + stage, k1);
-    __syncthreads();  // the previous pass's readers are done with the bitmaps
-    for (uint32_t k = kk; k < kend; ++k) {
-      if (!((due >> k) & 1ull)) continue;
-      const uint64_t* ring = ((late >> k) & 1ull) ? n.hvy : n.hist;  // a late-deferred step: its posted spikes
-      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring + ((t - k - 1) % n.R) * n.W + lo, nw);
-    }
+  };
+  load_first();
+  // the spike bitmap of delay k's step and the next due delay from k
+  auto ring_row = [&](uint32_t k) {
+    const uint64_t* ring = ((late >> k) & 1ull) ? n.hvy : n.hist;  // a late-deferred step: its posted spikes
+    return ring + ((t - k - 1) % n.R) * n.W + lo;
+  };
+  auto next_due = [&](uint32_t k) {
+    while (k < k1 && !((due >> k) & 1ull)) ++k;
+    return k;
+  };
+  // One delay per pass (not MULTI): the first pass's bitmap is staged here
+  // and each later one at the end of the pass before it, its loads issued
+  // ahead of the next column's first batch.
+  uint32_t kk = next_due(k0);
+  if (!MULTI && kk < k1) {
     __syncthreads();
+    stage_words<BLK, AGENT>(dyn, ring_row(kk), nw);
+    __syncthreads();
+  }
+  while (kk < k1) {  // block-uniform: every wave stages every pass
+    const uint32_t kend = MULTI ? min(kk + stage, k1) : kk + 1;
+    if (MULTI) {
+      __syncthreads();  // the previous pass's readers are done with the bitmaps
+      for (uint32_t k = kk; k < kend; ++k)
+        if ((due >> k) & 1ull) stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring_row(k), nw);
+      __syncthreads();
+    }
+    const uint32_t kn = next_due(kend);  // the next pass's first delay
+    const bool ahead = !MULTI && kn < k1;
     while (cq.k < nk && k0 + cq.k < kend) {  // the wave's columns of this pass
       PullCol nq = cq;
       col_step(nq, kWaves, S);
       uint32_t ns0 = 0, nnv = 0;
       col_from(nq, ns0, nnv);
       const uint32_t nln = lane_word(nq);
+      // the next column's first batch follows the column's last, unless the
+      // next column is in the next pass, whose bitmap loads go first
+      const uint32_t nnv_here = ahead && k0 + nq.k >= kend ? 0u : nnv;
       // one delay per pass: the bitmap at a fixed LDS address (folded into
       // the reads' offsets)
       const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq.k - kk) * bstride * 2 : bits0;
@@ -934,7 +991,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
 #pragma unroll
         for (uint32_t u = 0; u < kB; ++u) {
           const uint32_t j = j0 + 2 * kB + u;
-          const uint4* a = here ? (j < mine ? p + 64 * j : null_vec) : (u < nnv ? pn + 64 * u : null_vec);
+          const uint4* a = here ? (j < mine ? p + 64 * j : null_vec) : (u < nnv_here ? pn + 64 * u : null_vec);
           qa[u] = *a;
         }
         sum_batch(qb, j0 + kB);
@@ -948,7 +1005,15 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
       cnv = nnv;
       cln = nln;
     }
-    kk = kend;
+    if (ahead) {
+      StagedWords<BLK, AGENT> sw;
+      sw.load(ring_row(kn), nw);
+      load_first();  // the wave's next column, in a later pass (none: the null vector)
+      __syncthreads();  // the block is done with this pass's bitmap
+      sw.store(dyn, ring_row(kn), nw);
+      __syncthreads();
+    }
+    kk = kn;
   }
   __syncthreads();
   long long* out = n.acc_pull + (size_t)g * 2 * n.npad;
@@ -1065,24 +1130,42 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
   col_from(cq, cs0, cnv);
   uint32_t cln = lane_word(cq);
   vec4u qa[kB], qb[kB];
-  {
+  // batch A of the column cq: the lane's own first vectors
+  auto load_first = [&]() {
     const uint32_t v0 = voff_of(cs0);
     const uint32_t m0 = cnv ? cln >> kPullShift : 0u;
 #pragma unroll
     for (uint32_t u = 0; u < kB; ++u) qa[u] = vload(u < m0 ? v0 : kPullOutOfRange, u);
-  }
-  uint32_t kk = k0;
-  for (;;) {  // block-uniform: every wave stages every pass
-    while (kk < k1 && !((due >> kk) & 1ull)) ++kk;
-    if (kk >= k1) break;
-    const uint32_t kend = min(kk + stage, k1);
-    __syncthreads();  // the previous pass's readers are done with the bitmaps
-    for (uint32_t k = kk; k < kend; ++k) {
-      if (!((due >> k) & 1ull)) continue;
-      const uint64_t* ring = ((late >> k) & 1ull) ? n.hvy : n.hist;  // a late-deferred step: its posted spikes
-      stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring + ((t - k - 1) % n.R) * n.W + lo, nw);
-    }
+  };
+  load_first();
+  // the spike bitmap of delay k's step and the next due delay from k
+  auto ring_row = [&](uint32_t k) {
+    const uint64_t* ring = ((late >> k) & 1ull) ? n.hvy : n.hist;  // a late-deferred step: its posted spikes
+    return ring + ((t - k - 1) % n.R) * n.W + lo;
+  };
+  auto next_due = [&](uint32_t k) {
+    while (k < k1 && !((due >> k) & 1ull)) ++k;
+    return k;
+  };
+  // One delay per pass (not MULTI): the first pass's bitmap is staged here
+  // and each later one at the end of the pass before it, its loads issued
+  // ahead of the next column's first batch.
+  uint32_t kk = next_due(k0);
+  if (!MULTI && kk < k1) {
     __syncthreads();
+    stage_words<BLK, AGENT>(dyn, ring_row(kk), nw);
+    __syncthreads();
+  }
+  while (kk < k1) {  // block-uniform: every wave stages every pass
+    const uint32_t kend = MULTI ? min(kk + stage, k1) : kk + 1;
+    if (MULTI) {
+      __syncthreads();  // the previous pass's readers are done with the bitmaps
+      for (uint32_t k = kk; k < kend; ++k)
+        if ((due >> k) & 1ull) stage_words<BLK, AGENT>(dyn + (size_t)(k - kk) * bstride, ring_row(k), nw);
+      __syncthreads();
+    }
+    const uint32_t kn = next_due(kend);  // the next pass's first delay
+    const bool ahead = !MULTI && kn < k1;
     while (cq.k < nk && k0 + cq.k < kend) {  // the wave's columns of this pass
       const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq.k - kk) * bstride * 2 : bits0;
       const uint32_t tl = cln & kPullTMask, mine = cln >> kPullShift;  // target, the lane's own vectors
@@ -1104,6 +1187,9 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
       col_from(nq, ns0, nnv);
       const uint32_t nln = lane_word(nq);
       const uint32_t vnext = voff_of(ns0);
+      // the next column's first batch follows the column's last, unless the
+      // next column is in the next pass, whose bitmap loads go first
+      const uint32_t nnv_here = ahead && k0 + nq.k >= kend ? 0u : nnv;
       uint32_t cnt = 0;                  // kPullCountBits-bit fields per tag
       uint32_t ba = lds_address(bits);   // LDS byte address of the running 32-bit word
       auto sum_vec = [&](const vec4u& q) {
@@ -1151,7 +1237,7 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
         for (uint32_t u = 0; u < kB; ++u) {
           // one load either way: its offsets are selected
           const uint32_t j = j0 + 2 * kB + u;
-          const uint32_t vo = here ? (j < mine ? vcur : kPullOutOfRange) : (u < nnv ? vnext : kPullOutOfRange);
+          const uint32_t vo = here ? (j < mine ? vcur : kPullOutOfRange) : (u < nnv_here ? vnext : kPullOutOfRange);
           qa[u] = vload(vo, here ? j : u);
         }
         sum_batch(qb, j0 + kB);
@@ -1164,7 +1250,15 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
       cnv = nnv;
       cln = nln;
     }
-    kk = kend;
+    if (ahead) {
+      StagedWords<BLK, AGENT> sw;
+      sw.load(ring_row(kn), nw);
+      load_first();  // the wave's next column, in a later pass (none: no vectors, out-of-range loads)
+      __syncthreads();  // the block is done with this pass's bitmap
+      sw.store(dyn, ring_row(kn), nw);
+      __syncthreads();
+    }
+    kk = kn;
   }
   __syncthreads();
   // sums = counts times weights, per target
