@@ -8,9 +8,13 @@ namespace szsnn {
 
 // Pull layout: targets in blocks of 64 * slices (slices from kPullMinSlices
 // to kPullSlices, chosen per network), each (block, delay) segment cut into
-// slices of 64 targets (one target per lane).  A slice lane's word holds its
-// target within the block in the low kPullShift bits.
+// slices of 64 targets (one target per lane).  A slice lane's 16-bit word
+// holds its target within the block in the low kPullShift bits and, above
+// them, how many of its column's 16-byte vectors hold its entries, or
+// kPullLaneAll: all of the slice's (the lane's own and the stored padding
+// below them, which adds nothing).
 constexpr uint32_t kPullShift = 9;
+constexpr uint32_t kPullLaneAll = (1u << (16 - kPullShift)) - 1;
 constexpr uint32_t kPullTB = 1u << kPullShift;   // targets per pull block, at most
 constexpr uint32_t kPullTMask = kPullTB - 1;     // a slice lane's target field
 constexpr uint32_t kPullMaxWords = 6144;         // 48 KiB of staged bitmap (sources spanning 393k ids)

@@ -105,11 +105,12 @@ struct DevNet {
   // synapses into block b in pull_slices slices of 64 targets, slice s at
   // entries pull_off[(b*D+k)*pull_slices + s] .. (next), lane L serving
   // target pull_lane[(b*D+k)*pull_tb + s*64 + L] & kPullTMask of the block,
-  // whose entries fill the first pull_lane[...] >> kPullShift vectors of its
+  // whose entries fill the first pull_lane[...] >> kPullShift (kPullLaneAll:
+  // all the slice's) vectors of its
   // column; 16-byte vectors [L_s/V][64][V].  Entry formats (pull_fmt): see kPullWide, kPullCount,
   // kPullTable; rel counts from source pull_lo[b]*64.
   const uint32_t* pull_off; // [NB*D*pull_slices + 1] or null (no pull layout)
-  const uint32_t* pull_lane;  // [NB*D*pull_tb] target | vectors << kPullShift
+  const uint16_t* pull_lane;  // [NB*D*pull_tb] target | vectors << kPullShift
   const uint2* pull_syn;    // wide entries, or null
   const uint32_t* pull_syn32;  // table-format entries, or null
   const uint16_t* pull_syn16;  // count-format entries, or null

@@ -790,6 +790,12 @@ struct StagedWords {
   }
 };
 
+// The vectors a slice lane reads of its column of nv: its own, or all nv.
+__device__ inline uint32_t lane_vectors(uint32_t lane_word, uint32_t nv) {
+  const uint32_t m = lane_word >> kPullShift;
+  return m == kPullLaneAll ? nv : m;
+}
+
 // A wave's column in a pull work item: delay k (from the group's first) and
 // slice s, i.e. column q = k * slices + s of the item's stream; a wave walks
 // q = wave, wave + waves, ... (stepped without a division).
@@ -880,7 +886,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   // end of the stream it reads a valid word and ignores it.
   auto lane_word = [&](PullCol c) {
     if (c.k >= nk) c = PullCol{nk - 1, S - 1};
-    return n.pull_lane[(size_t)(b * n.D + k0 + c.k) * TB + c.s * 64 + lane];
+    return (uint32_t)n.pull_lane[(size_t)(b * n.D + k0 + c.k) * TB + c.s * 64 + lane];
   };
   // Entry loads run a batch ahead of the sums in two buffers, A and B, that
   // take turns (no register copies, so the compiler waits for the batch
@@ -905,7 +911,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
   // follow the lane word's)
   auto load_first = [&]() {
     const uint4* p0 = vec + cs0 / kV + lane;
-    const uint32_t m0 = cnv ? cln >> kPullShift : 0u;
+    const uint32_t m0 = cnv ? lane_vectors(cln, cnv) : 0u;
 #pragma unroll
     for (uint32_t u = 0; u < kB; ++u) qa[u] = *(u < m0 ? p0 + 64 * u : null_vec);
   };
@@ -950,7 +956,7 @@ __device__ inline void pull_body(const DevNet& n, uint64_t t, uint32_t b, uint32
       // one delay per pass: the bitmap at a fixed LDS address (folded into
       // the reads' offsets)
       const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq.k - kk) * bstride * 2 : bits0;
-      const uint32_t tl = cln & kPullTMask, mine = cln >> kPullShift;  // target, the lane's own vectors
+      const uint32_t tl = cln & kPullTMask, mine = lane_vectors(cln, cnv);  // target, the lane's own vectors
       const uint4* p = vec + cs0 / kV + lane;    // the lane's vector j at p[64 * j]
       const uint4* pn = vec + ns0 / kV + lane;   // the next column's
       const uint32_t nv = cnv;
@@ -1112,7 +1118,7 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
   // end of the stream it reads a valid word and ignores it.
   auto lane_word = [&](PullCol c) {
     if (c.k >= nk) c = PullCol{nk - 1, S - 1};
-    return n.pull_lane[(size_t)(b * n.D + k0 + c.k) * TB + c.s * 64 + lane];
+    return (uint32_t)n.pull_lane[(size_t)(b * n.D + k0 + c.k) * TB + c.s * 64 + lane];
   };
   // byte offset of the lane's vector 0 in the slice that starts at entry s0
   // (slices start at whole vectors of all 64 lanes)
@@ -1133,7 +1139,7 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
   // batch A of the column cq: the lane's own first vectors
   auto load_first = [&]() {
     const uint32_t v0 = voff_of(cs0);
-    const uint32_t m0 = cnv ? cln >> kPullShift : 0u;
+    const uint32_t m0 = cnv ? lane_vectors(cln, cnv) : 0u;
 #pragma unroll
     for (uint32_t u = 0; u < kB; ++u) qa[u] = vload(u < m0 ? v0 : kPullOutOfRange, u);
   };
@@ -1168,7 +1174,7 @@ __device__ inline void pull_count_body(const DevNet& n, uint64_t t, uint32_t b, 
     const bool ahead = !MULTI && kn < k1;
     while (cq.k < nk && k0 + cq.k < kend) {  // the wave's columns of this pass
       const uint32_t* bits = MULTI ? bits0 + (size_t)(k0 + cq.k - kk) * bstride * 2 : bits0;
-      const uint32_t tl = cln & kPullTMask, mine = cln >> kPullShift;  // target, the lane's own vectors
+      const uint32_t tl = cln & kPullTMask, mine = lane_vectors(cln, cnv);  // target, the lane's own vectors
       const uint32_t vcur = voff_of(cs0);
       const uint32_t nv = cnv;
       // the column's second batch goes out before the next column's facts

@@ -58,7 +58,7 @@ struct HostLayout {
   uint32_t pull_slices = kPullSlices; // slices per (block, delay) segment (set before building)
   std::vector<uint32_t> pull_off;     // [NB*D*pull_slices + 1] slice starts (entries), or empty
   std::vector<uint32_t> pull_lo;      // [NB] first staged raster word of each target block
-  std::vector<uint32_t> pull_lane;    // [NB*D][64*pull_slices] slice lane: target (within the block) | vectors << kPullShift
+  std::vector<uint16_t> pull_lane;    // [NB*D][64*pull_slices] slice lane: target (within the block) | vectors << kPullShift
   std::vector<LayoutPair> pull_syn;   // wide entries (8 bytes), or empty
   std::vector<uint32_t> pull_syn32;   // table-format entries (4 bytes), or empty
   std::vector<uint16_t> pull_syn16;   // count-format entries (2 bytes), or empty
@@ -285,7 +285,7 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L, const Sl
   // target order per segment and slice offsets; each slice lane records its
   // target and how many of the slice's vectors hold its entries
   std::vector<uint16_t> perm(TB);
-  std::vector<uint32_t> lanes(nseg * TB);
+  std::vector<uint16_t> lanes(nseg * TB);
   std::vector<uint32_t> off(nseg * SL + 1, 0);
   std::vector<uint32_t> pos_of(nseg * TB);  // (segment, target) -> its lane's first entry
   uint64_t pos = 0;
@@ -297,11 +297,11 @@ inline bool build_rows_and_pull(const szsnn_net_desc& d, HostLayout& L, const Sl
     for (uint32_t sl = 0; sl < SL; ++sl) {
       off[sg * SL + sl] = (uint32_t)pos;
       const uint64_t len = ((uint64_t)c[pm[sl * 64]] + V - 1) / V * V;  // longest lane of the slice
-      if (len / V >= (1u << (32 - kPullShift))) return true;
       for (uint32_t lane = 0; lane < 64; ++lane) {
         const uint32_t tl = pm[sl * 64 + lane];
         pos_of[sg * TB + tl] = (uint32_t)(pos + lane * V);
-        lanes[sg * TB + sl * 64 + lane] = tl | (uint32_t)(((uint64_t)c[tl] + V - 1) / V) << kPullShift;
+        const uint64_t own = ((uint64_t)c[tl] + V - 1) / V;  // vectors holding the lane's entries
+        lanes[sg * TB + sl * 64 + lane] = (uint16_t)(tl | std::min<uint64_t>(own, kPullLaneAll) << kPullShift);
       }
       pos += len * 64;
       if (pos > 0xffffffffull) return true;

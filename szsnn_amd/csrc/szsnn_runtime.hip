@@ -536,7 +536,7 @@ int upload_impl(szsnn_ctx* ctx, const szsnn_net_desc* d, const uint64_t* id_keys
   uint32_t* d_pull_syn32 = nullptr;
   uint16_t* d_pull_syn16 = nullptr;
   int32_t* d_pull_wtab = nullptr;
-  uint32_t* d_pull_lane = nullptr;
+  uint16_t* d_pull_lane = nullptr;
   uint32_t* heavy_cnt = nullptr;
   uint2* heavy_list = nullptr;
   uint64_t* hvy = nullptr;

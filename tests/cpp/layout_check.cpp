@@ -135,10 +135,14 @@ static void push_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
 // per (block, delay) segment SL = pull_slices slices, lane L of slice s
 // serving target pull_lane[seg*TB + s*64 + L] & kPullTMask (TB = 64 * SL
 // targets per block), its vector j at 16-byte
-// index s0/V + 64*j + L, read for j < pull_lane[...] >> kPullShift (the rest
+// index s0/V + 64*j + L, read for j < pull_lane[...] >> kPullShift, or for
+// every j < the slice's vectors when that field is kPullLaneAll (the rest
 // of the column must be padding); the spike bit of rel from 32-bit word
 // rel / 32 of the staged bitmap, padding from the zero word after it; FP64
 // sums as the kernel forms them.
+// slice lanes replayed, and those whose word says kPullLaneAll
+static uint64_t lanes_seen = 0, lanes_all = 0;
+
 static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D, uint64_t t, uint64_t due,
                         const std::vector<uint64_t>& hist, uint32_t R, uint32_t W, uint32_t G,
                         std::vector<long long>& out, uint64_t& events) {
@@ -199,8 +203,11 @@ static void pull_replay(const HostLayout& L, uint32_t N, uint32_t T, uint32_t D,
         const uint32_t nv = (s1 - s0) / (64 * V);
         for (uint32_t lane = 0; lane < 64; ++lane) {
           const uint32_t ln = L.pull_lane[seg * TB + sl * 64 + lane];
-          const uint32_t tl = ln & kPullTMask, mine = ln >> kPullShift;
+          const uint32_t tl = ln & kPullTMask, field = ln >> kPullShift;
+          const uint32_t mine = field == kPullLaneAll ? nv : field;
           CHECK(mine <= nv, "lane vectors %u beyond the slice's %u", mine, nv);
+          ++lanes_seen;
+          lanes_all += field == kPullLaneAll;
           // the column beyond the lane's own vectors is padding
           for (uint32_t j = mine; j < nv; ++j)
             for (uint32_t c = 0; c < V; ++c) {
@@ -382,7 +389,10 @@ int main() {
                      {255, 7, 64, 3000, false}, {256, 0, 2, 2000, true}, {257, 31, 9, 4000, false},
                      {1000, 100, 32, 30000, true}, {5000, 10, 16, 60000, false}, {3000, 500, 1, 20000, true},
                      // sparse columns over wide source ranges: count-format gaps beyond one delta (skips)
-                     {600, 60000, 4, 3000, false}, {300, 390000, 2, 2000, false}};
+                     {600, 60000, 4, 3000, false}, {300, 390000, 2, 2000, false},
+                     // two targets with about 1400 synapses each in one delay: lanes
+                     // of more than kPullLaneAll - 1 vectors
+                     {2, 3000, 1, 3000, false}};
   // every case at kPullSlices and, in turn, at each other slice count
   uint32_t turn = 0;
   for (const C& c : cases)
@@ -393,6 +403,8 @@ int main() {
       check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 200, 0, sl);  // 200 weights: table format
       check_case(rng, c.N, c.M, c.D, c.S, c.gaps, 3, 1, sl);    // wide entries asked for
     }
+  CHECK(lanes_all > 0 && lanes_all < lanes_seen, "lanes marked all: %llu of %llu", (unsigned long long)lanes_all,
+        (unsigned long long)lanes_seen);
   // the slice chooser against a brute force of its rule: in range, the
   // delay groups of kPullSlices kept, and no allowed count with fewer
   // rounds, or as few rounds and fewer busiest-CU columns, or as many and

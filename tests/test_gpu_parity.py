@@ -380,13 +380,16 @@ def test_persistent_mode_pull_volley_equals_graph(ctx):
     assert runs[0][2]["events"] == runs[1][2]["events"] > 0
 
 
-@pytest.mark.parametrize("which", ["count", "count_skips", "table"])
+@pytest.mark.parametrize("which", ["count", "count_skips", "table", "long_lanes"])
 def test_narrow_and_wide_pull_entries_agree(ctx, which):
     """The same network pulled through narrow entries (2-byte count format:
     few weights, sources as running deltas, gaps wider than a delta taken by
     skip entries; or the 4-byte table format, up to 255 weights looked up)
     and through 8-byte entries: rasters, state and event counts identical to
-    pushing every step; the pulled table network also matches the FP32 oracle."""
+    pushing every step; the pulled table network also matches the FP32 oracle.
+    long_lanes: 1850 synapses per target, all of one delay, so every slice
+    lane's column is longer than its 16-bit lane word can count (the lane
+    then reads all the slice's vectors, its padding included)."""
     import dataclasses
     from szsnn_amd import configs
     from szsnn_amd.simulation import Simulation
@@ -395,6 +398,11 @@ def test_narrow_and_wide_pull_entries_agree(ctx, which):
     elif which == "count_skips":
         # 50000 sources, about 8 per (target, delay): gaps beyond 2^14 - 1 are common
         net, steps, volley = configs.if_cond_exp_100k(n_exc=40000, n_inh=10000, k_exc=200, k_inh=50), 40, 12000
+    elif which == "long_lanes":
+        net, steps, volley = configs.if_cond_exp_100k(n_exc=1600, n_inh=400, k_exc=1500, k_inh=350,
+                                                      max_delay_steps=1), 40, 600
+        per_target = np.bincount(net.syn_tgt[net.syn_w != 0], minlength=net.n_neurons)
+        assert per_target.min() > 8 * 127  # 2-byte entries: more than 127 vectors per lane
     else:
         net, steps, volley = configs.izhikevich2003(), 300, 0
         w = net.syn_w.copy()
