@@ -328,20 +328,36 @@ def _mixed_fraction(net, raster, volley):
     return frac, int((pred >= thr).sum())
 
 
-@pytest.mark.parametrize("which", ["izh", "curr", "cond_volley"])
+@pytest.mark.parametrize("which", ["izh", "curr", "cond_volley", "cond_volley_6", "cond_volley_8"])
 def test_pull_gather_equals_push_gather(ctx, which):
     """Dense (pull) and sparse (push) gathers give identical sums, so rasters,
-    state and event counts agree bit for bit; pull-only also matches the oracle."""
+    state and event counts agree bit for bit; pull-only also matches the oracle.
+    The cond_volley cases differ in size, so the upload picks 4, 6 and 8
+    slices of 64 targets per pull block (on an MI355X: 256 CUs, 7 gather
+    blocks each)."""
     from szsnn_amd import configs
-    volley = 0
+    from szsnn_amd.simulation import Simulation
+    volley, slices = 0, None
     if which == "izh":
         net, steps = configs.izhikevich2003(), 300
     elif which == "curr":
         net, steps = configs.if_curr_exp_10k(n_exc=2000, n_inh=500, n_poisson=300, poisson_hz=100.0,
                                              w_poisson=1.5), 400
-    else:
+    elif which == "cond_volley":
         net, steps, volley = configs.if_cond_exp_100k(n_exc=4000, n_inh=1000, k_exc=200, k_inh=50), 60, 1000
-    print(f"case: pull vs push {which}", flush=True)
+        slices = 4
+    elif which == "cond_volley_6":
+        net, steps, volley = configs.if_cond_exp_100k(n_exc=8800, n_inh=2200, k_exc=200, k_inh=50), 60, 2200
+        slices = 6
+    else:
+        net, steps, volley = configs.if_cond_exp_100k(n_exc=12800, n_inh=3200, k_exc=200, k_inh=50), 60, 3200
+        slices = 8
+    s = Simulation(ctx, net)
+    got = int(s.info.pull_slices)
+    s.close()
+    print(f"case: pull vs push {which}, {got} slices per pull block", flush=True)
+    if slices is not None and "MI355" in ctx.name:
+        assert got == slices
     rp, vp, cp = _run_policy(ctx, net, steps, 2.0, volley)   # push only
     rq, vq, cq = _run_policy(ctx, net, steps, 0.0, volley)   # pull only
     frac, n_def = _mixed_fraction(net, rp, volley)
