@@ -241,7 +241,7 @@ constexpr uint32_t kHeavyPushMin = 1024;
 // threshold prices a step's spikes at the spread update pushes' rate (about
 // 2.3e10 synapse events per second); the gather grid pushes a posted list
 // at about half that (on 1e8 synapses, 1000 contiguous ids forced every
-// step: 98.6 us per step pushed, 61.1 us pulled; 250 ids: 43.1 against 60.8).
+// step: 97.7 us per step pushed, 60.1 us pulled; 250 ids: 43.1 against 60.4).
 __host__ __device__ inline bool defer_late(uint32_t posted, uint32_t defer_spikes) {
   return defer_spikes != 0xffffffffu && 2ull * posted >= defer_spikes;
 }
